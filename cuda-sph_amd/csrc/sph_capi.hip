@@ -1,0 +1,466 @@
+// sph_capi.hip -- host side of libsph_amd.so: context planning, workspace layout,
+// RNG seeding and the extern "C" entry points declared in include/sph_amd.h.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+
+#include "sph_internal.h"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int hip_check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return SPH_OK;
+    return fail(SPH_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+bool is_pow2(double v) {
+    int e;
+    double m = frexp(v, &e);
+    return v > 0 && m == 0.5;
+}
+
+int bit_length(uint64_t v) {
+    int b = 0;
+    while (v) { ++b; v >>= 1; }
+    return b;
+}
+
+// xoroshiro128+ stream seeding with the scheme of numba.cuda.random.create_xoroshiro128p_states
+// (used at abstract_sph_strategy.py:27): one SplitMix64 output in both state words for
+// stream 0, each further stream is the previous one advanced 2^64 draws by the published
+// jump polynomial.  Written from the published algorithms (Vigna, SplitMix64;
+// Blackman & Vigna, xoroshiro128+ 2016).
+uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+void xnext(uint64_t* s) {
+    uint64_t a = s[0], b = s[1];
+    b ^= a;
+    s[0] = rotl(a, 55) ^ b ^ (b << 14);
+    s[1] = rotl(b, 36);
+}
+void xjump(uint64_t* s) {
+    const uint64_t J[2] = {0xbeac0467eba5facbull, 0xd86b048b86aa9922ull};
+    uint64_t a = 0, b = 0;
+    for (int i = 0; i < 2; ++i)
+        for (int bit = 0; bit < 64; ++bit) {
+            if (J[i] >> bit & 1ull) { a ^= s[0]; b ^= s[1]; }
+            xnext(s);
+        }
+    s[0] = a;
+    s[1] = b;
+}
+void seed_streams(uint64_t seed, int64_t n, uint64_t* out) {
+    if (n <= 0) return;
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    out[0] = out[1] = z;
+    for (int64_t i = 1; i < n; ++i) {
+        out[2 * i] = out[2 * i - 2];
+        out[2 * i + 1] = out[2 * i - 1];
+        xjump(out + 2 * i);
+    }
+}
+
+size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct Layout {
+    size_t off[24];
+    size_t total;
+};
+
+// Workspace slices, in order.
+enum Slice {
+    S_POSA, S_VELA, S_POSB, S_VELB, S_AUX, S_KEYI, S_VALI, S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_CS,
+    S_HIST, S_DTOT, S_RNG, S_DBGP, S_DBGV, S_DBGF, S_NBR, S_DIAG, S_FLAGS, S_PIPE, S_COUNT
+};
+
+Layout plan_layout(const sph_ctx* c) {
+    const size_t n = (size_t)c->n, nn = n > 0 ? n : 1;
+    size_t sz[S_COUNT];
+    sz[S_POSA] = sz[S_VELA] = sz[S_POSB] = sz[S_VELB] = sz[S_AUX] = 16 * nn;
+    sz[S_KEYI] = sz[S_VALI] = sz[S_KEYS0] = sz[S_VALS0] = sz[S_KEYS1] = sz[S_VALS1] = 4 * nn;
+    sz[S_CS] = 4 * ((size_t)c->sc.n_cells + 1);
+    sz[S_HIST] = 4 * 256 * (size_t)(c->num_tiles > 0 ? c->num_tiles : 1);
+    sz[S_DTOT] = 4 * 256;
+    sz[S_RNG] = c->prm.mode == SPH_MODE_PIPE ? 16 * nn : 16;
+    sz[S_DBGP] = sz[S_DBGV] = sz[S_DBGF] = c->keep_terms ? 16 * nn : 16;
+    sz[S_NBR] = c->prm.formulation == SPH_FORM_VOXEL ? 4 * 33 * nn : 16;
+    sz[S_DIAG] = 8 * 8 * 512;
+    sz[S_FLAGS] = 64;
+    sz[S_PIPE] = sizeof(sph::PipeTable);
+    Layout L;
+    size_t o = 0;
+    for (int i = 0; i < S_COUNT; ++i) {
+        L.off[i] = o;
+        o = align_up(o + sz[i]);
+    }
+    L.total = o;
+    return L;
+}
+
+int plan(sph_ctx* c) {
+    const sph_params_t& P = c->prm;
+    const sph_constants_t& K = c->cst;
+    if (P.particle_count < 0 || P.particle_count > (int64_t(1) << 30))
+        return fail(SPH_E_INVALID, "particle_count %lld outside [0, 2^30]", (long long)P.particle_count);
+    if (!(K.h > 0) || !(K.mass > 0) || !isfinite(K.h) || !isfinite(K.mass))
+        return fail(SPH_E_INVALID, "h and mass must be positive and finite");
+    if (!(P.dt > 0) || !isfinite(P.dt)) return fail(SPH_E_INVALID, "dt must be positive and finite");
+    if (P.mode != SPH_MODE_BOX && P.mode != SPH_MODE_PIPE) return fail(SPH_E_INVALID, "unknown mode %d", P.mode);
+    if (P.formulation != SPH_FORM_NAIVE && P.formulation != SPH_FORM_VOXEL)
+        return fail(SPH_E_INVALID, "unknown formulation %d", P.formulation);
+    c->n = P.particle_count;
+    c->keep_terms = (P.flags & SPH_FLAG_KEEP_TERMS) ? 1 : 0;
+    sph::StepConsts& S = c->sc;
+    memset(&S, 0, sizeof(S));
+    S.n = (int)c->n;
+    S.mode = P.mode;
+    S.formulation = P.formulation;
+    S.ref_keys = P.formulation == SPH_FORM_VOXEL ? 1 : 0;
+    S.pow2 = 1;
+    int64_t cells = 1;
+    for (int d = 0; d < 3; ++d) {
+        const double sp = P.space_size[d], vx = P.voxel_size[d];
+        if (!(sp > 0) || !(vx > 0) || !isfinite(sp) || !isfinite(vx))
+            return fail(SPH_E_INVALID, "space_size and voxel_size must be positive and finite");
+        const double q = sp / vx;
+        if (q > 1 << 20) return fail(SPH_E_INVALID, "space/voxel = %g cells on axis %d is too many", q, d);
+        S.dims[d] = (int)ceil(q);        // voxel_sph_strategy.py:70-73
+        S.tdims[d] = (int)q;             // voxel_sph_strategy.py:109-116
+        if (S.dims[d] < 1) S.dims[d] = 1;
+        S.stencil[d] = (int)ceil(K.h / vx);
+        if (!is_pow2(vx)) S.pow2 = 0;
+        S.dcell[d] = vx;
+        S.cell[d] = (float)vx;
+        S.inv_cell[d] = (float)(1.0 / vx);
+        cells *= S.dims[d];
+        S.space[d] = (float)sp;
+        S.box_hi_target[d] = (float)(sp - 1e-3);
+        S.ext[d] = (float)P.external_force[d];
+    }
+    if (cells > (int64_t(1) << 30)) return fail(SPH_E_INVALID, "voxel grid of %lld cells is too large", (long long)cells);
+    S.n_cells = (int)cells;
+    S.box_lo_target = 1e-3f;
+    S.h = (float)K.h;
+    S.h2 = (float)(K.h * K.h);
+    S.inv_h2 = (float)(1.0 / (K.h * K.h));
+    double maxsp = fmax(P.space_size[0], fmax(P.space_size[1], P.space_size[2]));
+    S.h_search = (float)(K.h * (1.0 + 1e-5) + 4e-7 * maxsp);
+    S.mass = (float)K.mass;
+    S.rho0 = (float)K.rho0;
+    S.k = (float)K.k;
+    S.visc = (float)K.visc;
+    S.damp = (float)K.damp;
+    S.dt = (float)P.dt;
+    S.w_norm = (float)(K.mass * 315.0 / (64.0 * M_PI * K.h * K.h * K.h));
+    S.grad_c = (float)(-45.0 / (M_PI * pow(K.h, 6.0)));
+    S.lap_c = (float)(45.0 / (M_PI * pow(K.h, 6.0)));
+    S.wc9 = (float)(315.0 / (64.0 * M_PI * pow(K.h, 9.0)));
+    // sort plan
+    c->key_bits = S.ref_keys ? 32 : bit_length((uint64_t)(cells - 1));
+    c->passes = (c->key_bits + 7) / 8;
+    {
+        int shift = 0;
+        for (int p = 0; p < c->passes; ++p) {
+            int bits = (c->key_bits - shift + (c->passes - p) - 1) / (c->passes - p);
+            c->pass_shift[p] = shift;
+            c->pass_bits[p] = bits;
+            shift += bits;
+        }
+    }
+    c->num_tiles = sph::radix_num_tiles(c->n);
+    // pipe table
+    memset(&c->pipe_host, 0, sizeof(c->pipe_host));
+    S.pipe_rows = 0;
+    if (P.mode == SPH_MODE_PIPE) {
+        if (P.pipe_rows < 2 || P.pipe_rows > sph::kMaxPipeRows || !P.pipe)
+            return fail(SPH_E_INVALID, "PIPE mode needs a pipe table of 2..%d rows", sph::kMaxPipeRows);
+        S.pipe_rows = P.pipe_rows;
+        double xb = P.pipe[0];
+        for (int r = 0; r < P.pipe_rows; ++r) {
+            for (int k = 0; k < 5; ++k) c->pipe_host.row[r][k] = (float)P.pipe[r * 5 + k];
+            c->pipe_host.xbeg[r] = (float)xb;  // util_kernels.py:38-43, summed in FP64
+            xb += P.pipe[r * 5 + 4];
+        }
+    }
+    if (c->density_block <= 0) c->density_block = 256;
+    if (c->force_block <= 0) c->force_block = 256;
+    c->ws_bytes = plan_layout(c).total;
+    return SPH_OK;
+}
+
+int run_step_stages(sph_ctx* c, hipStream_t s, bool fused) {
+    int rc;
+    if (fused && c->passes > 0) {
+        if ((rc = hip_check(sph::launch_radix_sort_impl(c, true, s), "sort"))) return rc;
+    } else {
+        if ((rc = hip_check(sph::launch_radix_sort_impl(c, false, s), "sort"))) return rc;
+        if ((rc = hip_check(sph::launch_reorder(c, s), "reorder"))) return rc;
+    }
+    if ((rc = hip_check(sph::launch_cell_table(c, s), "cell table"))) return rc;
+    if ((rc = hip_check(sph::launch_density(c, s), "density"))) return rc;
+    if ((rc = hip_check(sph::launch_force(c, s), "force"))) return rc;
+    c->last_stage = SPH_STAGE_FORCE;
+    return SPH_OK;
+}
+
+int ready(const sph_ctx* c, bool need_state) {
+    if (!c) return fail(SPH_E_INVALID, "null context");
+    if (!c->bound) return fail(SPH_E_STATE, "no workspace bound (call sph_bind_workspace)");
+    if (need_state && !c->have_state) return fail(SPH_E_STATE, "no state uploaded (call sph_upload_state)");
+    return SPH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sph_abi_version(void) { return SPH_ABI_VERSION; }
+
+int sph_last_error(char* buf, size_t len) {
+    if (!buf || len == 0) return SPH_E_INVALID;
+    snprintf(buf, len, "%s", g_err);
+    return SPH_OK;
+}
+
+int sph_create(const sph_params_t* params, const sph_constants_t* consts, sph_ctx** out) {
+    if (!params || !consts || !out) return fail(SPH_E_INVALID, "null argument");
+    *out = nullptr;
+    sph_ctx* c = new (std::nothrow) sph_ctx();
+    if (!c) return fail(SPH_E_INVALID, "out of host memory");
+    c->prm = *params;
+    c->cst = *consts;
+    int rc = plan(c);          // copies the pipe table out of params->pipe
+    c->prm.pipe = nullptr;     // the caller's table is not referenced after this call
+    if (rc == SPH_OK && params->mode == SPH_MODE_PIPE) {
+        c->rng_host = new (std::nothrow) uint64_t[2 * (size_t)(c->n > 0 ? c->n : 1)];
+        if (!c->rng_host) rc = fail(SPH_E_INVALID, "out of host memory");
+        else seed_streams(params->seed, c->n, c->rng_host);
+    }
+    if (rc != SPH_OK) {
+        delete[] c->rng_host;
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return SPH_OK;
+}
+
+int sph_destroy(sph_ctx* ctx) {
+    if (!ctx) return SPH_OK;
+    delete[] ctx->rng_host;
+    delete ctx;
+    return SPH_OK;
+}
+
+size_t sph_workspace_bytes(const sph_ctx* ctx) { return ctx ? ctx->ws_bytes : 0; }
+
+int sph_bind_workspace(sph_ctx* c, void* dev, size_t bytes, void* stream) {
+    if (!c || !dev) return fail(SPH_E_INVALID, "null argument");
+    if (bytes < c->ws_bytes) return fail(SPH_E_INVALID, "workspace of %zu bytes, need %zu", bytes, c->ws_bytes);
+    if (((uintptr_t)dev & 255) != 0) return fail(SPH_E_INVALID, "workspace must be 256-byte aligned");
+    hipStream_t s = (hipStream_t)stream;
+    Layout L = plan_layout(c);
+    char* base = (char*)dev;
+    sph::Buffers& b = c->b;
+    b.posA = (float4*)(base + L.off[S_POSA]);
+    b.velA = (float4*)(base + L.off[S_VELA]);
+    b.posB = (float4*)(base + L.off[S_POSB]);
+    b.velB = (float4*)(base + L.off[S_VELB]);
+    b.aux = (float4*)(base + L.off[S_AUX]);
+    b.keyI = (uint32_t*)(base + L.off[S_KEYI]);
+    b.valI = (uint32_t*)(base + L.off[S_VALI]);
+    b.keyS[0] = (uint32_t*)(base + L.off[S_KEYS0]);
+    b.valS[0] = (uint32_t*)(base + L.off[S_VALS0]);
+    b.keyS[1] = (uint32_t*)(base + L.off[S_KEYS1]);
+    b.valS[1] = (uint32_t*)(base + L.off[S_VALS1]);
+    b.cs = (uint32_t*)(base + L.off[S_CS]);
+    b.tile_hist = (uint32_t*)(base + L.off[S_HIST]);
+    b.digit_total = (uint32_t*)(base + L.off[S_DTOT]);
+    b.rng = (uint64_t*)(base + L.off[S_RNG]);
+    b.dbgP = (float4*)(base + L.off[S_DBGP]);
+    b.dbgV = (float4*)(base + L.off[S_DBGV]);
+    b.dbgF = (float4*)(base + L.off[S_DBGF]);
+    b.nbr = (int32_t*)(base + L.off[S_NBR]);
+    b.diag_partials = (double*)(base + L.off[S_DIAG]);
+    b.flags = (uint32_t*)(base + L.off[S_FLAGS]);
+    b.pipe = (sph::PipeTable*)(base + L.off[S_PIPE]);
+    c->ws = dev;
+    c->sorted_keys = b.keyI;
+    c->sorted_vals = b.valI;
+    int rc;
+    if ((rc = hip_check(sph::launch_fill_u32(b.flags, 0u, 16, s), "clear flags"))) return rc;
+    if ((rc = hip_check(hipMemcpyAsync(b.pipe, &c->pipe_host, sizeof(sph::PipeTable), hipMemcpyHostToDevice, s),
+                        "upload pipe table")))
+        return rc;
+    if (c->prm.mode == SPH_MODE_PIPE && c->n > 0)
+        if ((rc = hip_check(hipMemcpyAsync(b.rng, c->rng_host, 16 * (size_t)c->n, hipMemcpyHostToDevice, s),
+                            "upload rng streams")))
+            return rc;
+    if ((rc = hip_check(hipStreamSynchronize(s), "bind"))) return rc;
+    c->bound = true;
+    c->have_state = false;
+    return SPH_OK;
+}
+
+int sph_upload_state(sph_ctx* c, const double* pos, const double* vel, void* stream) {
+    int rc = ready(c, false);
+    if (rc) return rc;
+    if (c->n > 0 && (!pos || !vel)) return fail(SPH_E_INVALID, "null state pointer");
+    if ((rc = hip_check(sph::launch_upload_f64(c, pos, vel, (hipStream_t)stream), "upload"))) return rc;
+    c->have_state = true;
+    c->last_stage = 0;
+    return SPH_OK;
+}
+
+int sph_upload_state_f32(sph_ctx* c, const float* x, const float* y, const float* z, const float* vx, const float* vy,
+                         const float* vz, void* stream) {
+    int rc = ready(c, false);
+    if (rc) return rc;
+    if (c->n > 0 && (!x || !y || !z || !vx || !vy || !vz)) return fail(SPH_E_INVALID, "null state pointer");
+    if ((rc = hip_check(sph::launch_upload_f32(c, x, y, z, vx, vy, vz, (hipStream_t)stream), "upload"))) return rc;
+    c->have_state = true;
+    c->last_stage = 0;
+    return SPH_OK;
+}
+
+int sph_step(sph_ctx* c, int32_t nsteps, void* stream) {
+    int rc = ready(c, true);
+    if (rc) return rc;
+    if (nsteps < 0) return fail(SPH_E_INVALID, "nsteps < 0");
+    if (c->last_stage != 0 && c->last_stage != SPH_STAGE_FORCE)
+        return fail(SPH_E_STATE, "a staged step is in progress (run it to SPH_STAGE_FORCE first)");
+    for (int32_t k = 0; k < nsteps; ++k)
+        if ((rc = run_step_stages(c, (hipStream_t)stream, true))) return rc;
+    return SPH_OK;
+}
+
+int sph_run_stage(sph_ctx* c, int32_t stage, void* stream) {
+    int rc = ready(c, true);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const int expected = c->last_stage == SPH_STAGE_FORCE ? SPH_STAGE_SORT : c->last_stage + 1;
+    if (stage != expected) return fail(SPH_E_STATE, "stage %d out of order (expected %d)", stage, expected);
+    switch (stage) {
+        case SPH_STAGE_SORT: rc = hip_check(sph::launch_radix_sort_impl(c, false, s), "sort"); break;
+        case SPH_STAGE_TABLE: rc = hip_check(sph::launch_cell_table(c, s), "cell table"); break;
+        case SPH_STAGE_REORDER: rc = hip_check(sph::launch_reorder(c, s), "reorder"); break;
+        case SPH_STAGE_DENSITY: rc = hip_check(sph::launch_density(c, s), "density"); break;
+        case SPH_STAGE_FORCE: rc = hip_check(sph::launch_force(c, s), "force"); break;
+        default: return fail(SPH_E_INVALID, "unknown stage %d", stage);
+    }
+    if (rc) return rc;
+    c->last_stage = stage;
+    return SPH_OK;
+}
+
+int sph_download_state(sph_ctx* c, double* pos, double* vel, double* rho, double* force, void* stream) {
+    int rc = ready(c, true);
+    if (rc) return rc;
+    if (force && !c->keep_terms) return fail(SPH_E_STATE, "result force needs SPH_FLAG_KEEP_TERMS");
+    return hip_check(sph::launch_download_f64(c, pos, vel, rho, force, (hipStream_t)stream), "download");
+}
+
+int sph_download_state_f32(sph_ctx* c, float* x, float* y, float* z, float* vx, float* vy, float* vz, float* rho,
+                           void* stream) {
+    int rc = ready(c, true);
+    if (rc) return rc;
+    return hip_check(sph::launch_download_f32(c, x, y, z, vx, vy, vz, rho, (hipStream_t)stream), "download");
+}
+
+int64_t sph_buffer_elems(const sph_ctx* c, int32_t which) {
+    if (!c) return -1;
+    switch (which) {
+        case SPH_BUF_KEYS:
+        case SPH_BUF_PERM:
+        case SPH_BUF_SORTED_KEYS:
+        case SPH_BUF_DENSITY: return c->n;
+        case SPH_BUF_CELL_BEGIN: return c->sc.n_cells;
+        case SPH_BUF_PRESSURE:
+        case SPH_BUF_VISCOSITY:
+        case SPH_BUF_FORCE: return 3 * c->n;
+        case SPH_BUF_RNG: return 2 * c->n;
+        case SPH_BUF_NEIGHBOURS: return 33 * c->n;
+        default: return -1;
+    }
+}
+
+int sph_read_buffer(sph_ctx* c, int32_t which, void* out, void* stream) {
+    int rc = ready(c, true);
+    if (rc) return rc;
+    if (!out) return fail(SPH_E_INVALID, "null output");
+    hipStream_t s = (hipStream_t)stream;
+    switch (which) {
+        case SPH_BUF_PRESSURE:
+        case SPH_BUF_VISCOSITY:
+        case SPH_BUF_FORCE:
+            if (!c->keep_terms) return fail(SPH_E_STATE, "terms need SPH_FLAG_KEEP_TERMS");
+            if (c->last_stage != SPH_STAGE_FORCE) return fail(SPH_E_STATE, "terms exist after the force stage");
+            break;
+        case SPH_BUF_DENSITY:
+            if (c->last_stage < SPH_STAGE_DENSITY) return fail(SPH_E_STATE, "density exists after the density stage");
+            break;
+        case SPH_BUF_NEIGHBOURS:
+            if (c->prm.formulation != SPH_FORM_VOXEL) return fail(SPH_E_STATE, "neighbour lists exist in VOXEL only");
+            if (c->last_stage < SPH_STAGE_DENSITY) return fail(SPH_E_STATE, "lists exist after the density stage");
+            break;
+        case SPH_BUF_PERM:
+        case SPH_BUF_SORTED_KEYS:
+            if (c->last_stage < SPH_STAGE_SORT) return fail(SPH_E_STATE, "sorted order exists after the sort stage");
+            break;
+        case SPH_BUF_CELL_BEGIN:
+            if (c->last_stage < SPH_STAGE_TABLE) return fail(SPH_E_STATE, "cell table exists after the table stage");
+            break;
+        case SPH_BUF_KEYS:
+            break;
+        case SPH_BUF_RNG:
+            if (c->prm.mode != SPH_MODE_PIPE) return fail(SPH_E_STATE, "RNG streams exist in PIPE mode only");
+            return hip_check(hipMemcpyAsync(out, c->b.rng, 16 * (size_t)c->n, hipMemcpyDeviceToDevice, s), "read rng");
+        default:
+            return fail(SPH_E_INVALID, "unknown buffer %d", which);
+    }
+    return hip_check(sph::launch_read_buffer(c, which, out, s), "read buffer");
+}
+
+int sph_diagnostics(sph_ctx* c, double* out8, void* stream) {
+    int rc = ready(c, true);
+    if (rc) return rc;
+    if (!out8) return fail(SPH_E_INVALID, "null output");
+    return hip_check(sph::launch_diagnostics(c, out8, (hipStream_t)stream), "diagnostics");
+}
+
+int sph_grid_info(const sph_ctx* c, int32_t* dims3, int32_t* stencil3, int32_t* key_bits, int32_t* passes) {
+    if (!c) return fail(SPH_E_INVALID, "null context");
+    for (int d = 0; d < 3; ++d) {
+        if (dims3) dims3[d] = c->sc.dims[d];
+        if (stencil3) stencil3[d] = c->sc.stencil[d];
+    }
+    if (key_bits) *key_bits = c->key_bits;
+    if (passes) *passes = c->passes;
+    return SPH_OK;
+}
+
+int sph_set_launch(sph_ctx* c, int32_t density_block, int32_t force_block) {
+    if (!c) return fail(SPH_E_INVALID, "null context");
+    for (int32_t b : {density_block, force_block})
+        if (b != 64 && b != 128 && b != 256) return fail(SPH_E_INVALID, "block size must be 64, 128 or 256");
+    c->density_block = density_block;
+    c->force_block = force_block;
+    return SPH_OK;
+}
+
+}  // extern "C"

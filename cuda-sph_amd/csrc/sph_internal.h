@@ -1,0 +1,116 @@
+// sph_internal.h -- types shared by the gfx950 kernels and the host side of libsph_amd.so.
+//
+// Device layout (all arrays in one caller-owned workspace, 256-byte aligned slices):
+//   posA/velA  float4[N]  state in cell order of the current step: (x, y, z, rho_prev),
+//                         (vx, vy, vz, particle id as int bits)
+//   posB/velB  float4[N]  state after integration, still in the previous cell order;
+//                         this is the resident state between steps
+//   aux        float4[N]  (rho, p, 1/rho, 0) in cell order, written by the density pass
+//   keyI/valI  u32[N]     sort input in particle-id order: voxel key, slot in posB
+//   keyS/valS  u32[2][N]  radix ping-pong buffers
+//   cs         u32[C+1]   cell start table: cs[c] = first cell-sorted index with key >= c
+//   rng        u64[2N]    xoroshiro128+ stream per particle id (PIPE mode)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sph_amd.h"
+
+namespace sph {
+
+constexpr int kMaxPipeRows = 64;
+constexpr int kMaxNeighbours = 32;  // MAX_NEIGHBOURS, config.py:30
+constexpr int kXcds = 8;            // MI355X: 8 accelerator complex dies, one L2 each
+
+// Everything a step kernel needs, passed by value (fits the kernel-argument segment).
+struct StepConsts {
+    int n;
+    int mode;         // SPH_MODE_*
+    int formulation;  // SPH_FORM_*
+    int ref_keys;     // 1: reference key semantics (trunc, no clamp, signed); 0: clamped floor
+    int pow2;         // 1: cell edges are powers of two, x * inv_cell is exact in FP32
+    int dims[3];      // voxels per axis for keys: ceil(space / voxel) (voxel_sph_strategy.py:70-73)
+    int tdims[3];     // voxels per axis for lookups: trunc(space / voxel) (voxel_sph_strategy.py:109-116)
+    int stencil[3];   // cells searched on each side: ceil(h / voxel)
+    int n_cells;      // dims[0] * dims[1] * dims[2]
+    float inv_cell[3];
+    float cell[3];
+    double dcell[3];  // voxel edges as given (used when pow2 == 0)
+    float h, h2, inv_h2;
+    float h_search;   // h plus a rounding margin, for choosing cells (the pair test uses h2)
+    float mass, rho0, k, visc, damp, dt;
+    float w_norm;     // mass * 315 / (64 pi h^3): rho = w_norm * sum (1 - r^2/h^2)^3
+    float grad_c;     // -45 / (pi h^6)           (config.py:28)
+    float lap_c;      // 45 / (pi h^6)            (config.py:29)
+    float wc9;        // 315 / (64 pi h^9)        (config.py:27), VOXEL density
+    float ext[3];
+    float space[3];
+    float box_lo_target;     // 1e-3                  (base_kernels.py:91)
+    float box_hi_target[3];  // space - 1e-3          (base_kernels.py:94)
+    int pipe_rows;
+};
+
+struct PipeTable {
+    float row[kMaxPipeRows][5];   // Pipe.to_numpy() rows
+    float xbeg[kMaxPipeRows];     // calc_x_at_segment_beginning of each row (util_kernels.py:38-43)
+};
+
+struct Buffers {
+    float4 *posA, *velA, *posB, *velB, *aux;
+    uint32_t *keyI, *valI;
+    uint32_t *keyS[2], *valS[2];
+    uint32_t* cs;
+    uint32_t* tile_hist;   // radix digit counts per tile, digit-major
+    uint32_t* digit_total;
+    uint64_t* rng;
+    float4 *dbgP, *dbgV, *dbgF;
+    int32_t* nbr;          // VOXEL formulation neighbour lists, N x (1 + 32)
+    double* diag_partials;
+    uint32_t* flags;       // [0] count of non-finite positions seen by the hash
+    PipeTable* pipe;
+};
+
+}  // namespace sph
+
+struct sph_ctx {
+    sph_params_t prm;
+    sph_constants_t cst;
+    sph::StepConsts sc;
+    sph::PipeTable pipe_host;
+    int64_t n;
+    int key_bits;
+    int passes;
+    int pass_shift[4];
+    int pass_bits[4];
+    int num_tiles;
+    int density_block, force_block;
+    int keep_terms;
+    size_t ws_bytes;
+    void* ws;
+    sph::Buffers b;
+    uint32_t* sorted_keys;   // final radix output (key) in cell order
+    uint32_t* sorted_vals;   // final radix output (slot in posB)
+    bool bound;
+    bool have_state;
+    int last_stage;          // last stage run in the current step (0 = hashed, ready)
+    uint64_t* rng_host;      // seeded on the host, uploaded at bind
+};
+
+namespace sph {
+// Launchers (sph_kernels.hip / sph_radix.hip).  All return hipError_t of the last launch.
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+hipError_t launch_upload_f64(sph_ctx* c, const double* pos, const double* vel, hipStream_t s);
+hipError_t launch_upload_f32(sph_ctx* c, const float* x, const float* y, const float* z, const float* vx,
+                             const float* vy, const float* vz, hipStream_t s);
+hipError_t launch_radix_sort_impl(sph_ctx* c, bool gather, hipStream_t s);
+hipError_t launch_cell_table(sph_ctx* c, hipStream_t s);
+hipError_t launch_reorder(sph_ctx* c, hipStream_t s);
+hipError_t launch_density(sph_ctx* c, hipStream_t s);
+hipError_t launch_force(sph_ctx* c, hipStream_t s);
+hipError_t launch_download_f64(sph_ctx* c, double* pos, double* vel, double* rho, double* force, hipStream_t s);
+hipError_t launch_download_f32(sph_ctx* c, float* x, float* y, float* z, float* vx, float* vy, float* vz,
+                               float* rho, hipStream_t s);
+hipError_t launch_read_buffer(sph_ctx* c, int which, void* out, hipStream_t s);
+hipError_t launch_diagnostics(sph_ctx* c, double* out8, hipStream_t s);
+int radix_num_tiles(int64_t n);
+}  // namespace sph

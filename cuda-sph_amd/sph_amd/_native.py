@@ -1,0 +1,244 @@
+"""ctypes binding of libsph_amd.so (declarations in include/sph_amd.h).
+
+torch is imported before the library is opened: libsph_amd.so needs libamdhip64.so.7 and
+torch has already loaded its own copy under that soname, so both share one HIP runtime
+(streams and device pointers pass between them).  There is no fallback: if the library
+is missing or fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_int32, c_int64, c_size_t, c_uint64, c_void_p
+
+import numpy as np
+import torch
+
+from ._build import LIB_PATH
+
+SPH_MODE_BOX, SPH_MODE_PIPE = 0, 1
+SPH_FORM_NAIVE, SPH_FORM_VOXEL = 0, 1
+SPH_FLAG_KEEP_TERMS = 1
+
+STAGE_SORT, STAGE_TABLE, STAGE_REORDER, STAGE_DENSITY, STAGE_FORCE = 1, 2, 3, 4, 5
+
+BUF_KEYS, BUF_PERM, BUF_SORTED_KEYS, BUF_CELL_BEGIN, BUF_DENSITY = 1, 2, 3, 4, 5
+BUF_PRESSURE, BUF_VISCOSITY, BUF_FORCE, BUF_RNG, BUF_NEIGHBOURS = 6, 7, 8, 9, 10
+
+_BUF_DTYPE = {
+    BUF_KEYS: torch.int32, BUF_PERM: torch.int32, BUF_SORTED_KEYS: torch.int32, BUF_CELL_BEGIN: torch.int32,
+    BUF_DENSITY: torch.float32, BUF_PRESSURE: torch.float32, BUF_VISCOSITY: torch.float32,
+    BUF_FORCE: torch.float32, BUF_RNG: torch.int64, BUF_NEIGHBOURS: torch.int32,
+}
+
+
+class SphConstantsC(ctypes.Structure):
+    _fields_ = [("mass", c_double), ("rho0", c_double), ("h", c_double), ("visc", c_double), ("k", c_double),
+                ("damp", c_double)]
+
+
+class SphParamsC(ctypes.Structure):
+    _fields_ = [
+        ("particle_count", c_int64),
+        ("external_force", c_double * 3),
+        ("dt", c_double),
+        ("space_size", c_double * 3),
+        ("voxel_size", c_double * 3),
+        ("mode", c_int32),
+        ("formulation", c_int32),
+        ("pipe_rows", c_int32),
+        ("flags", c_int32),
+        ("pipe", POINTER(c_double)),
+        ("seed", c_uint64),
+    ]
+
+
+EXPORTED = [
+    "sph_abi_version", "sph_last_error", "sph_create", "sph_destroy", "sph_workspace_bytes", "sph_bind_workspace",
+    "sph_upload_state", "sph_upload_state_f32", "sph_step", "sph_download_state", "sph_download_state_f32",
+    "sph_run_stage", "sph_buffer_elems", "sph_read_buffer", "sph_diagnostics", "sph_grid_info", "sph_set_launch",
+]
+
+_lib = None
+
+
+def load_library() -> ctypes.CDLL:
+    """Opens libsph_amd.so (once) and declares its signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "(there is no CPU fallback for the SPH step)")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64 = c_void_p, c_int32, c_int64
+    sig = {
+        "sph_abi_version": ([], i32),
+        "sph_last_error": ([ctypes.c_char_p, c_size_t], i32),
+        "sph_create": ([POINTER(SphParamsC), POINTER(SphConstantsC), POINTER(vp)], i32),
+        "sph_destroy": ([vp], i32),
+        "sph_workspace_bytes": ([vp], c_size_t),
+        "sph_bind_workspace": ([vp, vp, c_size_t, vp], i32),
+        "sph_upload_state": ([vp, vp, vp, vp], i32),
+        "sph_upload_state_f32": ([vp] + [vp] * 6 + [vp], i32),
+        "sph_step": ([vp, i32, vp], i32),
+        "sph_download_state": ([vp, vp, vp, vp, vp, vp], i32),
+        "sph_download_state_f32": ([vp] + [vp] * 7 + [vp], i32),
+        "sph_run_stage": ([vp, i32, vp], i32),
+        "sph_buffer_elems": ([vp, i32], i64),
+        "sph_read_buffer": ([vp, i32, vp, vp], i32),
+        "sph_diagnostics": ([vp, vp, vp], i32),
+        "sph_grid_info": ([vp, POINTER(i32), POINTER(i32), POINTER(i32), POINTER(i32)], i32),
+        "sph_set_launch": ([vp, i32, i32], i32),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(512)
+    load_library().sph_last_error(buf, 512)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise RuntimeError(f"{what} failed ({rc}): {last_error()}")
+
+
+def _ptr(t) -> c_void_p:
+    return c_void_p(0 if t is None else t.data_ptr())
+
+
+def _stream(device) -> c_void_p:
+    return c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+class NativeContext:
+    """One SPH context on one device: owns the workspace tensor and the C handle."""
+
+    def __init__(self, params: SphParamsC, consts: SphConstantsC, device="cuda:0"):
+        if not torch.cuda.is_available():
+            raise RuntimeError("the SPH step runs on a ROCm GPU; none is available")
+        self.lib = load_library()
+        self.device = torch.device(device)
+        self.n = int(params.particle_count)
+        h = c_void_p()
+        check(self.lib.sph_create(ctypes.byref(params), ctypes.byref(consts), ctypes.byref(h)), "sph_create")
+        self.handle = h
+        nbytes = int(self.lib.sph_workspace_bytes(h))
+        # 256-byte alignment for the slices: over-allocate and offset
+        self._ws = torch.empty(nbytes + 256, dtype=torch.uint8, device=self.device)
+        off = (-self._ws.data_ptr()) % 256
+        self.workspace_bytes = nbytes
+        with torch.cuda.device(self.device):
+            check(self.lib.sph_bind_workspace(h, c_void_p(self._ws.data_ptr() + off), nbytes, _stream(self.device)),
+                  "sph_bind_workspace")
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self.lib.sph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state ---------------------------------------------------------------------------
+    def upload(self, pos: torch.Tensor, vel: torch.Tensor):
+        """pos, vel: float64 (N, 3) contiguous on the device."""
+        assert pos.dtype == torch.float64 and vel.dtype == torch.float64
+        assert pos.shape == (self.n, 3) and vel.shape == (self.n, 3)
+        assert pos.is_contiguous() and vel.is_contiguous()
+        check(self.lib.sph_upload_state(self.handle, _ptr(pos), _ptr(vel), _stream(self.device)), "sph_upload_state")
+
+    def upload_f32(self, x, y, z, vx, vy, vz):
+        for t in (x, y, z, vx, vy, vz):
+            assert t.dtype == torch.float32 and t.shape == (self.n,) and t.is_contiguous()
+        check(self.lib.sph_upload_state_f32(self.handle, *[_ptr(t) for t in (x, y, z, vx, vy, vz)],
+                                            _stream(self.device)), "sph_upload_state_f32")
+
+    def step(self, nsteps: int = 1):
+        check(self.lib.sph_step(self.handle, int(nsteps), _stream(self.device)), "sph_step")
+
+    def run_stage(self, stage: int):
+        check(self.lib.sph_run_stage(self.handle, int(stage), _stream(self.device)), "sph_run_stage")
+
+    def download(self, force: bool = False):
+        """Returns float64 device tensors pos (N,3), vel (N,3), rho (N,) [, force (N,3)]."""
+        kw = dict(dtype=torch.float64, device=self.device)
+        pos = torch.empty((self.n, 3), **kw)
+        vel = torch.empty((self.n, 3), **kw)
+        rho = torch.empty((self.n,), **kw)
+        f = torch.empty((self.n, 3), **kw) if force else None
+        check(self.lib.sph_download_state(self.handle, _ptr(pos), _ptr(vel), _ptr(rho), _ptr(f),
+                                          _stream(self.device)), "sph_download_state")
+        return (pos, vel, rho, f) if force else (pos, vel, rho)
+
+    def download_f32(self):
+        """Returns seven float32 device tensors x, y, z, vx, vy, vz, rho."""
+        out = [torch.empty((self.n,), dtype=torch.float32, device=self.device) for _ in range(7)]
+        check(self.lib.sph_download_state_f32(self.handle, *[_ptr(t) for t in out], _stream(self.device)),
+              "sph_download_state_f32")
+        return out
+
+    def read_buffer(self, which: int) -> torch.Tensor:
+        n = int(self.lib.sph_buffer_elems(self.handle, which))
+        if n < 0:
+            raise ValueError(f"unknown buffer {which}")
+        out = torch.empty((max(n, 1),), dtype=_BUF_DTYPE[which], device=self.device)
+        check(self.lib.sph_read_buffer(self.handle, which, _ptr(out), _stream(self.device)), "sph_read_buffer")
+        return out[:n]
+
+    def diagnostics(self) -> np.ndarray:
+        out = torch.empty((8,), dtype=torch.float64, device=self.device)
+        check(self.lib.sph_diagnostics(self.handle, _ptr(out), _stream(self.device)), "sph_diagnostics")
+        return out.cpu().numpy()
+
+    def grid_info(self):
+        dims = (c_int32 * 3)()
+        sten = (c_int32 * 3)()
+        bits, passes = c_int32(), c_int32()
+        check(self.lib.sph_grid_info(self.handle, dims, sten, ctypes.byref(bits), ctypes.byref(passes)),
+              "sph_grid_info")
+        return dict(dims=tuple(dims), stencil=tuple(sten), key_bits=bits.value, passes=passes.value)
+
+    def set_launch(self, density_block: int, force_block: int):
+        check(self.lib.sph_set_launch(self.handle, density_block, force_block), "sph_set_launch")
+
+
+def make_params(n, external_force, dt, space_size, voxel_size, mode, formulation, pipe_table=None, flags=0,
+                seed=16435234):
+    """Builds the C parameter struct; keeps the pipe table alive on the returned object."""
+    p = SphParamsC()
+    p.particle_count = int(n)
+    p.external_force[:] = [float(v) for v in external_force]
+    p.dt = float(dt)
+    p.space_size[:] = [float(v) for v in space_size]
+    p.voxel_size[:] = [float(v) for v in voxel_size]
+    p.mode = int(mode)
+    p.formulation = int(formulation)
+    p.flags = int(flags)
+    p.seed = int(seed)
+    if pipe_table is not None and np.size(pipe_table) > 0:
+        tbl = np.ascontiguousarray(np.asarray(pipe_table, dtype=np.float64).reshape(-1, 5))
+        p.pipe_rows = tbl.shape[0]
+        p.pipe = tbl.ctypes.data_as(POINTER(c_double))
+        p._keep = tbl
+    else:
+        p.pipe_rows = 0
+        p.pipe = None
+    return p
+
+
+def make_consts(c) -> SphConstantsC:
+    k = SphConstantsC()
+    k.mass, k.rho0, k.h, k.visc, k.k, k.damp = (float(c.mass), float(c.rho0), float(c.h), float(c.visc),
+                                                float(c.k), float(c.damp))
+    return k

@@ -1,0 +1,194 @@
+/*
+ * sph_amd.h -- C ABI of the MI355X (gfx950) SPH step library, libsph_amd.so.
+ *
+ * The reference (iwoplaza/cuda-sph) has no FFI: its step is a Python template method,
+ * AbstractSPHStrategy.compute_next_state (sim/src/sph/strategies/abstract_sph_strategy.py:31-46),
+ * whose stages launch numba kernels.  The entry points below are what a binding of that
+ * path needs: the strategy constructor becomes sph_create + sph_bind_workspace, the
+ * per-step host->device copies become sph_upload_state, the stage launches become
+ * sph_step (or sph_run_stage one stage at a time) and the device->host copies become
+ * sph_download_state.  Each declaration cites the reference code it replaces.
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative SPH_E_* code; sph_last_error
+ *     returns the message of the last failure on the calling thread.
+ *   - Pointers named dev_* are device pointers (for example torch tensor data_ptr()).
+ *     Pointers without the prefix are host pointers.
+ *   - stream is a hipStream_t passed as void*; NULL means the default stream.  All work
+ *     is enqueued on that stream; no call synchronises except sph_create/sph_destroy.
+ *   - The library never allocates device memory: the caller allocates the workspace
+ *     (sph_workspace_bytes) and keeps it alive while the context is used.
+ *   - A context is not re-entrant: calls on one context must not overlap.
+ */
+#ifndef SPH_AMD_H
+#define SPH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPH_ABI_VERSION 1
+
+#define SPH_OK 0
+#define SPH_E_INVALID (-1)   /* bad argument or size                    */
+#define SPH_E_STATE (-2)     /* call out of order (no workspace, no state) */
+#define SPH_E_HIP (-3)       /* HIP runtime error                        */
+#define SPH_E_NONFINITE (-4) /* a particle position is NaN or infinite   */
+
+/* Physics constants.  In the reference these are module globals frozen into the numba
+ * kernels at JIT time (config.py:18-23, derived forms config.py:24-29). */
+typedef struct {
+    double mass; /* MASS  */
+    double rho0; /* RHO_0 */
+    double h;    /* INF_R, the smoothing radius */
+    double visc; /* VISC  */
+    double k;    /* K, gas constant of p = K (rho - rho0) */
+    double damp; /* DAMP, box-wall velocity damping */
+} sph_constants_t;
+
+#define SPH_MODE_BOX 0  /* config.py:13 SIM_MODE == 'BOX'  -> collision_kernel_box */
+#define SPH_MODE_PIPE 1 /* config.py:13 SIM_MODE == 'PIPE' -> collision_kernel     */
+
+#define SPH_FORM_NAIVE 0 /* config.py:14 'NAIVE' (naive_kernels.py) */
+#define SPH_FORM_VOXEL 1 /* config.py:14 'VOXEL' (voxel_kernels.py) */
+
+/* SimulationParameters (common/data_classes.py:70-78) plus the reference globals the
+ * step reads.  dt = 1 / fps (abstract_sph_strategy.py:20). */
+typedef struct {
+    int64_t particle_count;   /* data_classes.py:72 */
+    double external_force[3]; /* data_classes.py:73, a force density (divided by rho) */
+    double dt;                /* 1 / fps */
+    double space_size[3];     /* data_classes.py:77 */
+    double voxel_size[3];     /* data_classes.py:78 */
+    int32_t mode;             /* SPH_MODE_* */
+    int32_t formulation;      /* SPH_FORM_* */
+    int32_t pipe_rows;        /* rows of Pipe.to_numpy() (data_classes.py:34-44); 0 in BOX mode */
+    int32_t flags;            /* SPH_FLAG_* */
+    const double* pipe;       /* host, pipe_rows x 5 row-major [x, y, z, start_radius, length] */
+    uint64_t seed;            /* collision RNG seed, abstract_sph_strategy.py:27 uses 16435234 */
+} sph_params_t;
+
+/* Keep the per-particle pressure term, viscosity term and result force of the last step
+ * (readable with sph_read_buffer).  Off by default: they cost 48 bytes per particle. */
+#define SPH_FLAG_KEEP_TERMS 1
+
+typedef struct sph_ctx sph_ctx;
+
+/* Replaces AbstractSPHStrategy.__init__ (abstract_sph_strategy.py:18-29) together with
+ * thread_layout.organize (thread_layout.py:27-33): validates the parameters, plans the
+ * voxel grid and launch shapes and, in PIPE mode, seeds one xoroshiro128+ stream per
+ * particle exactly as create_xoroshiro128p_states(n, seed) does.  Host only. */
+int sph_create(const sph_params_t* params, const sph_constants_t* consts, sph_ctx** out);
+
+/* Releases the host context.  The workspace belongs to the caller. */
+int sph_destroy(sph_ctx* ctx);
+
+/* Device bytes the context needs (state, sort buffers, cell table, RNG streams). */
+size_t sph_workspace_bytes(const sph_ctx* ctx);
+
+/* Binds caller-owned device memory of at least sph_workspace_bytes bytes (256-byte
+ * aligned) and uploads the pipe table and RNG streams.  Replaces the per-step
+ * allocations of _send_arrays_to_gpu (abstract_sph_strategy.py:64-74): after this call
+ * nothing is allocated per step. */
+int sph_bind_workspace(sph_ctx* ctx, void* dev_workspace, size_t bytes, void* stream);
+
+/* Loads a SimulationState (data_classes.py:81-85): dev_pos and dev_vel are float64
+ * (N, 3) row-major in particle order, as cuda.to_device(old_state.position) holds them
+ * (abstract_sph_strategy.py:65-66).  The state is converted to the internal FP32
+ * cell-sorted layout and stays on the device across steps. */
+int sph_upload_state(sph_ctx* ctx, const double* dev_pos, const double* dev_vel, void* stream);
+
+/* Same as sph_upload_state from float32 structure-of-arrays inputs (x, y, z, vx, vy, vz
+ * each of N floats). */
+int sph_upload_state_f32(sph_ctx* ctx, const float* dev_x, const float* dev_y, const float* dev_z,
+                         const float* dev_vx, const float* dev_vy, const float* dev_vz, void* stream);
+
+/* Advances the resident state by nsteps steps.  One step is the body of
+ * compute_next_state (abstract_sph_strategy.py:31-46): voxel hash, sort, cell table,
+ * density, pressure and viscosity terms, integration (base_kernels.py:30-53) and
+ * collision (base_kernels.py:56-98).  Returns SPH_E_STATE before the first upload. */
+int sph_step(sph_ctx* ctx, int32_t nsteps, void* stream);
+
+/* Replaces __finalize_computation (abstract_sph_strategy.py:76-84): writes the state in
+ * particle order as float64 (N, 3) / (N,), i.e. the new SimulationState's position,
+ * velocity and density, and the result_force of the last step.  Any pointer may be
+ * NULL; dev_force requires SPH_FLAG_KEEP_TERMS. */
+int sph_download_state(sph_ctx* ctx, double* dev_pos, double* dev_vel, double* dev_rho, double* dev_force,
+                       void* stream);
+
+/* Same as sph_download_state into float32 structure-of-arrays outputs (any may be NULL). */
+int sph_download_state_f32(sph_ctx* ctx, float* dev_x, float* dev_y, float* dev_z, float* dev_vx, float* dev_vy,
+                           float* dev_vz, float* dev_rho, void* stream);
+
+/* Stage-by-stage execution of one step, for parity tests and profiling.  The stages in
+ * order; sph_step(ctx, 1) runs exactly this sequence.
+ *   SPH_STAGE_SORT     voxel_sph_strategy.py:81-89 (sort of the voxel map, ties by id)
+ *   SPH_STAGE_TABLE    voxel_sph_strategy.py:92-107 (cell start table)
+ *   SPH_STAGE_REORDER  gather of the state into cell order (no reference counterpart)
+ *   SPH_STAGE_DENSITY  naive_kernels.py:7-21 / voxel_kernels.py:108-132
+ *   SPH_STAGE_FORCE    naive_kernels.py:24-80 / voxel_kernels.py:135-211, then
+ *                      base_kernels.py:30-53 and :56-98, then the voxel hash of the
+ *                      new positions for the next step (voxel_kernels.py:88-105)
+ * The hash of the uploaded state is part of sph_upload_state. */
+#define SPH_STAGE_SORT 1
+#define SPH_STAGE_TABLE 2
+#define SPH_STAGE_REORDER 3
+#define SPH_STAGE_DENSITY 4
+#define SPH_STAGE_FORCE 5
+int sph_run_stage(sph_ctx* ctx, int32_t stage, void* stream);
+
+/* Copies an internal buffer to dev_out.  Buffers in particle order unless noted:
+ *   SPH_BUF_KEYS        int32[N]   voxel key of each particle (voxel_kernels.py:88-105)
+ *   SPH_BUF_PERM        int32[N]   particle id at each sorted position (voxel_particle_map
+ *                                  ["particle_id"] after the sort, voxel_sph_strategy.py:88)
+ *   SPH_BUF_SORTED_KEYS int32[N]   key at each sorted position (voxel_particle_map["voxel_id"])
+ *   SPH_BUF_CELL_BEGIN  int32[C]   voxel_begin (voxel_sph_strategy.py:92-107): first sorted
+ *                                  position of each voxel, -1 if empty; C = product of
+ *                                  ceil(space/voxel)
+ *   SPH_BUF_DENSITY     float32[N] density of the current step
+ *   SPH_BUF_PRESSURE    float32[N*3] pressure term  (needs SPH_FLAG_KEEP_TERMS)
+ *   SPH_BUF_VISCOSITY   float32[N*3] viscosity term (needs SPH_FLAG_KEEP_TERMS)
+ *   SPH_BUF_FORCE       float32[N*3] result_force   (needs SPH_FLAG_KEEP_TERMS)
+ *   SPH_BUF_RNG         uint64[N*2] xoroshiro128+ stream of each particle (s0, s1)
+ *   SPH_BUF_NEIGHBOURS  int32[N*33] VOXEL formulation: count then up to 32 particle ids
+ *                                  (get_neighbours, voxel_kernels.py:29-85)
+ * sph_buffer_elems gives the element count of a buffer. */
+#define SPH_BUF_KEYS 1
+#define SPH_BUF_PERM 2
+#define SPH_BUF_SORTED_KEYS 3
+#define SPH_BUF_CELL_BEGIN 4
+#define SPH_BUF_DENSITY 5
+#define SPH_BUF_PRESSURE 6
+#define SPH_BUF_VISCOSITY 7
+#define SPH_BUF_FORCE 8
+#define SPH_BUF_RNG 9
+#define SPH_BUF_NEIGHBOURS 10
+int64_t sph_buffer_elems(const sph_ctx* ctx, int32_t which);
+int sph_read_buffer(sph_ctx* ctx, int32_t which, void* dev_out, void* stream);
+
+/* FP64 global diagnostics of the resident state (no reference counterpart; the reference
+ * only logs particle 0, abstract_sph_strategy.py:120-125).  dev_out8 receives eight
+ * doubles: total mass, momentum x/y/z, kinetic energy, count of non-finite particles,
+ * maximum speed, mean density. */
+int sph_diagnostics(sph_ctx* ctx, double* dev_out8, void* stream);
+
+/* Grid plan of the context: dims[3] (voxels per axis used for keys), stencil[3] (cells
+ * searched on each side), key_bits and sort passes.  Host only. */
+int sph_grid_info(const sph_ctx* ctx, int32_t* dims3, int32_t* stencil3, int32_t* key_bits, int32_t* passes);
+
+/* Launch shapes of the neighbour kernels (threads per block), for the autotuner. */
+int sph_set_launch(sph_ctx* ctx, int32_t density_block, int32_t force_block);
+
+/* Message of the last error on this thread (always NUL-terminated). */
+int sph_last_error(char* buf, size_t len);
+
+int sph_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPH_AMD_H */

@@ -1,0 +1,297 @@
+"""GPU parity of the plain step against the CPU FP64 oracle, through the C ABI.
+
+Bars: voxel keys, sorted permutation and cell table bit-exact; density within 1e-5
+relative; pressure / viscosity terms within 1e-5 of sum_j |term_ij| (the per-particle
+magnitude the sum cancels from); positions within 1e-5 relative.  Inputs are rounded to
+FP32 before both sides see them, so the comparison measures the kernels' arithmetic.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from _brute import naive_terms, near_boundary
+from sph_amd import _native as N
+from sph_amd.constants import SPHConstants
+from sph_amd.data_classes import SimulationParameters, SimulationState
+from sph_amd.strategy import NaiveSPHStrategy, VoxelSPHStrategy
+from sph_amd.workloads import box_uniform, pipe_flow
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def f32(a):
+    return np.asarray(a, dtype=np.float64).astype(np.float32).astype(np.float64)
+
+
+def upload(strat, pos, vel):
+    dev = strat.device
+    strat.ctx.upload(torch.from_numpy(np.ascontiguousarray(pos, dtype=np.float64)).to(dev),
+                     torch.from_numpy(np.ascontiguousarray(vel, dtype=np.float64)).to(dev))
+
+
+def read(strat, which):
+    return strat.ctx.read_buffer(which).cpu().numpy()
+
+
+def oracle_step(w, pos, vel, formulation="NAIVE", rng=None, mode="BOX"):
+    return oracle.step(pos, vel, consts=w.constants, ext=w.params.external_force, dt=1.0 / w.params.fps,
+                       space=w.params.space_size, voxel=w.params.voxel_size, mode=mode, formulation=formulation,
+                       pipe=w.params.pipe.to_numpy() if mode == "PIPE" else None, rng=rng)
+
+
+def outside_points(pos, space):
+    pos = pos.copy()
+    L = np.asarray(space, dtype=np.float64)
+    pts = np.array([[-0.3, 0.2, 0.2], [1.2, 0.5, 0.5], [0.5, -0.01, 0.9], [1.0, 1.0, 1.0], [0.0, 0.0, 0.0],
+                    [0.99999, 0.5, 0.5], [-2.0, -3.0, 5.0], [0.3, 1.7, -0.2]]) * L
+    pos[: len(pts)] = f32(pts)
+    return pos
+
+
+@pytest.mark.parametrize("voxel", [0.0625, 0.0457])
+def test_keys_sort_table_bit_exact(gpu, voxel):
+    """voxel_kernels.py:88-105 keys, voxel_sph_strategy.py:84-88 order, :92-107 voxel_begin."""
+    w = box_uniform(4096, seed=11)
+    pos = outside_points(f32(w.state.position), w.params.space_size)
+    params = SimulationParameters(4096, w.params.external_force, 1, 1000, w.params.pipe, w.params.space_size,
+                                  np.array([voxel] * 3))
+    strat = VoxelSPHStrategy(params, constants=w.constants, device=gpu)
+    upload(strat, pos, np.zeros_like(pos))
+    dims = oracle.space_dims(params.space_size, params.voxel_size, True)
+    ref_keys = oracle.voxel_keys(pos, params.voxel_size, dims)
+    assert np.array_equal(read(strat, N.BUF_KEYS), ref_keys)
+    strat.ctx.run_stage(N.STAGE_SORT)
+    ref_perm = oracle.sort_map(ref_keys)
+    assert np.array_equal(ref_perm, np.argsort(ref_keys, kind="stable"))
+    assert np.array_equal(read(strat, N.BUF_PERM), ref_perm)
+    assert np.array_equal(read(strat, N.BUF_SORTED_KEYS), ref_keys[ref_perm])
+    strat.ctx.run_stage(N.STAGE_TABLE)
+    ref_begin = oracle.voxel_begin(ref_keys[ref_perm], int(np.prod(dims)))
+    assert np.array_equal(read(strat, N.BUF_CELL_BEGIN), ref_begin)
+
+
+def test_grid_keys_match_reference_inside_box(gpu):
+    """The NAIVE path sorts on clamped keys; inside the box they equal the reference keys."""
+    w = box_uniform(4096, seed=12)
+    pos = f32(w.state.position)
+    strat = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu)
+    upload(strat, pos, np.zeros_like(pos))
+    dims = oracle.space_dims(w.params.space_size, w.params.voxel_size, True)
+    ref_keys = oracle.voxel_keys(pos, w.params.voxel_size, dims)
+    assert np.array_equal(read(strat, N.BUF_KEYS), ref_keys)
+    strat.ctx.run_stage(N.STAGE_SORT)
+    assert np.array_equal(read(strat, N.BUF_PERM), np.argsort(ref_keys, kind="stable"))
+
+
+def _force_case(seed=13, n=4096):
+    w = box_uniform(n, seed=seed)
+    rng = np.random.default_rng(seed)
+    pos = f32(w.state.position)
+    vel = f32(rng.normal(0.0, 0.5, size=pos.shape))
+    return w, pos, vel
+
+
+def test_density_and_force_stages(gpu):
+    """naive_kernels.py:7-80 stage by stage."""
+    w, pos, vel = _force_case()
+    c = w.constants
+    strat = NaiveSPHStrategy(w.params, constants=c, device=gpu, keep_terms=True)
+    upload(strat, pos, vel)
+    for st in (N.STAGE_SORT, N.STAGE_TABLE, N.STAGE_REORDER, N.STAGE_DENSITY):
+        strat.ctx.run_stage(st)
+    rho = read(strat, N.BUF_DENSITY).astype(np.float64)
+    rho_ref = oracle.naive_density(pos, c)
+    rel = np.abs(rho - rho_ref) / rho_ref
+    print(f"density max rel err {rel.max():.3e}")
+    assert rel.max() <= TOL
+    strat.ctx.run_stage(N.STAGE_FORCE)
+    P = read(strat, N.BUF_PRESSURE).reshape(-1, 3).astype(np.float64)
+    V = read(strat, N.BUF_VISCOSITY).reshape(-1, 3).astype(np.float64)
+    # kernel error alone: the oracle's sums over the GPU's densities
+    P_ref = oracle.naive_pressure(pos, rho, c)
+    V_ref = oracle.naive_viscosity(pos, vel, rho, c)
+    _, _, _, sP, sV = naive_terms(pos, vel, c, rho=rho)
+    eP = np.linalg.norm(P - P_ref, axis=1) / np.maximum(sP, 1e-30)
+    eV = np.linalg.norm(V - V_ref, axis=1) / np.maximum(sV, 1e-30)
+    print(f"pressure err {eP.max():.3e} viscosity err {eV.max():.3e} (normalised)")
+    assert eP.max() <= TOL and eV.max() <= TOL
+    # end to end against the oracle's own densities (includes density rounding)
+    P_ref2 = oracle.naive_pressure(pos, rho_ref, c)
+    _, _, _, sP2, _ = naive_terms(pos, vel, c, rho=rho_ref)
+    eP2 = np.linalg.norm(P - P_ref2, axis=1) / np.maximum(sP2, 1e-30)
+    print(f"pressure err vs FP64 densities {eP2.max():.3e}")
+    assert eP2.max() <= 1e-4
+
+
+def test_full_step_matches_oracle(gpu):
+    """compute_next_state (abstract_sph_strategy.py:31-46), NAIVE, BOX."""
+    w, pos, vel = _force_case(seed=14)
+    strat = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu, keep_terms=True)
+    out = strat.compute_next_state(SimulationState(pos, vel, np.zeros(len(pos))))
+    rp, rv, rrho, rf, _ = oracle_step(w, pos, vel)
+    drho = np.abs(out.density - rrho) / rrho
+    dpos = np.abs(out.position - rp) / np.maximum(np.abs(rp), 1e-3)
+    rho_, P_, V_, sP, sV = naive_terms(pos, vel, w.constants)
+    scale_v = (sP + sV + np.linalg.norm(w.params.external_force)) / rrho / w.params.fps
+    dvel = np.linalg.norm(out.velocity - rv, axis=1) / np.maximum(scale_v, 1e-30)
+    dfor = np.linalg.norm(strat.result_force - rf, axis=1) / np.maximum(sP + sV, 1e-30)
+    print(f"step: rho {drho.max():.2e} pos {dpos.max():.2e} vel(norm) {dvel.max():.2e} force(norm) {dfor.max():.2e}")
+    assert drho.max() <= TOL and dpos.max() <= TOL
+    assert dvel.max() <= 1e-4 and dfor.max() <= 1e-4
+
+
+def test_multi_step_trajectory(gpu):
+    """20 steps of the box workload: positions stay within 1e-5 relative of the FP64 oracle."""
+    w = box_uniform(4096, seed=15)
+    pos, vel = f32(w.state.position), f32(w.state.velocity)
+    strat = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu)
+    out = strat.advance(SimulationState(pos, vel, np.zeros(len(pos))), 20)
+    rp, rv = pos, vel
+    for _ in range(20):
+        rp, rv, rrho, _, _ = oracle_step(w, rp, rv)
+    dpos = np.abs(out.position - rp) / np.maximum(np.abs(rp), 1e-3)
+    drho = np.abs(out.density - rrho) / rrho
+    print(f"20 steps: pos {dpos.max():.2e} rho {drho.max():.2e}")
+    assert dpos.max() <= TOL
+    assert drho.max() <= 1e-4
+
+
+def test_box_collisions_all_walls(gpu):
+    """base_kernels.py:75-98 on particles leaving through each face and a corner."""
+    c = SPHConstants(mass=0.02, rho0=1000.0, h=0.0457, visc=3.5, k=3.0, damp=0.5)
+    pos = f32([[0.0005, 0.5, 0.5], [0.9995, 0.5, 0.5], [0.5, 0.0005, 0.5], [0.5, 0.9995, 0.5],
+               [0.5, 0.5, 0.0005], [0.5, 0.5, 0.9995], [0.0005, 0.0005, 0.9995], [0.3, 0.3, 0.3]])
+    vel = f32([[-2, 0, 0], [2, 0, 0], [0, -2, 0], [0, 2, 0], [0, 0, -2], [0, 0, 2], [-3, -3, 3], [0.1, 0.2, 0.3]])
+    params = SimulationParameters(len(pos), np.array([0.0, -9.81, 0.0]), 1, 1000, space_size=np.ones(3),
+                                  voxel_size=np.full(3, 0.0625))
+    strat = NaiveSPHStrategy(params, constants=c, device=gpu)
+    out = strat.compute_next_state(SimulationState(pos, vel, np.zeros(len(pos))))
+    rp, rv, _, _, _ = oracle.step(pos, vel, consts=c, ext=params.external_force, dt=1e-3, space=np.ones(3),
+                                  voxel=params.voxel_size)
+    assert np.allclose(out.position, rp, rtol=1e-6, atol=1e-7)
+    assert np.allclose(out.velocity, rv, rtol=1e-5, atol=1e-6)
+    # every bounced component flipped and damped
+    assert out.velocity[0, 0] > 0 and out.velocity[1, 0] < 0 and abs(out.velocity[0, 0]) < 2
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 63, 1000, 2049])
+def test_small_and_ragged_sizes(gpu, n):
+    w = box_uniform(max(n, 1), seed=16)
+    params = SimulationParameters(n, w.params.external_force, 1, 1000, w.params.pipe, w.params.space_size,
+                                  w.params.voxel_size)
+    pos = f32(w.state.position[:n])
+    vel = np.zeros((n, 3))
+    strat = NaiveSPHStrategy(params, constants=w.constants, device=gpu)
+    out = strat.advance(SimulationState(pos, vel, np.zeros(n)), 3)
+    rp, rv = pos, vel
+    for _ in range(3):
+        rp, rv, rrho, _, _ = oracle_step(w, rp, rv)
+    assert out.position.shape == (n, 3)
+    if n:
+        assert np.allclose(out.position, rp, rtol=1e-6, atol=1e-7)
+        assert np.allclose(out.density, rrho, rtol=TOL)
+    if n == 1:
+        # an isolated particle: rho = m W(0) = m 315 / (64 pi h^3)
+        c = w.constants
+        assert math.isclose(out.density[0], c.mass * 315 / (64 * math.pi * c.h ** 3), rel_tol=1e-6)
+
+
+def test_all_particles_in_one_cell(gpu):
+    c = SPHConstants(mass=0.02, rho0=1000.0, h=0.0457, visc=3.5, k=3.0, damp=0.5)
+    rng = np.random.default_rng(17)
+    pos = f32(0.5 + rng.uniform(0, 0.06, size=(700, 3)))
+    vel = f32(rng.normal(0, 0.1, size=(700, 3)))
+    params = SimulationParameters(700, np.array([0.0, -9.81, 0.0]), 1, 1000, space_size=np.ones(3),
+                                  voxel_size=np.full(3, 0.0625))
+    strat = NaiveSPHStrategy(params, constants=c, device=gpu)
+    out = strat.compute_next_state(SimulationState(pos, vel, np.zeros(700)))
+    rp, rv, rrho, _, _ = oracle.step(pos, vel, consts=c, ext=params.external_force, dt=1e-3, space=np.ones(3),
+                                     voxel=params.voxel_size)
+    assert np.allclose(out.density, rrho, rtol=TOL)
+    assert np.allclose(out.position, rp, rtol=TOL)
+
+
+def test_voxel_formulation(gpu):
+    """voxel_kernels.py:29-211: capped neighbour lists, then rho / P / V of that formulation."""
+    w = box_uniform(4096, seed=18)
+    pos = f32(w.state.position)
+    vel = f32(np.random.default_rng(18).normal(0, 0.3, size=pos.shape))
+    c = w.constants
+    strat = VoxelSPHStrategy(w.params, constants=c, device=gpu, keep_terms=True)
+    upload(strat, pos, vel)
+    for st in (N.STAGE_SORT, N.STAGE_TABLE, N.STAGE_REORDER, N.STAGE_DENSITY, N.STAGE_FORCE):
+        strat.ctx.run_stage(st)
+    lists = read(strat, N.BUF_NEIGHBOURS).reshape(-1, 33)
+    dims = oracle.space_dims(w.params.space_size, w.params.voxel_size, True)
+    tdims = oracle.space_dims(w.params.space_size, w.params.voxel_size, False)
+    keys = oracle.voxel_keys(pos, w.params.voxel_size, dims)
+    perm = oracle.sort_map(keys)
+    begin = oracle.voxel_begin(keys[perm], int(np.prod(dims)))
+    n = len(pos)
+    nb = np.full((n, 32), -1, dtype=np.int32)
+    nn = np.zeros(n, dtype=np.int32)
+    for i in range(n):
+        l = oracle.voxel_neighbours(pos, begin, perm, w.params.voxel_size, tdims, c.h, i)
+        nn[i] = len(l)
+        nb[i, : len(l)] = l
+    # pairs within 1e-5 h of the boundary may fall either side in FP32; skip those particles
+    close = near_boundary(pos, c.h, 1e-5)
+    same = (lists[:, 0] == nn) & np.all((lists[:, 1:] == nb) | (nb == -1), axis=1)
+    assert np.all(same | close), f"{np.sum(~(same | close))} neighbour lists differ"
+    rho_ref, P_ref, V_ref = oracle.voxel_forces(pos, vel, nb, nn, c)
+    ok = ~close
+    rho = read(strat, N.BUF_DENSITY).astype(np.float64)
+    assert np.max(np.abs(rho - rho_ref)[ok] / np.abs(rho_ref[ok])) <= TOL
+    P = read(strat, N.BUF_PRESSURE).reshape(-1, 3)
+    V = read(strat, N.BUF_VISCOSITY).reshape(-1, 3)
+    scaleP = np.maximum(np.abs(P_ref).max(axis=1), 1e-30)
+    print("voxel P err", np.max(np.linalg.norm(P - P_ref, axis=1)[ok] / scaleP[ok]))
+    assert np.allclose(V[ok], V_ref[ok], rtol=1e-4, atol=1e-6)
+
+
+def test_pipe_mode_and_rng(gpu):
+    """base_kernels.py:56-72, 101-127: wall reflection, mirror at x < 0, re-injection at the
+    inlet with the particle's own xoroshiro128+ stream (seed 16435234, one stream per id)."""
+    w = pipe_flow(n=4096, seed=19, steps=10)
+    pos, vel = f32(w.state.position), f32(w.state.velocity)
+    # particles about to leave through the outlet, the inlet and the wall
+    pos[:40, 0] = 3.999
+    vel[:40, 0] = 5.0
+    pos[40:60, 0] = 0.0005
+    vel[40:60, 0] = -3.0
+    strat = NaiveSPHStrategy(w.params, constants=w.constants, mode="PIPE", device=gpu)
+    rng0 = oracle.rng_init(16435234, len(pos))
+    assert np.array_equal(read_rng_after_upload(strat, pos, vel), rng0)
+    out = strat.advance(SimulationState(pos, vel, np.zeros(len(pos))), 3)
+    rp, rv, rr = pos, vel, rng0
+    for _ in range(3):
+        rp, rv, _, _, rr = oracle_step(w, rp, rv, rng=rr, mode="PIPE")
+    assert np.array_equal(read(strat, N.BUF_RNG).view(np.uint64), rr)
+    moved = np.any(rr != rng0)
+    assert moved
+    err = np.abs(out.position - rp).max(axis=1)
+    print("pipe max abs pos err", err.max(), "re-injected", int(np.sum(rr.reshape(-1, 2)[:, 0] != rng0.reshape(-1, 2)[:, 0])))
+    assert err.max() <= 1e-4
+
+
+def read_rng_after_upload(strat, pos, vel):
+    upload(strat, pos, vel)
+    return read(strat, N.BUF_RNG).view(np.uint64)
+
+
+def test_diagnostics_fp64(gpu):
+    w, pos, vel = _force_case(seed=20)
+    strat = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu)
+    out = strat.advance(SimulationState(pos, vel, np.zeros(len(pos))), 2)
+    d = strat.diagnostics()
+    m = w.constants.mass
+    v = out.velocity
+    assert math.isclose(d["mass"], m * len(pos))
+    assert np.allclose(d["momentum"], m * v.sum(axis=0), rtol=1e-12, atol=1e-12)
+    assert math.isclose(d["kinetic_energy"], 0.5 * m * np.sum(v * v), rel_tol=1e-12)
+    assert d["non_finite"] == 0
