@@ -77,21 +77,22 @@ void seed_streams(uint64_t seed, int64_t n, uint64_t* out) {
 
 size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
-struct Layout {
-    size_t off[24];
-    size_t total;
-};
-
 // Workspace slices, in order.
 enum Slice {
-    S_POSA, S_VELA, S_POSB, S_VELB, S_AUX, S_KEYI, S_VALI, S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_CS,
-    S_HIST, S_DTOT, S_RNG, S_DBGP, S_DBGV, S_DBGF, S_NBR, S_DIAG, S_FLAGS, S_PIPE, S_COUNT
+    S_POSA, S_VELA, S_POSB, S_VELB, S_PIDA, S_PIDB, S_RHOA, S_KEYI, S_VALI, S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_CS,
+    S_HIST, S_DTOT, S_RNG, S_DBGP, S_DBGV, S_DBGF, S_NBR, S_NL, S_NN, S_DIAG, S_FLAGS, S_PIPE, S_COUNT
+};
+
+struct Layout {
+    size_t off[S_COUNT];
+    size_t total;
 };
 
 Layout plan_layout(const sph_ctx* c) {
     const size_t n = (size_t)c->n, nn = n > 0 ? n : 1;
     size_t sz[S_COUNT];
-    sz[S_POSA] = sz[S_VELA] = sz[S_POSB] = sz[S_VELB] = sz[S_AUX] = 16 * nn;
+    sz[S_POSA] = sz[S_VELA] = sz[S_POSB] = sz[S_VELB] = 16 * nn;
+    sz[S_PIDA] = sz[S_PIDB] = sz[S_RHOA] = 4 * nn;
     sz[S_KEYI] = sz[S_VALI] = sz[S_KEYS0] = sz[S_VALS0] = sz[S_KEYS1] = sz[S_VALS1] = 4 * nn;
     sz[S_CS] = 4 * ((size_t)c->sc.n_cells + 1);
     sz[S_HIST] = 4 * 256 * (size_t)(c->num_tiles > 0 ? c->num_tiles : 1);
@@ -99,6 +100,8 @@ Layout plan_layout(const sph_ctx* c) {
     sz[S_RNG] = c->prm.mode == SPH_MODE_PIPE ? 16 * nn : 16;
     sz[S_DBGP] = sz[S_DBGV] = sz[S_DBGF] = c->keep_terms ? 16 * nn : 16;
     sz[S_NBR] = c->prm.formulation == SPH_FORM_VOXEL ? 4 * 33 * nn : 16;
+    sz[S_NL] = c->prm.formulation == SPH_FORM_NAIVE ? 4 * (size_t)sph::kListCap * nn : 16;
+    sz[S_NN] = 4 * nn;
     sz[S_DIAG] = 8 * 8 * 512;
     sz[S_FLAGS] = 64;
     sz[S_PIPE] = sizeof(sph::PipeTable);
@@ -115,8 +118,9 @@ Layout plan_layout(const sph_ctx* c) {
 int plan(sph_ctx* c) {
     const sph_params_t& P = c->prm;
     const sph_constants_t& K = c->cst;
-    if (P.particle_count < 0 || P.particle_count > (int64_t(1) << 30))
-        return fail(SPH_E_INVALID, "particle_count %lld outside [0, 2^30]", (long long)P.particle_count);
+    // 2^27 particles keep every byte offset of a 16-byte-per-particle array within 31 bits
+    if (P.particle_count < 0 || P.particle_count > (int64_t(1) << 27))
+        return fail(SPH_E_INVALID, "particle_count %lld outside [0, 2^27]", (long long)P.particle_count);
     if (!(K.h > 0) || !(K.mass > 0) || !isfinite(K.h) || !isfinite(K.mass))
         return fail(SPH_E_INVALID, "h and mass must be positive and finite");
     if (!(P.dt > 0) || !isfinite(P.dt)) return fail(SPH_E_INVALID, "dt must be positive and finite");
@@ -142,7 +146,8 @@ int plan(sph_ctx* c) {
         S.dims[d] = (int)ceil(q);        // voxel_sph_strategy.py:70-73
         S.tdims[d] = (int)q;             // voxel_sph_strategy.py:109-116
         if (S.dims[d] < 1) S.dims[d] = 1;
-        S.stencil[d] = (int)ceil(K.h / vx);
+        // cells searched on each side; exact for power-of-two voxels, with the search margin otherwise
+        S.stencil[d] = (int)ceil((is_pow2(vx) ? K.h : K.h * (1.0 + 2e-5)) / vx);
         if (!is_pow2(vx)) S.pow2 = 0;
         S.dcell[d] = vx;
         S.cell[d] = (float)vx;
@@ -152,7 +157,7 @@ int plan(sph_ctx* c) {
         S.box_hi_target[d] = (float)(sp - 1e-3);
         S.ext[d] = (float)P.external_force[d];
     }
-    if (cells > (int64_t(1) << 30)) return fail(SPH_E_INVALID, "voxel grid of %lld cells is too large", (long long)cells);
+    if (cells > (int64_t(1) << 28)) return fail(SPH_E_INVALID, "voxel grid of %lld cells is too large", (long long)cells);
     S.n_cells = (int)cells;
     S.box_lo_target = 1e-3f;
     S.h = (float)K.h;
@@ -281,7 +286,9 @@ int sph_bind_workspace(sph_ctx* c, void* dev, size_t bytes, void* stream) {
     b.velA = (float4*)(base + L.off[S_VELA]);
     b.posB = (float4*)(base + L.off[S_POSB]);
     b.velB = (float4*)(base + L.off[S_VELB]);
-    b.aux = (float4*)(base + L.off[S_AUX]);
+    b.pidA = (uint32_t*)(base + L.off[S_PIDA]);
+    b.pidB = (uint32_t*)(base + L.off[S_PIDB]);
+    b.rhoA = (float*)(base + L.off[S_RHOA]);
     b.keyI = (uint32_t*)(base + L.off[S_KEYI]);
     b.valI = (uint32_t*)(base + L.off[S_VALI]);
     b.keyS[0] = (uint32_t*)(base + L.off[S_KEYS0]);
@@ -296,6 +303,8 @@ int sph_bind_workspace(sph_ctx* c, void* dev, size_t bytes, void* stream) {
     b.dbgV = (float4*)(base + L.off[S_DBGV]);
     b.dbgF = (float4*)(base + L.off[S_DBGF]);
     b.nbr = (int32_t*)(base + L.off[S_NBR]);
+    b.nl = (uint32_t*)(base + L.off[S_NL]);
+    b.nn = (uint32_t*)(base + L.off[S_NN]);
     b.diag_partials = (double*)(base + L.off[S_DIAG]);
     b.flags = (uint32_t*)(base + L.off[S_FLAGS]);
     b.pipe = (sph::PipeTable*)(base + L.off[S_PIPE]);

@@ -1,15 +1,21 @@
 // sph_internal.h -- types shared by the gfx950 kernels and the host side of libsph_amd.so.
 //
 // Device layout (all arrays in one caller-owned workspace, 256-byte aligned slices):
-//   posA/velA  float4[N]  state in cell order of the current step: (x, y, z, rho_prev),
-//                         (vx, vy, vz, particle id as int bits)
-//   posB/velB  float4[N]  state after integration, still in the previous cell order;
-//                         this is the resident state between steps
-//   aux        float4[N]  (rho, p, 1/rho, 0) in cell order, written by the density pass
+//   posA/velA  float4[N]  state in cell order of the current step; the density pass puts
+//                         the pressure p in posA.w and 1/rho in velA.w, so one neighbour costs
+//                         two 16-byte loads: (x, y, z, p) and (vx, vy, vz, 1/rho)
+//   pidA, rhoA u32[N], f32[N]  particle id and density of each cell-ordered slot
+//   posB/velB  float4[N]  state after integration, still in the previous cell order, with
+//   pidB       u32[N]     rho of the step in posB.w; this is the resident state between steps
 //   keyI/valI  u32[N]     sort input in particle-id order: voxel key, slot in posB
 //   keyS/valS  u32[2][N]  radix ping-pong buffers
 //   cs         u32[C+1]   cell start table: cs[c] = first cell-sorted index with key >= c
 //   rng        u64[2N]    xoroshiro128+ stream per particle id (PIPE mode)
+//   nl/nn      u32[96][N], u32[N]  NAIVE neighbour lists of the density pass, read by the
+//                         force pass.  Column-major: slot t of consecutive particles is
+//                         contiguous.  The density pass appends each lane's first 32 hits to
+//                         a lane-private list in LDS and copies it out slot by slot at the
+//                         end (every store coalesced); hits beyond 32 go straight to HBM.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -20,6 +26,8 @@ namespace sph {
 
 constexpr int kMaxPipeRows = 64;
 constexpr int kMaxNeighbours = 32;  // MAX_NEIGHBOURS, config.py:30
+constexpr int kListCap = 96;        // NAIVE neighbour-list slots per particle; more falls back to a rescan
+constexpr int kLdsListCap = 32;     // of which staged in LDS by the density pass
 constexpr int kXcds = 8;            // MI355X: 8 accelerator complex dies, one L2 each
 
 // Everything a step kernel needs, passed by value (fits the kernel-argument segment).
@@ -56,7 +64,9 @@ struct PipeTable {
 };
 
 struct Buffers {
-    float4 *posA, *velA, *posB, *velB, *aux;
+    float4 *posA, *velA, *posB, *velB;
+    uint32_t *pidA, *pidB;
+    float* rhoA;
     uint32_t *keyI, *valI;
     uint32_t *keyS[2], *valS[2];
     uint32_t* cs;
@@ -65,6 +75,8 @@ struct Buffers {
     uint64_t* rng;
     float4 *dbgP, *dbgV, *dbgF;
     int32_t* nbr;          // VOXEL formulation neighbour lists, N x (1 + 32)
+    uint32_t* nl;          // NAIVE neighbour lists, column-major [kListCap][N] (cell-ordered indices, self excluded)
+    uint32_t* nn;          // NAIVE neighbour count per cell-ordered particle (may exceed kListCap)
     double* diag_partials;
     uint32_t* flags;       // [0] count of non-finite positions seen by the hash
     PipeTable* pipe;

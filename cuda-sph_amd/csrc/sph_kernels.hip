@@ -3,7 +3,7 @@
 // One thread per particle, particles in cell order.  A step is
 //   sort (sph_radix.hip, last pass gathers posB/velB -> posA/velA in cell order)
 //   cell_table   cs[c] = first sorted index with key >= c (dense, no empty markers)
-//   density      rho, p = K (rho - rho0) and 1/rho into aux
+//   density      rho, p = K (rho - rho0) into posA.w and 1/rho into velA.w, neighbour lists
 //   force        pressure + viscosity sums, symplectic Euler, wall collision, then the
 //                voxel hash of the new position for the next step's sort
 // The NAIVE formulation (the reference default) sums over every j with |x_i - x_j|^2 <= h^2;
@@ -26,8 +26,20 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
     return x < r ? x * (q + 1) + local : r * (q + 1) + (x - r) * q + local;
 }
 
-__device__ __forceinline__ float pid_bits(uint32_t pid) { return __uint_as_float(pid); }
-__device__ __forceinline__ uint32_t pid_of(float4 v) { return __float_as_uint(v.w); }
+
+// Raw buffer loads with 32-bit byte offsets: no 64-bit address arithmetic per access, and
+// an offset past num_records reads zero instead of faulting.  Word 3 of the descriptor is
+// the gfx9 value for untyped 32-bit-element buffers.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float4 ld_f4(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)byte_off, 0, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
+}
+__device__ __forceinline__ uint32_t ld_u32(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)byte_off, 0, 0);
+}
 
 // ---------------------------------------------------------------------------------------
 // Voxel keys.  Reference semantics (voxel_kernels.py:9-17, 88-105): per axis
@@ -84,14 +96,16 @@ __device__ __forceinline__ void hash_particle(const StepConsts& C, float x, floa
 // ---------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void upload_f64_kernel(StepConsts C, const double* __restrict__ pos,
                                                           const double* __restrict__ vel, float4* __restrict__ posB,
-                                                          float4* __restrict__ velB, uint32_t* __restrict__ keyI,
-                                                          uint32_t* __restrict__ valI, uint32_t* __restrict__ flags) {
+                                                          float4* __restrict__ velB, uint32_t* __restrict__ pidB,
+                                                          uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
+                                                          uint32_t* __restrict__ flags) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= C.n) return;
     const float x = (float)pos[3 * (int64_t)i], y = (float)pos[3 * (int64_t)i + 1], z = (float)pos[3 * (int64_t)i + 2];
     posB[i] = make_float4(x, y, z, 0.0f);
     velB[i] = make_float4((float)vel[3 * (int64_t)i], (float)vel[3 * (int64_t)i + 1], (float)vel[3 * (int64_t)i + 2],
-                          pid_bits((uint32_t)i));
+                          0.0f);
+    pidB[i] = (uint32_t)i;
     hash_particle(C, x, y, z, (uint32_t)i, (uint32_t)i, keyI, valI, flags);
 }
 
@@ -99,13 +113,15 @@ __global__ __launch_bounds__(256) void upload_f32_kernel(StepConsts C, const flo
                                                           const float* __restrict__ py, const float* __restrict__ pz,
                                                           const float* __restrict__ vx, const float* __restrict__ vy,
                                                           const float* __restrict__ vz, float4* __restrict__ posB,
-                                                          float4* __restrict__ velB, uint32_t* __restrict__ keyI,
-                                                          uint32_t* __restrict__ valI, uint32_t* __restrict__ flags) {
+                                                          float4* __restrict__ velB, uint32_t* __restrict__ pidB,
+                                                          uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
+                                                          uint32_t* __restrict__ flags) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= C.n) return;
     const float x = px[i], y = py[i], z = pz[i];
     posB[i] = make_float4(x, y, z, 0.0f);
-    velB[i] = make_float4(vx[i], vy[i], vz[i], pid_bits((uint32_t)i));
+    velB[i] = make_float4(vx[i], vy[i], vz[i], 0.0f);
+    pidB[i] = (uint32_t)i;
     hash_particle(C, x, y, z, (uint32_t)i, (uint32_t)i, keyI, valI, flags);
 }
 
@@ -147,12 +163,14 @@ __global__ __launch_bounds__(256) void cell_table_kernel(StepConsts C, const uin
 
 __global__ __launch_bounds__(256) void reorder_kernel(int n, const uint32_t* __restrict__ svals,
                                                        const float4* __restrict__ posB, const float4* __restrict__ velB,
-                                                       float4* __restrict__ posA, float4* __restrict__ velA) {
+                                                       const uint32_t* __restrict__ pidB, float4* __restrict__ posA,
+                                                       float4* __restrict__ velA, uint32_t* __restrict__ pidA) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const uint32_t s = svals[j];
     posA[j] = posB[s];
     velA[j] = velB[s];
+    pidA[j] = pidB[s];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -191,24 +209,75 @@ __device__ __forceinline__ void for_each_candidate(float px, float py, float pz,
     }
 }
 
-__global__ __launch_bounds__(256) void naive_density_kernel(StepConsts C, const float4* __restrict__ pos,
-                                                             const uint32_t* __restrict__ cs,
-                                                             float4* __restrict__ aux) {
+// Writes pressure and 1/rho into the .w lanes of posA / velA (4-byte stores; the .w lanes
+// are not read by this kernel, other threads read only x, y, z of their neighbours).
+__device__ __forceinline__ void store_density(const StepConsts& C, int i, float acc, float4* pos, float4* vel,
+                                              float* __restrict__ rhoA) {
+    const float rho = C.w_norm * acc;
+    rhoA[i] = rho;
+    reinterpret_cast<float*>(pos + i)[3] = C.k * (rho - C.rho0);
+    reinterpret_cast<float*>(vel + i)[3] = 1.0f / rho;
+}
+
+// Density (naive_kernels.py:7-21) and the particle's neighbour list: every k with
+// r^2 <= h^2 in candidate order, the particle itself included (the force pass gives the
+// self pair a zero contribution).  (1 - r^2/h^2)^3 is accumulated without a branch and is
+// zero outside h.  Candidates are read two at a time through 32-bit buffer offsets.
+constexpr int kDensityThreads = 256;
+
+__global__ __launch_bounds__(kDensityThreads) void naive_density_kernel(StepConsts C, float4* pos, float4* vel,
+                                                                         const uint32_t* __restrict__ cs,
+                                                                         float* __restrict__ rhoA,
+                                                                         uint32_t* __restrict__ nl,
+                                                                         uint32_t* __restrict__ nn) {
+    __shared__ uint32_t s_list[kLdsListCap][kDensityThreads];
     const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= C.n) return;
+    const __amdgpu_buffer_rsrc_t rpos = make_rsrc(pos, (uint32_t)C.n * 16u);
+    const __amdgpu_buffer_rsrc_t rcs = make_rsrc(cs, ((uint32_t)C.n_cells + 1u) * 4u);
     const float4 p = pos[i];
+    const size_t stride = (size_t)C.n;
     float acc = 0.0f;
-    for_each_candidate(p.x, p.y, p.z, C, cs, [&](uint32_t k) {
-        const float4 q = pos[k];
+    uint32_t cnt = 0;
+    auto visit = [&](float4 q, uint32_t off) {
         const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
+        const float t = fmaxf(1.0f - r2 * C.inv_h2, 0.0f);
+        acc += t * t * t;
         if (r2 <= C.h2) {
-            const float t = 1.0f - r2 * C.inv_h2;
-            acc += t * t * t;
+            const uint32_t k = off >> 4;
+            if (cnt < (uint32_t)kLdsListCap) s_list[cnt][threadIdx.x] = k;
+            else if (cnt < (uint32_t)kListCap) nl[cnt * stride + i] = k;
+            ++cnt;
         }
-    });
-    const float rho = C.w_norm * acc;
-    aux[i] = make_float4(rho, C.k * (rho - C.rho0), 1.0f / rho, 0.0f);
+    };
+    const float hs = C.h_search;
+    const float hs2 = hs * hs;
+    const int ylo = grid_coord(p.y - hs, 1, C), yhi = grid_coord(p.y + hs, 1, C);
+    const int zlo = grid_coord(p.z - hs, 2, C), zhi = grid_coord(p.z + hs, 2, C);
+    for (int cz = zlo; cz <= zhi; ++cz) {
+        const float ez = slab_dist(p.z, cz, 2, C);
+        for (int cy = ylo; cy <= yhi; ++cy) {
+            const float ey = slab_dist(p.y, cy, 1, C);
+            const float rem = hs2 - ez * ez - ey * ey;
+            if (rem < 0.0f) continue;
+            const float e = __builtin_amdgcn_sqrtf(rem);  // within 1 ulp; hs carries the margin
+            const int xlo = grid_coord(p.x - e, 0, C), xhi = grid_coord(p.x + e, 0, C);
+            const uint32_t row = (uint32_t)(C.dims[0] * (cy + C.dims[1] * cz));
+            uint32_t off = ld_u32(rcs, (row + (uint32_t)xlo) * 4u) * 16u;
+            const uint32_t end = ld_u32(rcs, (row + (uint32_t)xhi + 1u) * 4u) * 16u;
+            for (; off + 16u < end; off += 32u) {
+                const float4 q0 = ld_f4(rpos, off), q1 = ld_f4(rpos, off + 16u);
+                visit(q0, off);
+                visit(q1, off + 16u);
+            }
+            if (off < end) visit(ld_f4(rpos, off), off);
+        }
+    }
+    nn[i] = cnt;
+    const uint32_t staged = cnt < (uint32_t)kLdsListCap ? cnt : (uint32_t)kLdsListCap;
+    for (uint32_t t = 0; t < staged; ++t) nl[t * stride + i] = s_list[t][threadIdx.x];
+    store_density(C, i, acc, pos, vel, rhoA);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -333,8 +402,9 @@ __device__ void collide_pipe(const StepConsts& C, const PipeTable& T, Vec3& p, V
 
 // Symplectic Euler (base_kernels.py:30-53), collision, output in the same sorted slot of
 // posB/velB, hash of the new position for the next step.
-__device__ __forceinline__ void finish_particle(const StepConsts& C, int i, float4 p, float4 v, float rho, Vec3 P,
-                                                Vec3 V, float4* __restrict__ posOut, float4* __restrict__ velOut,
+__device__ __forceinline__ void finish_particle(const StepConsts& C, int i, float4 p, float4 v, uint32_t pid,
+                                                float rho, Vec3 P, Vec3 V, float4* __restrict__ posOut,
+                                                float4* __restrict__ velOut, uint32_t* __restrict__ pidOut,
                                                 uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
                                                 uint64_t* __restrict__ rng, const PipeTable* __restrict__ pipe,
                                                 float4* __restrict__ dbgP, float4* __restrict__ dbgV,
@@ -344,7 +414,6 @@ __device__ __forceinline__ void finish_particle(const StepConsts& C, int i, floa
     const float fz = C.ext[2] + -P.z + V.z;
     Vec3 vv{v.x + fx / rho * C.dt, v.y + fy / rho * C.dt, v.z + fz / rho * C.dt};
     Vec3 pp{p.x + vv.x * C.dt, p.y + vv.y * C.dt, p.z + vv.z * C.dt};
-    const uint32_t pid = pid_of(v);
     if (C.mode == SPH_MODE_PIPE) {
         collide_pipe(C, *pipe, pp, vv, rng, pid);
     } else {
@@ -353,7 +422,8 @@ __device__ __forceinline__ void finish_particle(const StepConsts& C, int i, floa
         collide_box(C, pp.z, vv.z, 2);
     }
     posOut[i] = make_float4(pp.x, pp.y, pp.z, rho);
-    velOut[i] = make_float4(vv.x, vv.y, vv.z, v.w);
+    velOut[i] = make_float4(vv.x, vv.y, vv.z, 0.0f);
+    pidOut[i] = pid;
     if (dbgP) {
         dbgP[i] = make_float4(P.x, P.y, P.z, 0.0f);
         dbgV[i] = make_float4(V.x, V.y, V.z, 0.0f);
@@ -362,39 +432,74 @@ __device__ __forceinline__ void finish_particle(const StepConsts& C, int i, floa
     hash_particle(C, pp.x, pp.y, pp.z, pid, (uint32_t)i, keyI, valI, flags);
 }
 
-__global__ __launch_bounds__(256) void naive_force_kernel(
-    StepConsts C, const float4* __restrict__ pos, const float4* __restrict__ vel, const float4* __restrict__ aux,
-    const uint32_t* __restrict__ cs, float4* __restrict__ posOut, float4* __restrict__ velOut,
-    uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI, uint64_t* __restrict__ rng,
-    const PipeTable* __restrict__ pipe, float4* __restrict__ dbgP, float4* __restrict__ dbgV,
-    float4* __restrict__ dbgF, uint32_t* __restrict__ flags) {
-    const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= C.n) return;
-    const float4 p = pos[i], v = vel[i], a = aux[i];
+// Pressure and viscosity sums of one pair (naive_kernels.py:38-52, :70-77) without the
+// constant factors.  q = (x, y, z, p) and vj = (vx, vy, vz, 1/rho) of the neighbour.
+struct PairSums {
     float ax = 0.f, ay = 0.f, az = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
-    for_each_candidate(p.x, p.y, p.z, C, cs, [&](uint32_t k) {
-        const float4 q = pos[k];
+    // self = the pair is the particle itself: 1/r is taken as 0, which makes both terms zero
+    // (the pressure term carries a factor 1/r, the viscosity term a factor v_j - v_i = 0)
+    __device__ __forceinline__ void add(const StepConsts& C, float4 p, float4 v, float4 q, float4 vj,
+                                        bool self = false) {
         const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 <= C.h2 && k != (uint32_t)i) {
-            const float4 aj = aux[k];
-            const float4 vj = vel[k];
-            const float rinv = rsqrtf(r2);
-            const float hr = C.h - r2 * rinv;
-            const float cp = (a.y + aj.y) * aj.z * hr * hr * rinv;
-            ax += cp * dx;
-            ay += cp * dy;
-            az += cp * dz;
-            const float cv = hr * aj.z;
-            bx += cv * (vj.x - v.x);
-            by += cv * (vj.y - v.y);
-            bz += cv * (vj.z - v.z);
+        const float rinv = self ? 0.0f : rsqrtf(r2);
+        const float hr = C.h - r2 * rinv;
+        const float cp = (p.w + q.w) * vj.w * hr * hr * rinv;
+        ax += cp * dx;
+        ay += cp * dy;
+        az += cp * dz;
+        const float cv = hr * vj.w;
+        bx += cv * (vj.x - v.x);
+        by += cv * (vj.y - v.y);
+        bz += cv * (vj.z - v.z);
+    }
+};
+
+// Force pass (naive_kernels.py:24-80) fused with integration and collision: each lane
+// walks its particle's neighbour list (one coalesced load per slot, two slots in flight);
+// a particle with more neighbours than list slots rescans its candidates instead.
+__global__ __launch_bounds__(256) void naive_force_kernel(
+    StepConsts C, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
+    const float* __restrict__ rhoA, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ nl,
+    const uint32_t* __restrict__ nn, float4* __restrict__ posOut, float4* __restrict__ velOut,
+    uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
+    uint64_t* __restrict__ rng, const PipeTable* __restrict__ pipe, float4* __restrict__ dbgP,
+    float4* __restrict__ dbgV, float4* __restrict__ dbgF, uint32_t* __restrict__ flags) {
+    const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= C.n) return;
+    const __amdgpu_buffer_rsrc_t rpos = make_rsrc(pos, (uint32_t)C.n * 16u);
+    const __amdgpu_buffer_rsrc_t rvel = make_rsrc(vel, (uint32_t)C.n * 16u);
+    const float4 p = pos[i], v = vel[i];
+    PairSums S;
+    const uint32_t cnt = nn[i];
+    if (cnt <= (uint32_t)kListCap) {
+        const uint32_t* slot = nl + i;
+        const size_t stride = (size_t)C.n;
+        uint32_t t = 0;
+        for (; t + 1 < cnt; t += 2) {
+            const uint32_t k0 = slot[0], k1 = slot[stride];
+            slot += 2 * stride;
+            const float4 q0 = ld_f4(rpos, k0 * 16u), w0 = ld_f4(rvel, k0 * 16u);
+            const float4 q1 = ld_f4(rpos, k1 * 16u), w1 = ld_f4(rvel, k1 * 16u);
+            S.add(C, p, v, q0, w0, k0 == (uint32_t)i);
+            S.add(C, p, v, q1, w1, k1 == (uint32_t)i);
         }
-    });
+        if (t < cnt) {
+            const uint32_t k = slot[0];
+            S.add(C, p, v, ld_f4(rpos, k * 16u), ld_f4(rvel, k * 16u), k == (uint32_t)i);
+        }
+    } else {
+        for_each_candidate(p.x, p.y, p.z, C, cs, [&](uint32_t k) {
+            const float4 q = pos[k];
+            const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+            if (dx * dx + dy * dy + dz * dz <= C.h2 && k != (uint32_t)i) S.add(C, p, v, q, vel[k]);
+        });
+    }
     const float ps = 0.5f * C.mass * C.grad_c;   // naive_kernels.py:40-52: m * (pi+pj)/(2 rho_j) * grad W
     const float vs = C.mass * C.visc * C.lap_c;  // naive_kernels.py:71-77: m * mu * (vj-vi)/rho_j * lap W
-    finish_particle(C, i, p, v, a.x, Vec3{ps * ax, ps * ay, ps * az}, Vec3{vs * bx, vs * by, vs * bz}, posOut, velOut,
-                    keyI, valI, rng, pipe, dbgP, dbgV, dbgF, flags);
+    finish_particle(C, i, p, v, pidA[i], rhoA[i], Vec3{ps * S.ax, ps * S.ay, ps * S.az},
+                    Vec3{vs * S.bx, vs * S.by, vs * S.bz}, posOut, velOut, pidOut, keyI, valI, rng, pipe, dbgP, dbgV,
+                    dbgF, flags);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -405,9 +510,9 @@ __global__ __launch_bounds__(256) void naive_force_kernel(
 // ---------------------------------------------------------------------------------------
 constexpr int kNbrStride = 1 + kMaxNeighbours;
 
-__global__ __launch_bounds__(256) void voxel_density_kernel(StepConsts C, const float4* __restrict__ pos,
+__global__ __launch_bounds__(256) void voxel_density_kernel(StepConsts C, float4* pos, float4* vel,
                                                              const uint32_t* __restrict__ cs,
-                                                             float4* __restrict__ aux, int32_t* __restrict__ nbr) {
+                                                             float* __restrict__ rhoA, int32_t* __restrict__ nbr) {
     const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= C.n) return;
     const float4 p = pos[i];
@@ -440,47 +545,48 @@ __global__ __launch_bounds__(256) void voxel_density_kernel(StepConsts C, const 
         const float t = 1.0f - (ddx * ddx + ddy * ddy + ddz * ddz) * C.inv_h2;
         acc += t * t * t;
     }
-    const float rho = C.w_norm * acc;
-    aux[i] = make_float4(rho, C.k * (rho - C.rho0), 1.0f / rho, 0.0f);
+    store_density(C, i, acc, pos, vel, rhoA);
 }
 
 __global__ __launch_bounds__(256) void voxel_force_kernel(
-    StepConsts C, const float4* __restrict__ pos, const float4* __restrict__ vel, const float4* __restrict__ aux,
-    const int32_t* __restrict__ nbr, float4* __restrict__ posOut, float4* __restrict__ velOut,
-    uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI, uint64_t* __restrict__ rng,
+    StepConsts C, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
+    const float* __restrict__ rhoA, const int32_t* __restrict__ nbr, float4* __restrict__ posOut,
+    float4* __restrict__ velOut, uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI,
+    uint32_t* __restrict__ valI, uint64_t* __restrict__ rng,
     const PipeTable* __restrict__ pipe, float4* __restrict__ dbgP, float4* __restrict__ dbgV,
     float4* __restrict__ dbgF, uint32_t* __restrict__ flags) {
     const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= C.n) return;
-    const float4 p = pos[i], v = vel[i], a = aux[i];
+    const float4 p = pos[i], v = vel[i];
     const int32_t* list = nbr + (int64_t)i * kNbrStride;
     const int cnt = list[0];
-    const float rho_i = a.x;
-    const float fi = a.y / (rho_i * rho_i);
+    const float rho_i = rhoA[i];
+    const float fi = p.w / (rho_i * rho_i);
     float ax = 0.f, ay = 0.f, az = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
     for (int t = 0; t < cnt; ++t) {
         const int j = list[1 + t];
         if (j == i) continue;
-        const float4 q = pos[j], aj = aux[j], vj = vel[j];
+        const float4 q = pos[j], vj = vel[j];
+        const float rho_j = rhoA[j];
         const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         const float r = sqrtf(dx * dx + dy * dy + dz * dz);
         const float hr = C.h - r;
         // voxel_kernels.py:162-168: (p_i/rho_i^2 + p_j/rho_j^2) * grad W
-        const float cp = (fi + aj.y / (aj.x * aj.x)) * C.grad_c * hr * hr / r;
+        const float cp = (fi + q.w / (rho_j * rho_j)) * C.grad_c * hr * hr / r;
         ax += cp * dx;
         ay += cp * dy;
         az += cp * dz;
         // voxel_kernels.py:203-208: (v_j - v_i) / rho_j * lap W * m * mu / rho_i
         const float lap = C.lap_c * hr;
         const float s = C.mass * C.visc / rho_i;
-        bx += (vj.x - v.x) / aj.x * lap * s;
-        by += (vj.y - v.y) / aj.x * lap * s;
-        bz += (vj.z - v.z) / aj.x * lap * s;
+        bx += (vj.x - v.x) / rho_j * lap * s;
+        by += (vj.y - v.y) / rho_j * lap * s;
+        bz += (vj.z - v.z) / rho_j * lap * s;
     }
     // voxel_kernels.py:211 min(1, .)
     const Vec3 V{fminf(1.0f, bx), fminf(1.0f, by), fminf(1.0f, bz)};
-    finish_particle(C, i, p, v, rho_i, Vec3{ax, ay, az}, V, posOut, velOut, keyI, valI, rng, pipe, dbgP, dbgV, dbgF,
-                    flags);
+    finish_particle(C, i, p, v, pidA[i], rho_i, Vec3{ax, ay, az}, V, posOut, velOut, pidOut, keyI, valI, rng, pipe,
+                    dbgP, dbgV, dbgF, flags);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -488,13 +594,14 @@ __global__ __launch_bounds__(256) void voxel_force_kernel(
 // ---------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void download_f64_kernel(int n, const float4* __restrict__ posB,
                                                             const float4* __restrict__ velB,
+                                                            const uint32_t* __restrict__ pidB,
                                                             const float4* __restrict__ dbgF, double* __restrict__ pos,
                                                             double* __restrict__ vel, double* __restrict__ rho,
                                                             double* __restrict__ force) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 p = posB[i], v = velB[i];
-    const int64_t id = pid_of(v);
+    const int64_t id = pidB[i];
     if (pos) { pos[3 * id] = p.x; pos[3 * id + 1] = p.y; pos[3 * id + 2] = p.z; }
     if (vel) { vel[3 * id] = v.x; vel[3 * id + 1] = v.y; vel[3 * id + 2] = v.z; }
     if (rho) rho[id] = p.w;
@@ -502,12 +609,13 @@ __global__ __launch_bounds__(256) void download_f64_kernel(int n, const float4* 
 }
 
 __global__ __launch_bounds__(256) void download_f32_kernel(int n, const float4* __restrict__ posB,
-                                                            const float4* __restrict__ velB, float* x, float* y,
+                                                            const float4* __restrict__ velB,
+                                                            const uint32_t* __restrict__ pidB, float* x, float* y,
                                                             float* z, float* vx, float* vy, float* vz, float* rho) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 p = posB[i], v = velB[i];
-    const uint32_t id = pid_of(v);
+    const uint32_t id = pidB[i];
     if (x) x[id] = p.x;
     if (y) y[id] = p.y;
     if (z) z[id] = p.z;
@@ -523,9 +631,9 @@ __global__ __launch_bounds__(256) void read_buffer_kernel(StepConsts C, int whic
                                                            const uint32_t* __restrict__ skeys,
                                                            const uint32_t* __restrict__ svals,
                                                            const uint32_t* __restrict__ cs,
-                                                           const float4* __restrict__ velA,
-                                                           const float4* __restrict__ velB,
-                                                           const float4* __restrict__ aux, const float4* __restrict__ dbg,
+                                                           const uint32_t* __restrict__ pidA,
+                                                           const uint32_t* __restrict__ pidB,
+                                                           const float* __restrict__ rhoA, const float4* __restrict__ dbg,
                                                            const int32_t* __restrict__ nbr, void* out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (which == SPH_BUF_CELL_BEGIN) {
@@ -542,8 +650,7 @@ __global__ __launch_bounds__(256) void read_buffer_kernel(StepConsts C, int whic
             break;
         }
         case SPH_BUF_PERM: {
-            const float4 v = perm_from_A ? velA[i] : velB[svals[i]];
-            ((int32_t*)out)[i] = (int32_t)pid_of(v);
+            ((int32_t*)out)[i] = (int32_t)(perm_from_A ? pidA[i] : pidB[svals[i]]);
             break;
         }
         case SPH_BUF_SORTED_KEYS: {
@@ -552,12 +659,12 @@ __global__ __launch_bounds__(256) void read_buffer_kernel(StepConsts C, int whic
             break;
         }
         case SPH_BUF_DENSITY:
-            ((float*)out)[pid_of(velA[i])] = aux[i].x;
+            ((float*)out)[pidA[i]] = rhoA[i];
             break;
         case SPH_BUF_PRESSURE:
         case SPH_BUF_VISCOSITY:
         case SPH_BUF_FORCE: {
-            const int64_t id = pid_of(velA[i]);
+            const int64_t id = pidA[i];
             const float4 t = dbg[i];
             ((float*)out)[3 * id] = t.x;
             ((float*)out)[3 * id + 1] = t.y;
@@ -565,12 +672,12 @@ __global__ __launch_bounds__(256) void read_buffer_kernel(StepConsts C, int whic
             break;
         }
         case SPH_BUF_NEIGHBOURS: {
-            const int64_t id = pid_of(velA[i]);
+            const int64_t id = pidA[i];
             const int32_t* src = nbr + i * kNbrStride;
             int32_t* dst = (int32_t*)out + id * kNbrStride;
             const int cnt = src[0];
             dst[0] = cnt;
-            for (int t = 0; t < kMaxNeighbours; ++t) dst[1 + t] = t < cnt ? (int32_t)pid_of(velA[src[1 + t]]) : -1;
+            for (int t = 0; t < kMaxNeighbours; ++t) dst[1 + t] = t < cnt ? (int32_t)pidA[src[1 + t]] : -1;
             break;
         }
         default:
@@ -649,8 +756,8 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s) {
 
 hipError_t launch_upload_f64(sph_ctx* c, const double* pos, const double* vel, hipStream_t s) {
     if (c->n > 0)
-        upload_f64_kernel<<<blocks_for(c->n, 256), 256, 0, s>>>(c->sc, pos, vel, c->b.posB, c->b.velB, c->b.keyI,
-                                                                  c->b.valI, c->b.flags);
+        upload_f64_kernel<<<blocks_for(c->n, 256), 256, 0, s>>>(c->sc, pos, vel, c->b.posB, c->b.velB, c->b.pidB,
+                                                                  c->b.keyI, c->b.valI, c->b.flags);
     return hipGetLastError();
 }
 
@@ -658,7 +765,7 @@ hipError_t launch_upload_f32(sph_ctx* c, const float* x, const float* y, const f
                              const float* vy, const float* vz, hipStream_t s) {
     if (c->n > 0)
         upload_f32_kernel<<<blocks_for(c->n, 256), 256, 0, s>>>(c->sc, x, y, z, vx, vy, vz, c->b.posB, c->b.velB,
-                                                                  c->b.keyI, c->b.valI, c->b.flags);
+                                                                  c->b.pidB, c->b.keyI, c->b.valI, c->b.flags);
     return hipGetLastError();
 }
 
@@ -670,7 +777,7 @@ hipError_t launch_cell_table(sph_ctx* c, hipStream_t s) {
 hipError_t launch_reorder(sph_ctx* c, hipStream_t s) {
     if (c->n > 0)
         reorder_kernel<<<blocks_for(c->n, 256), 256, 0, s>>>((int)c->n, c->sorted_vals, c->b.posB, c->b.velB,
-                                                               c->b.posA, c->b.velA);
+                                                               c->b.pidB, c->b.posA, c->b.velA, c->b.pidA);
     return hipGetLastError();
 }
 
@@ -678,9 +785,11 @@ hipError_t launch_density(sph_ctx* c, hipStream_t s) {
     if (c->n == 0) return hipGetLastError();
     const int bs = c->density_block;
     if (c->sc.formulation == SPH_FORM_VOXEL)
-        voxel_density_kernel<<<blocks_for(c->n, bs), bs, 0, s>>>(c->sc, c->b.posA, c->b.cs, c->b.aux, c->b.nbr);
+        voxel_density_kernel<<<blocks_for(c->n, bs), bs, 0, s>>>(c->sc, c->b.posA, c->b.velA, c->b.cs, c->b.rhoA,
+                                                                   c->b.nbr);
     else
-        naive_density_kernel<<<blocks_for(c->n, bs), bs, 0, s>>>(c->sc, c->b.posA, c->b.cs, c->b.aux);
+        naive_density_kernel<<<blocks_for(c->n, bs), bs, 0, s>>>(c->sc, c->b.posA, c->b.velA, c->b.cs, c->b.rhoA,
+                                                                   c->b.nl, c->b.nn);
     return hipGetLastError();
 }
 
@@ -691,28 +800,30 @@ hipError_t launch_force(sph_ctx* c, hipStream_t s) {
     float4* dV = c->keep_terms ? c->b.dbgV : nullptr;
     float4* dF = c->keep_terms ? c->b.dbgF : nullptr;
     if (c->sc.formulation == SPH_FORM_VOXEL)
-        voxel_force_kernel<<<blocks_for(c->n, bs), bs, 0, s>>>(c->sc, c->b.posA, c->b.velA, c->b.aux, c->b.nbr,
-                                                                 c->b.posB, c->b.velB, c->b.keyI, c->b.valI, c->b.rng,
-                                                                 c->b.pipe, dP, dV, dF, c->b.flags);
+        voxel_force_kernel<<<blocks_for(c->n, bs), bs, 0, s>>>(c->sc, c->b.posA, c->b.velA, c->b.pidA, c->b.rhoA,
+                                                                 c->b.nbr, c->b.posB, c->b.velB, c->b.pidB, c->b.keyI,
+                                                                 c->b.valI, c->b.rng, c->b.pipe, dP, dV, dF,
+                                                                 c->b.flags);
     else
-        naive_force_kernel<<<blocks_for(c->n, bs), bs, 0, s>>>(c->sc, c->b.posA, c->b.velA, c->b.aux, c->b.cs,
-                                                                 c->b.posB, c->b.velB, c->b.keyI, c->b.valI, c->b.rng,
-                                                                 c->b.pipe, dP, dV, dF, c->b.flags);
+        naive_force_kernel<<<blocks_for(c->n, bs), bs, 0, s>>>(c->sc, c->b.posA, c->b.velA, c->b.pidA, c->b.rhoA,
+                                                                 c->b.cs, c->b.nl, c->b.nn, c->b.posB, c->b.velB,
+                                                                 c->b.pidB, c->b.keyI, c->b.valI, c->b.rng, c->b.pipe,
+                                                                 dP, dV, dF, c->b.flags);
     return hipGetLastError();
 }
 
 hipError_t launch_download_f64(sph_ctx* c, double* pos, double* vel, double* rho, double* force, hipStream_t s) {
     if (c->n > 0)
-        download_f64_kernel<<<blocks_for(c->n, 256), 256, 0, s>>>((int)c->n, c->b.posB, c->b.velB, c->b.dbgF, pos, vel,
-                                                                    rho, force);
+        download_f64_kernel<<<blocks_for(c->n, 256), 256, 0, s>>>((int)c->n, c->b.posB, c->b.velB, c->b.pidB,
+                                                                    c->b.dbgF, pos, vel, rho, force);
     return hipGetLastError();
 }
 
 hipError_t launch_download_f32(sph_ctx* c, float* x, float* y, float* z, float* vx, float* vy, float* vz, float* rho,
                                hipStream_t s) {
     if (c->n > 0)
-        download_f32_kernel<<<blocks_for(c->n, 256), 256, 0, s>>>((int)c->n, c->b.posB, c->b.velB, x, y, z, vx, vy, vz,
-                                                                    rho);
+        download_f32_kernel<<<blocks_for(c->n, 256), 256, 0, s>>>((int)c->n, c->b.posB, c->b.velB, c->b.pidB, x, y, z,
+                                                                    vx, vy, vz, rho);
     return hipGetLastError();
 }
 
@@ -722,8 +833,8 @@ hipError_t launch_read_buffer(sph_ctx* c, int which, void* out, hipStream_t s) {
     const int perm_from_A = c->last_stage >= SPH_STAGE_REORDER ? 1 : 0;
     if (count > 0)
         read_buffer_kernel<<<blocks_for(count, 256), 256, 0, s>>>(c->sc, which, perm_from_A, c->b.keyI, c->sorted_keys,
-                                                                    c->sorted_vals, c->b.cs, c->b.velA, c->b.velB,
-                                                                    c->b.aux, dbg, c->b.nbr, out);
+                                                                    c->sorted_vals, c->b.cs, c->b.pidA, c->b.pidB,
+                                                                    c->b.rhoA, dbg, c->b.nbr, out);
     return hipGetLastError();
 }
 

@@ -12,7 +12,7 @@
 //   downsweep each tile ranks its keys stably with wave64 ballots (the lanes holding the
 //             same digit are found with one ballot per digit bit), places them in LDS in
 //             digit order and writes each digit run contiguously.  On the last pass it can
-//             also gather the particle state into cell order (posB/velB -> posA/velA), so
+//             also gather the particle state into cell order (posB/velB/pidB -> A), so
 //             the separate reorder pass disappears.
 #include "sph_internal.h"
 
@@ -92,7 +92,8 @@ __global__ __launch_bounds__(ST) void radix_downsweep(const uint32_t* __restrict
                                                        int shift, int bits, const uint32_t* __restrict__ hist,
                                                        const uint32_t* __restrict__ digit_total, int ntiles,
                                                        const float4* __restrict__ posB, const float4* __restrict__ velB,
-                                                       float4* __restrict__ posA, float4* __restrict__ velA) {
+                                                       const uint32_t* __restrict__ pidB, float4* __restrict__ posA,
+                                                       float4* __restrict__ velA, uint32_t* __restrict__ pidA) {
     __shared__ uint32_t s_keys[TILE];
     __shared__ uint32_t s_vals[TILE];
     __shared__ uint32_t s_whist[WAVES][RADIX];
@@ -176,6 +177,7 @@ __global__ __launch_bounds__(ST) void radix_downsweep(const uint32_t* __restrict
         if (GATHER) {
             posA[g] = posB[vv];
             velA[g] = velB[vv];
+            pidA[g] = pidB[vv];
         }
     }
 }
@@ -205,10 +207,12 @@ hipError_t launch_radix_sort_impl(sph_ctx* c, bool gather, hipStream_t s) {
         radix_scan_digits<<<RADIX, ST, 0, s>>>(c->b.tile_hist, T, c->b.digit_total);
         if (gather && p == c->passes - 1)
             radix_downsweep<true><<<T, ST, 0, s>>>(kin, vin, kout, vout, n, shift, bits, c->b.tile_hist,
-                                                    c->b.digit_total, T, c->b.posB, c->b.velB, c->b.posA, c->b.velA);
+                                                    c->b.digit_total, T, c->b.posB, c->b.velB, c->b.pidB, c->b.posA,
+                                                    c->b.velA, c->b.pidA);
         else
             radix_downsweep<false><<<T, ST, 0, s>>>(kin, vin, kout, vout, n, shift, bits, c->b.tile_hist,
-                                                     c->b.digit_total, T, nullptr, nullptr, nullptr, nullptr);
+                                                     c->b.digit_total, T, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                     nullptr);
         kin = kout;
         vin = vout;
     }

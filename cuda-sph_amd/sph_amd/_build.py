@@ -27,6 +27,7 @@ HIP_FLAGS = [
     "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
+    "-Werror=array-bounds",
     "-Wno-unused-function",
 ]
 

@@ -12,7 +12,9 @@
   batch_64      64 box instances, N_b log-uniform in [2048, 65536], mu U[1, 8], h U[0.03, 0.06],
                 box edge U[0.5, 2]; particles uniform in [0.1, 0.5]^3 * edge.
   adjoint_16k   16,384 particles as box_4k.
-Voxel edges are powers of two no smaller than h, so voxel keys are exact in FP32.
+Voxel edges are the largest power of two not above h (exact keys in FP32; each axis
+searches ceil(h / voxel) = 2 cells on each side, which tests fewer candidates than cells
+of edge >= h).  The NAIVE formulation's results do not depend on the voxel edge.
 """
 from __future__ import annotations
 
@@ -31,6 +33,11 @@ BOX_CONSTANTS = dict(mass=0.02, rho0=1000.0, k=3.0, visc=3.5, damp=0.5)
 def pow2_at_least(x: float) -> float:
     """Smallest power of two >= x."""
     return 2.0 ** math.ceil(math.log2(x))
+
+
+def voxel_for(h: float) -> float:
+    """Largest power of two <= h: the voxel edge used by the workloads."""
+    return 2.0 ** math.floor(math.log2(h))
 
 
 @dataclass
@@ -80,7 +87,7 @@ def dam_break(nx: int = 64, ny: int = 128, nz: int = 128, seed: int = 1, steps: 
     if h is None:
         spacing = (sx * sy * sz) ** (1.0 / 3.0)
         h = spacing * (30.0 / (4.0 / 3.0 * math.pi)) ** (1.0 / 3.0)
-    vx = pow2_at_least(h)
+    vx = voxel_for(h)
     c = SPHConstants(h=h, **BOX_CONSTANTS)
     p = _params(n, (0.0, -9.81, 0.0), 1000, (2.0, 1.0, 1.0), (vx, vx, vx))
     return Workload(f"dam_break_{n}", p, c, SimulationState(pos, np.zeros((n, 3)), np.zeros(n)), steps)
@@ -129,7 +136,7 @@ def pipe_flow(n: int = 524288, seed: int = 2, steps: int = 1000, segments: int =
     vel += rng.normal(0.0, 0.05, size=(n, 3))
     spacing = (vol / n) ** (1.0 / 3.0)
     h = spacing * (neighbours / (4.0 / 3.0 * math.pi)) ** (1.0 / 3.0)
-    vx = pow2_at_least(h)
+    vx = voxel_for(h)
     c = SPHConstants(h=h, **BOX_CONSTANTS)
     p = _params(n, (2.0, 0.0, 0.0), 1000, (segments * seg_len, 2 * axis_yz, 2 * axis_yz), (vx, vx, vx), pipe=pipe)
     return Workload(f"pipe_{n}", p, c, SimulationState(pos, vel, np.zeros(n)), steps, mode="PIPE",
