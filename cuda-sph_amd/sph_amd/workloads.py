@@ -64,7 +64,7 @@ def box_uniform(n: int, seed: int = 0, h: float = 0.0457, edge: float = 1.0, vis
     pos = rng.uniform(0.1 * edge, 0.5 * edge, size=(n, 3))
     vel = np.zeros((n, 3))
     c = SPHConstants(h=h, **{**BOX_CONSTANTS, "visc": visc})
-    vx = pow2_at_least(h)
+    vx = voxel_for(h)
     p = _params(n, (0.0, -9.81, 0.0), 1000, (edge, edge, edge), (vx, vx, vx))
     return Workload(f"box_uniform_n{n}_seed{seed}", p, c, SimulationState(pos, vel, np.zeros(n)), steps)
 
