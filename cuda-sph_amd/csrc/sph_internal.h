@@ -27,7 +27,10 @@ namespace sph {
 constexpr int kMaxPipeRows = 64;
 constexpr int kMaxNeighbours = 32;  // MAX_NEIGHBOURS, config.py:30
 constexpr int kListCap = 96;        // NAIVE neighbour-list slots per particle; more falls back to a rescan
-constexpr int kLdsListCap = 32;     // of which staged in LDS by the density pass
+#ifndef SPH_LDS_LIST_CAP
+#define SPH_LDS_LIST_CAP 16
+#endif
+constexpr int kLdsListCap = SPH_LDS_LIST_CAP;  // of which staged in LDS by the density pass
 constexpr int kXcds = 8;            // MI355X: 8 accelerator complex dies, one L2 each
 
 // Everything a step kernel needs, passed by value (fits the kernel-argument segment).

@@ -224,6 +224,10 @@ __device__ __forceinline__ void store_density(const StepConsts& C, int i, float 
 // self pair a zero contribution).  (1 - r^2/h^2)^3 is accumulated without a branch and is
 // zero outside h.  Candidates are read two at a time through 32-bit buffer offsets.
 constexpr int kDensityThreads = 256;
+#ifndef SPH_DENSITY_UNROLL
+#define SPH_DENSITY_UNROLL 2
+#endif
+constexpr int kDensityUnroll = SPH_DENSITY_UNROLL;
 
 __global__ __launch_bounds__(kDensityThreads) void naive_density_kernel(StepConsts C, float4* pos, float4* vel,
                                                                          const uint32_t* __restrict__ cs,
@@ -266,12 +270,14 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_kernel(StepCons
             const uint32_t row = (uint32_t)(C.dims[0] * (cy + C.dims[1] * cz));
             uint32_t off = ld_u32(rcs, (row + (uint32_t)xlo) * 4u) * 16u;
             const uint32_t end = ld_u32(rcs, (row + (uint32_t)xhi + 1u) * 4u) * 16u;
-            for (; off + 16u < end; off += 32u) {
-                const float4 q0 = ld_f4(rpos, off), q1 = ld_f4(rpos, off + 16u);
-                visit(q0, off);
-                visit(q1, off + 16u);
+            for (; off + 16u * (kDensityUnroll - 1) < end; off += 16u * kDensityUnroll) {
+                float4 q[kDensityUnroll];
+#pragma unroll
+                for (int u = 0; u < kDensityUnroll; ++u) q[u] = ld_f4(rpos, off + 16u * u);
+#pragma unroll
+                for (int u = 0; u < kDensityUnroll; ++u) visit(q[u], off + 16u * u);
             }
-            if (off < end) visit(ld_f4(rpos, off), off);
+            for (; off < end; off += 16u) visit(ld_f4(rpos, off), off);
         }
     }
     nn[i] = cnt;
@@ -402,6 +408,7 @@ __device__ void collide_pipe(const StepConsts& C, const PipeTable& T, Vec3& p, V
 
 // Symplectic Euler (base_kernels.py:30-53), collision, output in the same sorted slot of
 // posB/velB, hash of the new position for the next step.
+template <int MODE>
 __device__ __forceinline__ void finish_particle(const StepConsts& C, int i, float4 p, float4 v, uint32_t pid,
                                                 float rho, Vec3 P, Vec3 V, float4* __restrict__ posOut,
                                                 float4* __restrict__ velOut, uint32_t* __restrict__ pidOut,
@@ -414,7 +421,7 @@ __device__ __forceinline__ void finish_particle(const StepConsts& C, int i, floa
     const float fz = C.ext[2] + -P.z + V.z;
     Vec3 vv{v.x + fx / rho * C.dt, v.y + fy / rho * C.dt, v.z + fz / rho * C.dt};
     Vec3 pp{p.x + vv.x * C.dt, p.y + vv.y * C.dt, p.z + vv.z * C.dt};
-    if (C.mode == SPH_MODE_PIPE) {
+    if (MODE == SPH_MODE_PIPE) {
         collide_pipe(C, *pipe, pp, vv, rng, pid);
     } else {
         collide_box(C, pp.x, vv.x, 0);
@@ -456,8 +463,15 @@ struct PairSums {
 };
 
 // Force pass (naive_kernels.py:24-80) fused with integration and collision: each lane
-// walks its particle's neighbour list (one coalesced load per slot, two slots in flight);
-// a particle with more neighbours than list slots rescans its candidates instead.
+// walks its particle's neighbour list (one coalesced load per slot, kForceUnroll slots and
+// their gathers in flight); a particle with more neighbours than list slots rescans its
+// candidates instead.  MODE selects the wall collision at compile time.
+#ifndef SPH_FORCE_UNROLL
+#define SPH_FORCE_UNROLL 4
+#endif
+constexpr int kForceUnroll = SPH_FORCE_UNROLL;
+
+template <int MODE>
 __global__ __launch_bounds__(256) void naive_force_kernel(
     StepConsts C, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
     const float* __restrict__ rhoA, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ nl,
@@ -476,16 +490,23 @@ __global__ __launch_bounds__(256) void naive_force_kernel(
         const uint32_t* slot = nl + i;
         const size_t stride = (size_t)C.n;
         uint32_t t = 0;
-        for (; t + 1 < cnt; t += 2) {
-            const uint32_t k0 = slot[0], k1 = slot[stride];
-            slot += 2 * stride;
-            const float4 q0 = ld_f4(rpos, k0 * 16u), w0 = ld_f4(rvel, k0 * 16u);
-            const float4 q1 = ld_f4(rpos, k1 * 16u), w1 = ld_f4(rvel, k1 * 16u);
-            S.add(C, p, v, q0, w0, k0 == (uint32_t)i);
-            S.add(C, p, v, q1, w1, k1 == (uint32_t)i);
+        for (; t + kForceUnroll <= cnt; t += kForceUnroll) {
+            uint32_t k[kForceUnroll];
+            float4 q[kForceUnroll], w[kForceUnroll];
+#pragma unroll
+            for (int u = 0; u < kForceUnroll; ++u) k[u] = slot[u * stride];
+            slot += kForceUnroll * stride;
+#pragma unroll
+            for (int u = 0; u < kForceUnroll; ++u) {
+                q[u] = ld_f4(rpos, k[u] * 16u);
+                w[u] = ld_f4(rvel, k[u] * 16u);
+            }
+#pragma unroll
+            for (int u = 0; u < kForceUnroll; ++u) S.add(C, p, v, q[u], w[u], k[u] == (uint32_t)i);
         }
-        if (t < cnt) {
+        for (; t < cnt; ++t) {
             const uint32_t k = slot[0];
+            slot += stride;
             S.add(C, p, v, ld_f4(rpos, k * 16u), ld_f4(rvel, k * 16u), k == (uint32_t)i);
         }
     } else {
@@ -497,9 +518,9 @@ __global__ __launch_bounds__(256) void naive_force_kernel(
     }
     const float ps = 0.5f * C.mass * C.grad_c;   // naive_kernels.py:40-52: m * (pi+pj)/(2 rho_j) * grad W
     const float vs = C.mass * C.visc * C.lap_c;  // naive_kernels.py:71-77: m * mu * (vj-vi)/rho_j * lap W
-    finish_particle(C, i, p, v, pidA[i], rhoA[i], Vec3{ps * S.ax, ps * S.ay, ps * S.az},
-                    Vec3{vs * S.bx, vs * S.by, vs * S.bz}, posOut, velOut, pidOut, keyI, valI, rng, pipe, dbgP, dbgV,
-                    dbgF, flags);
+    finish_particle<MODE>(C, i, p, v, pidA[i], rhoA[i], Vec3{ps * S.ax, ps * S.ay, ps * S.az},
+                          Vec3{vs * S.bx, vs * S.by, vs * S.bz}, posOut, velOut, pidOut, keyI, valI, rng, pipe, dbgP,
+                          dbgV, dbgF, flags);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -548,6 +569,7 @@ __global__ __launch_bounds__(256) void voxel_density_kernel(StepConsts C, float4
     store_density(C, i, acc, pos, vel, rhoA);
 }
 
+template <int MODE>
 __global__ __launch_bounds__(256) void voxel_force_kernel(
     StepConsts C, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
     const float* __restrict__ rhoA, const int32_t* __restrict__ nbr, float4* __restrict__ posOut,
@@ -585,7 +607,7 @@ __global__ __launch_bounds__(256) void voxel_force_kernel(
     }
     // voxel_kernels.py:211 min(1, .)
     const Vec3 V{fminf(1.0f, bx), fminf(1.0f, by), fminf(1.0f, bz)};
-    finish_particle(C, i, p, v, pidA[i], rho_i, Vec3{ax, ay, az}, V, posOut, velOut, pidOut, keyI, valI, rng, pipe,
+    finish_particle<MODE>(C, i, p, v, pidA[i], rho_i, Vec3{ax, ay, az}, V, posOut, velOut, pidOut, keyI, valI, rng, pipe,
                     dbgP, dbgV, dbgF, flags);
 }
 
@@ -793,22 +815,26 @@ hipError_t launch_density(sph_ctx* c, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_force(sph_ctx* c, hipStream_t s) {
-    if (c->n == 0) return hipGetLastError();
+template <int MODE>
+void launch_force_mode(sph_ctx* c, hipStream_t s) {
     const int bs = c->force_block;
     float4* dP = c->keep_terms ? c->b.dbgP : nullptr;
     float4* dV = c->keep_terms ? c->b.dbgV : nullptr;
     float4* dF = c->keep_terms ? c->b.dbgF : nullptr;
     if (c->sc.formulation == SPH_FORM_VOXEL)
-        voxel_force_kernel<<<blocks_for(c->n, bs), bs, 0, s>>>(c->sc, c->b.posA, c->b.velA, c->b.pidA, c->b.rhoA,
-                                                                 c->b.nbr, c->b.posB, c->b.velB, c->b.pidB, c->b.keyI,
-                                                                 c->b.valI, c->b.rng, c->b.pipe, dP, dV, dF,
-                                                                 c->b.flags);
+        voxel_force_kernel<MODE><<<blocks_for(c->n, bs), bs, 0, s>>>(
+            c->sc, c->b.posA, c->b.velA, c->b.pidA, c->b.rhoA, c->b.nbr, c->b.posB, c->b.velB, c->b.pidB, c->b.keyI,
+            c->b.valI, c->b.rng, c->b.pipe, dP, dV, dF, c->b.flags);
     else
-        naive_force_kernel<<<blocks_for(c->n, bs), bs, 0, s>>>(c->sc, c->b.posA, c->b.velA, c->b.pidA, c->b.rhoA,
-                                                                 c->b.cs, c->b.nl, c->b.nn, c->b.posB, c->b.velB,
-                                                                 c->b.pidB, c->b.keyI, c->b.valI, c->b.rng, c->b.pipe,
-                                                                 dP, dV, dF, c->b.flags);
+        naive_force_kernel<MODE><<<blocks_for(c->n, bs), bs, 0, s>>>(
+            c->sc, c->b.posA, c->b.velA, c->b.pidA, c->b.rhoA, c->b.cs, c->b.nl, c->b.nn, c->b.posB, c->b.velB,
+            c->b.pidB, c->b.keyI, c->b.valI, c->b.rng, c->b.pipe, dP, dV, dF, c->b.flags);
+}
+
+hipError_t launch_force(sph_ctx* c, hipStream_t s) {
+    if (c->n == 0) return hipGetLastError();
+    if (c->sc.mode == SPH_MODE_PIPE) launch_force_mode<SPH_MODE_PIPE>(c, s);
+    else launch_force_mode<SPH_MODE_BOX>(c, s);
     return hipGetLastError();
 }
 
