@@ -60,6 +60,9 @@ def parse() -> argparse.Namespace:
                     help="threads per block of the two neighbour kernels")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched", choices=["on", "off"], default="on",
+                    help="also measure the batched form (64 box simulations, 300 steps, HIP graph)")
+    ap.add_argument("--batch-steps", type=int, default=300)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain device copy of known size (for counter calibration)")
     return ap.parse_args()
@@ -152,6 +155,55 @@ def cpu_baseline(w, budget_s: float):
                       f"step (NAIVE formulation through a grid search), OpenMP, {el:.1f} s"}
 
 
+def bench_batched(steps: int, warmup: int = 3):
+    """Aggregate particle-updates/s of the batch_64 workload: one launch per stage for all 64
+    instances, the timed steps replayed as one HIP graph; against the same 64 simulations
+    advanced one after another with the plain step (same kernels, one context each)."""
+    import numpy as np
+    import torch
+
+    from sph_amd import workloads
+    from sph_amd.batched import BatchedSPH
+    from sph_amd.strategy import NaiveSPHStrategy
+
+    bw = workloads.batch_64(steps=steps)
+    total = bw.total_particles
+    states = [w.state for w in bw.instances]
+    batch = BatchedSPH([(w.params, w.constants) for w in bw.instances])
+    batch.upload(states)
+    graph = batch.capture(steps)  # recorded, not run
+    batch.upload(states)
+    batch.step(warmup)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    graph.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    t_batch = e0.elapsed_time(e1) / 1e3
+    log(f"batched: {steps} steps of {total} particles in {t_batch * 1e3:.1f} ms")
+    t_seq = 0.0
+    for w in bw.instances:
+        strat = NaiveSPHStrategy(w.params, constants=w.constants)
+        strat.ctx.upload(torch.from_numpy(np.ascontiguousarray(w.state.position)).cuda(),
+                         torch.from_numpy(np.ascontiguousarray(w.state.velocity)).cuda())
+        strat.ctx.step(warmup)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        strat.ctx.step(steps)
+        b.record()
+        torch.cuda.synchronize()
+        t_seq += a.elapsed_time(b) / 1e3
+        del strat
+    log(f"sequential plain runs: {t_seq * 1e3:.1f} ms")
+    return {"workload": "batch_64", "instances": len(bw.instances), "particles": total, "steps": steps,
+            "value": round(total * steps / t_batch, 1), "unit": "particle-updates/s",
+            "ms_per_step": round(t_batch * 1e3 / steps, 4), "graph": True,
+            "sequential_plain_value": round(total * steps / t_seq, 1),
+            "speedup_vs_sequential": round(t_seq / t_batch, 3)}
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -228,6 +280,8 @@ def main() -> int:
         torch.cuda.synchronize()
         log("copy calibration: 5 copies of 256 MiB")
 
+    batched = bench_batched(args.batch_steps) if args.batched == "on" else None
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
@@ -243,6 +297,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stages_ms": {k: round(v, 4) for k, v in st.items()},
+        "batched": batched,
         "wall_ms_per_step": round(wall * 1e3 / max(args.steps, 1), 5),
         "final_state": {"non_finite": diag["non_finite"], "max_speed": float(diag["max_speed"]),
                         "mean_density": float(diag["mean_density"])},

@@ -80,7 +80,7 @@ size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 // Workspace slices, in order.
 enum Slice {
     S_POSA, S_VELA, S_POSB, S_VELB, S_PIDA, S_PIDB, S_RHOA, S_KEYI, S_VALI, S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_CS,
-    S_HIST, S_DTOT, S_RNG, S_DBGP, S_DBGV, S_DBGF, S_NBR, S_NL, S_NN, S_DIAG, S_FLAGS, S_PIPE, S_COUNT
+    S_HIST, S_DTOT, S_RNG, S_DBGP, S_DBGV, S_DBGF, S_NBR, S_NL, S_NN, S_DIAG, S_FLAGS, S_PIPE, S_INST, S_INSTF, S_COUNT
 };
 
 struct Layout {
@@ -105,6 +105,8 @@ Layout plan_layout(const sph_ctx* c) {
     sz[S_DIAG] = 8 * 8 * 512;
     sz[S_FLAGS] = 64;
     sz[S_PIPE] = sizeof(sph::PipeTable);
+    sz[S_INST] = sizeof(sph::StepConsts) * (c->inst.empty() ? 1 : c->inst.size());
+    sz[S_INSTF] = sizeof(int) * (c->inst_first.empty() ? 1 : c->inst_first.size());
     Layout L;
     size_t o = 0;
     for (int i = 0; i < S_COUNT; ++i) {
@@ -115,9 +117,8 @@ Layout plan_layout(const sph_ctx* c) {
     return L;
 }
 
-int plan(sph_ctx* c) {
-    const sph_params_t& P = c->prm;
-    const sph_constants_t& K = c->cst;
+// Validates one (params, constants) pair and fills its kernel constants and pipe table.
+int plan_instance(const sph_params_t& P, const sph_constants_t& K, sph::StepConsts& S, sph::PipeTable& pipe) {
     // 2^27 particles keep every byte offset of a 16-byte-per-particle array within 31 bits
     if (P.particle_count < 0 || P.particle_count > (int64_t(1) << 27))
         return fail(SPH_E_INVALID, "particle_count %lld outside [0, 2^27]", (long long)P.particle_count);
@@ -127,15 +128,13 @@ int plan(sph_ctx* c) {
     if (P.mode != SPH_MODE_BOX && P.mode != SPH_MODE_PIPE) return fail(SPH_E_INVALID, "unknown mode %d", P.mode);
     if (P.formulation != SPH_FORM_NAIVE && P.formulation != SPH_FORM_VOXEL)
         return fail(SPH_E_INVALID, "unknown formulation %d", P.formulation);
-    c->n = P.particle_count;
-    c->keep_terms = (P.flags & SPH_FLAG_KEEP_TERMS) ? 1 : 0;
-    sph::StepConsts& S = c->sc;
     memset(&S, 0, sizeof(S));
-    S.n = (int)c->n;
+    S.n = (int)P.particle_count;
     S.mode = P.mode;
     S.formulation = P.formulation;
     S.ref_keys = P.formulation == SPH_FORM_VOXEL ? 1 : 0;
     S.pow2 = 1;
+    S.n_inst = 1;
     int64_t cells = 1;
     for (int d = 0; d < 3; ++d) {
         const double sp = P.space_size[d], vx = P.voxel_size[d];
@@ -159,6 +158,7 @@ int plan(sph_ctx* c) {
     }
     if (cells > (int64_t(1) << 28)) return fail(SPH_E_INVALID, "voxel grid of %lld cells is too large", (long long)cells);
     S.n_cells = (int)cells;
+    S.cs_cells = (int)cells;
     S.box_lo_target = 1e-3f;
     S.h = (float)K.h;
     S.h2 = (float)(K.h * K.h);
@@ -175,21 +175,7 @@ int plan(sph_ctx* c) {
     S.grad_c = (float)(-45.0 / (M_PI * pow(K.h, 6.0)));
     S.lap_c = (float)(45.0 / (M_PI * pow(K.h, 6.0)));
     S.wc9 = (float)(315.0 / (64.0 * M_PI * pow(K.h, 9.0)));
-    // sort plan
-    c->key_bits = S.ref_keys ? 32 : bit_length((uint64_t)(cells - 1));
-    c->passes = (c->key_bits + 7) / 8;
-    {
-        int shift = 0;
-        for (int p = 0; p < c->passes; ++p) {
-            int bits = (c->key_bits - shift + (c->passes - p) - 1) / (c->passes - p);
-            c->pass_shift[p] = shift;
-            c->pass_bits[p] = bits;
-            shift += bits;
-        }
-    }
-    c->num_tiles = sph::radix_num_tiles(c->n);
-    // pipe table
-    memset(&c->pipe_host, 0, sizeof(c->pipe_host));
+    memset(&pipe, 0, sizeof(pipe));
     S.pipe_rows = 0;
     if (P.mode == SPH_MODE_PIPE) {
         if (P.pipe_rows < 2 || P.pipe_rows > sph::kMaxPipeRows || !P.pipe)
@@ -197,14 +183,84 @@ int plan(sph_ctx* c) {
         S.pipe_rows = P.pipe_rows;
         double xb = P.pipe[0];
         for (int r = 0; r < P.pipe_rows; ++r) {
-            for (int k = 0; k < 5; ++k) c->pipe_host.row[r][k] = (float)P.pipe[r * 5 + k];
-            c->pipe_host.xbeg[r] = (float)xb;  // util_kernels.py:38-43, summed in FP64
+            for (int k = 0; k < 5; ++k) pipe.row[r][k] = (float)P.pipe[r * 5 + k];
+            pipe.xbeg[r] = (float)xb;  // util_kernels.py:38-43, summed in FP64
             xb += P.pipe[r * 5 + 4];
         }
     }
+    return SPH_OK;
+}
+
+// Sort plan and workspace size from c->sc (totals) and c->n.
+void plan_sort_and_layout(sph_ctx* c) {
+    const int64_t cells = c->sc.cs_cells;
+    c->key_bits = c->sc.ref_keys ? 32 : bit_length((uint64_t)(cells > 1 ? cells - 1 : 0));
+    c->passes = (c->key_bits + 7) / 8;
+    int shift = 0;
+    for (int p = 0; p < c->passes; ++p) {
+        const int bits = (c->key_bits - shift + (c->passes - p) - 1) / (c->passes - p);
+        c->pass_shift[p] = shift;
+        c->pass_bits[p] = bits;
+        shift += bits;
+    }
+    c->num_tiles = sph::radix_num_tiles(c->n);
     if (c->density_block <= 0) c->density_block = 256;
     if (c->force_block <= 0) c->force_block = 256;
     c->ws_bytes = plan_layout(c).total;
+}
+
+int plan(sph_ctx* c) {
+    int rc = plan_instance(c->prm, c->cst, c->sc, c->pipe_host);
+    if (rc) return rc;
+    c->n = c->prm.particle_count;
+    c->keep_terms = (c->prm.flags & SPH_FLAG_KEEP_TERMS) ? 1 : 0;
+    plan_sort_and_layout(c);
+    return SPH_OK;
+}
+
+// Batched form: instances are concatenated in the given order; instance b owns particle ids
+// [first_b, first_b + N_b) and cell keys [cell_off_b, cell_off_b + C_b).  All instances use
+// the NAIVE formulation and box walls.
+int plan_batch(sph_ctx* c, const sph_params_t* params, const sph_constants_t* consts, int count) {
+    if (count < 1 || count > 4096) return fail(SPH_E_INVALID, "batch of %d instances (1..4096)", count);
+    c->inst.resize(count);
+    c->inst_first.resize(count + 1);
+    c->inst_cst.assign(consts, consts + count);
+    int64_t n = 0, cells = 0;
+    for (int b = 0; b < count; ++b) {
+        if (params[b].formulation != SPH_FORM_NAIVE || params[b].mode != SPH_MODE_BOX)
+            return fail(SPH_E_INVALID, "batched instances use the NAIVE formulation in BOX mode (instance %d)", b);
+        sph::PipeTable unused;
+        int rc = plan_instance(params[b], consts[b], c->inst[b], unused);
+        if (rc) {
+            char why[sizeof(g_err)];
+            snprintf(why, sizeof(why), "%s", g_err);
+            return fail(rc, "instance %d: %s", b, why);
+        }
+        c->inst[b].first = (int)n;
+        c->inst[b].cell_off = (int)cells;
+        c->inst_first[b] = (int)n;
+        n += params[b].particle_count;
+        cells += c->inst[b].n_cells;
+        if (n > (int64_t(1) << 27)) return fail(SPH_E_INVALID, "batch of %lld particles is too large", (long long)n);
+        if (cells > (int64_t(1) << 28)) return fail(SPH_E_INVALID, "batch grid of %lld cells is too large", (long long)cells);
+    }
+    c->inst_first[count] = (int)n;
+    c->prm = params[0];
+    c->prm.pipe = nullptr;
+    c->prm.particle_count = n;
+    c->cst = consts[0];
+    c->sc = c->inst[0];
+    c->sc.n = (int)n;
+    c->sc.n_cells = (int)cells;
+    c->sc.cs_cells = (int)cells;
+    c->sc.cell_off = 0;
+    c->sc.first = 0;
+    c->sc.n_inst = count;
+    c->n = n;
+    c->keep_terms = (params[0].flags & SPH_FLAG_KEEP_TERMS) ? 1 : 0;
+    memset(&c->pipe_host, 0, sizeof(c->pipe_host));
+    plan_sort_and_layout(c);
     return SPH_OK;
 }
 
@@ -265,6 +321,31 @@ int sph_create(const sph_params_t* params, const sph_constants_t* consts, sph_ct
     return SPH_OK;
 }
 
+int sph_create_batch(const sph_params_t* params, const sph_constants_t* consts, int32_t count, sph_ctx** out) {
+    if (!params || !consts || !out) return fail(SPH_E_INVALID, "null argument");
+    *out = nullptr;
+    sph_ctx* c = new (std::nothrow) sph_ctx();
+    if (!c) return fail(SPH_E_INVALID, "out of host memory");
+    int rc = plan_batch(c, params, consts, count);
+    if (rc != SPH_OK) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return SPH_OK;
+}
+
+int sph_batch_info(const sph_ctx* c, int32_t* count, int32_t* first) {
+    if (!c) return fail(SPH_E_INVALID, "null context");
+    const int nb = c->inst.empty() ? 1 : (int)c->inst.size();
+    if (count) *count = nb;
+    if (first) {
+        if (c->inst.empty()) { first[0] = 0; first[1] = (int)c->n; }
+        else for (int b = 0; b <= nb; ++b) first[b] = c->inst_first[b];
+    }
+    return SPH_OK;
+}
+
 int sph_destroy(sph_ctx* ctx) {
     if (!ctx) return SPH_OK;
     delete[] ctx->rng_host;
@@ -308,6 +389,8 @@ int sph_bind_workspace(sph_ctx* c, void* dev, size_t bytes, void* stream) {
     b.diag_partials = (double*)(base + L.off[S_DIAG]);
     b.flags = (uint32_t*)(base + L.off[S_FLAGS]);
     b.pipe = (sph::PipeTable*)(base + L.off[S_PIPE]);
+    b.inst_table = (sph::StepConsts*)(base + L.off[S_INST]);
+    b.inst_first = (int*)(base + L.off[S_INSTF]);
     c->ws = dev;
     c->sorted_keys = b.keyI;
     c->sorted_vals = b.valI;
@@ -320,6 +403,20 @@ int sph_bind_workspace(sph_ctx* c, void* dev, size_t bytes, void* stream) {
         if ((rc = hip_check(hipMemcpyAsync(b.rng, c->rng_host, 16 * (size_t)c->n, hipMemcpyHostToDevice, s),
                             "upload rng streams")))
             return rc;
+    if (!c->inst.empty()) {
+        c->sc.inst_table = b.inst_table;
+        c->sc.inst_first = b.inst_first;
+        for (auto& st : c->inst) {
+            st.inst_table = b.inst_table;
+            st.inst_first = b.inst_first;
+        }
+        if ((rc = hip_check(hipMemcpyAsync(b.inst_table, c->inst.data(), sizeof(sph::StepConsts) * c->inst.size(),
+                                           hipMemcpyHostToDevice, s), "upload instance table")))
+            return rc;
+        if ((rc = hip_check(hipMemcpyAsync(b.inst_first, c->inst_first.data(), sizeof(int) * c->inst_first.size(),
+                                           hipMemcpyHostToDevice, s), "upload instance offsets")))
+            return rc;
+    }
     if ((rc = hip_check(hipStreamSynchronize(s), "bind"))) return rc;
     c->bound = true;
     c->have_state = false;
@@ -449,6 +546,7 @@ int sph_diagnostics(sph_ctx* c, double* out8, void* stream) {
     int rc = ready(c, true);
     if (rc) return rc;
     if (!out8) return fail(SPH_E_INVALID, "null output");
+    if (!c->inst.empty()) return fail(SPH_E_STATE, "diagnostics are per simulation; a batch mixes particle masses");
     return hip_check(sph::launch_diagnostics(c, out8, (hipStream_t)stream), "diagnostics");
 }
 

@@ -59,6 +59,14 @@ struct StepConsts {
     float box_lo_target;     // 1e-3                  (base_kernels.py:91)
     float box_hi_target[3];  // space - 1e-3          (base_kernels.py:94)
     int pipe_rows;
+    // batched form: this instance's first cell key and first particle (0 for a plain step);
+    // the kernel argument holds the totals, the device table one entry per instance
+    int cell_off;
+    int first;
+    int cs_cells;                          // cells covered by the cell table (all instances)
+    int n_inst;                            // 1 for a plain step
+    const struct StepConsts* inst_table;   // device, n_inst entries (batched form)
+    const int* inst_first;                 // device, n_inst + 1 particle offsets (batched form)
 };
 
 struct PipeTable {
@@ -83,9 +91,13 @@ struct Buffers {
     double* diag_partials;
     uint32_t* flags;       // [0] count of non-finite positions seen by the hash
     PipeTable* pipe;
+    StepConsts* inst_table;
+    int* inst_first;
 };
 
 }  // namespace sph
+
+#include <vector>
 
 struct sph_ctx {
     sph_params_t prm;
@@ -109,6 +121,10 @@ struct sph_ctx {
     bool have_state;
     int last_stage;          // last stage run in the current step (0 = hashed, ready)
     uint64_t* rng_host;      // seeded on the host, uploaded at bind
+    // batched form (empty for a plain step)
+    std::vector<sph::StepConsts> inst;
+    std::vector<int> inst_first;
+    std::vector<sph_constants_t> inst_cst;
 };
 
 namespace sph {

@@ -26,6 +26,25 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
     return x < r ? x * (q + 1) + local : r * (q + 1) + (x - r) * q + local;
 }
 
+// Constants of the instance that owns particle slot i.  A plain step has one instance (the
+// kernel argument itself).  In the batched form every instance owns a contiguous range of
+// particle ids and, because its cell keys form a contiguous range too, the same range of
+// cell-ordered slots; the range is found by binary search over the instance offsets.
+template <bool BATCH>
+__device__ __forceinline__ const StepConsts& inst_consts(const StepConsts& G, int i) {
+    if constexpr (!BATCH) {
+        return G;
+    } else {
+        int lo = 0, hi = G.n_inst;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (G.inst_first[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        return G.inst_table[lo];
+    }
+}
+
 
 // Raw buffer loads with 32-bit byte offsets: no 64-bit address arithmetic per access, and
 // an offset past num_records reads zero instead of faulting.  Word 3 of the descriptor is
@@ -69,7 +88,8 @@ __device__ __forceinline__ uint32_t sort_key(float x, float y, float z, const St
         uint32_t key = vx + vy * (uint32_t)C.dims[0] + vz * (uint32_t)C.dims[0] * (uint32_t)C.dims[1];
         return key ^ 0x80000000u;  // signed order as unsigned
     }
-    return (uint32_t)(grid_coord(x, 0, C) + C.dims[0] * (grid_coord(y, 1, C) + C.dims[1] * grid_coord(z, 2, C)));
+    return (uint32_t)(C.cell_off + grid_coord(x, 0, C) +
+                      C.dims[0] * (grid_coord(y, 1, C) + C.dims[1] * grid_coord(z, 2, C)));
 }
 
 // bucket of a sort key in the cell table: -1 below the grid, n_cells above it
@@ -94,13 +114,15 @@ __device__ __forceinline__ void hash_particle(const StepConsts& C, float x, floa
 // ---------------------------------------------------------------------------------------
 // Upload: float64 (N,3) AoS or float32 SoA in particle order -> posB/velB (slot = id), hash.
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void upload_f64_kernel(StepConsts C, const double* __restrict__ pos,
+template <bool BATCH>
+__global__ __launch_bounds__(256) void upload_f64_kernel(StepConsts G, const double* __restrict__ pos,
                                                           const double* __restrict__ vel, float4* __restrict__ posB,
                                                           float4* __restrict__ velB, uint32_t* __restrict__ pidB,
                                                           uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
                                                           uint32_t* __restrict__ flags) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= C.n) return;
+    if (i >= G.n) return;
+    const StepConsts& C = inst_consts<BATCH>(G, i);
     const float x = (float)pos[3 * (int64_t)i], y = (float)pos[3 * (int64_t)i + 1], z = (float)pos[3 * (int64_t)i + 2];
     posB[i] = make_float4(x, y, z, 0.0f);
     velB[i] = make_float4((float)vel[3 * (int64_t)i], (float)vel[3 * (int64_t)i + 1], (float)vel[3 * (int64_t)i + 2],
@@ -109,7 +131,8 @@ __global__ __launch_bounds__(256) void upload_f64_kernel(StepConsts C, const dou
     hash_particle(C, x, y, z, (uint32_t)i, (uint32_t)i, keyI, valI, flags);
 }
 
-__global__ __launch_bounds__(256) void upload_f32_kernel(StepConsts C, const float* __restrict__ px,
+template <bool BATCH>
+__global__ __launch_bounds__(256) void upload_f32_kernel(StepConsts G, const float* __restrict__ px,
                                                           const float* __restrict__ py, const float* __restrict__ pz,
                                                           const float* __restrict__ vx, const float* __restrict__ vy,
                                                           const float* __restrict__ vz, float4* __restrict__ posB,
@@ -117,7 +140,8 @@ __global__ __launch_bounds__(256) void upload_f32_kernel(StepConsts C, const flo
                                                           uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
                                                           uint32_t* __restrict__ flags) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= C.n) return;
+    if (i >= G.n) return;
+    const StepConsts& C = inst_consts<BATCH>(G, i);
     const float x = px[i], y = py[i], z = pz[i];
     posB[i] = make_float4(x, y, z, 0.0f);
     velB[i] = make_float4(vx[i], vy[i], vz[i], 0.0f);
@@ -202,7 +226,7 @@ __device__ __forceinline__ void for_each_candidate(float px, float py, float pz,
             if (rem < 0.0f) continue;
             const float e = sqrtf(rem);
             const int xlo = grid_coord(px - e, 0, C), xhi = grid_coord(px + e, 0, C);
-            const int row = C.dims[0] * (cy + C.dims[1] * cz);
+            const int row = C.cell_off + C.dims[0] * (cy + C.dims[1] * cz);
             const uint32_t k0 = cs[row + xlo], k1 = cs[row + xhi + 1];
             for (uint32_t k = k0; k < k1; ++k) f(k);
         }
@@ -229,18 +253,20 @@ constexpr int kDensityThreads = 256;
 #endif
 constexpr int kDensityUnroll = SPH_DENSITY_UNROLL;
 
-__global__ __launch_bounds__(kDensityThreads) void naive_density_kernel(StepConsts C, float4* pos, float4* vel,
+template <bool BATCH>
+__global__ __launch_bounds__(kDensityThreads) void naive_density_kernel(StepConsts G, float4* pos, float4* vel,
                                                                          const uint32_t* __restrict__ cs,
                                                                          float* __restrict__ rhoA,
                                                                          uint32_t* __restrict__ nl,
                                                                          uint32_t* __restrict__ nn) {
     __shared__ uint32_t s_list[kLdsListCap][kDensityThreads];
     const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= C.n) return;
-    const __amdgpu_buffer_rsrc_t rpos = make_rsrc(pos, (uint32_t)C.n * 16u);
-    const __amdgpu_buffer_rsrc_t rcs = make_rsrc(cs, ((uint32_t)C.n_cells + 1u) * 4u);
+    if (i >= G.n) return;
+    const StepConsts& C = inst_consts<BATCH>(G, i);
+    const __amdgpu_buffer_rsrc_t rpos = make_rsrc(pos, (uint32_t)G.n * 16u);
+    const __amdgpu_buffer_rsrc_t rcs = make_rsrc(cs, ((uint32_t)G.cs_cells + 1u) * 4u);
     const float4 p = pos[i];
-    const size_t stride = (size_t)C.n;
+    const size_t stride = (size_t)G.n;
     float acc = 0.0f;
     uint32_t cnt = 0;
     auto visit = [&](float4 q, uint32_t off) {
@@ -267,7 +293,7 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_kernel(StepCons
             if (rem < 0.0f) continue;
             const float e = __builtin_amdgcn_sqrtf(rem);  // within 1 ulp; hs carries the margin
             const int xlo = grid_coord(p.x - e, 0, C), xhi = grid_coord(p.x + e, 0, C);
-            const uint32_t row = (uint32_t)(C.dims[0] * (cy + C.dims[1] * cz));
+            const uint32_t row = (uint32_t)(C.cell_off + C.dims[0] * (cy + C.dims[1] * cz));
             uint32_t off = ld_u32(rcs, (row + (uint32_t)xlo) * 4u) * 16u;
             const uint32_t end = ld_u32(rcs, (row + (uint32_t)xhi + 1u) * 4u) * 16u;
             for (; off + 16u * (kDensityUnroll - 1) < end; off += 16u * kDensityUnroll) {
@@ -471,24 +497,25 @@ struct PairSums {
 #endif
 constexpr int kForceUnroll = SPH_FORCE_UNROLL;
 
-template <int MODE>
+template <int MODE, bool BATCH>
 __global__ __launch_bounds__(256) void naive_force_kernel(
-    StepConsts C, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
+    StepConsts G, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
     const float* __restrict__ rhoA, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ nl,
     const uint32_t* __restrict__ nn, float4* __restrict__ posOut, float4* __restrict__ velOut,
     uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
     uint64_t* __restrict__ rng, const PipeTable* __restrict__ pipe, float4* __restrict__ dbgP,
     float4* __restrict__ dbgV, float4* __restrict__ dbgF, uint32_t* __restrict__ flags) {
     const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= C.n) return;
-    const __amdgpu_buffer_rsrc_t rpos = make_rsrc(pos, (uint32_t)C.n * 16u);
-    const __amdgpu_buffer_rsrc_t rvel = make_rsrc(vel, (uint32_t)C.n * 16u);
+    if (i >= G.n) return;
+    const StepConsts& C = inst_consts<BATCH>(G, i);
+    const __amdgpu_buffer_rsrc_t rpos = make_rsrc(pos, (uint32_t)G.n * 16u);
+    const __amdgpu_buffer_rsrc_t rvel = make_rsrc(vel, (uint32_t)G.n * 16u);
     const float4 p = pos[i], v = vel[i];
     PairSums S;
     const uint32_t cnt = nn[i];
     if (cnt <= (uint32_t)kListCap) {
         const uint32_t* slot = nl + i;
-        const size_t stride = (size_t)C.n;
+        const size_t stride = (size_t)G.n;
         uint32_t t = 0;
         for (; t + kForceUnroll <= cnt; t += kForceUnroll) {
             uint32_t k[kForceUnroll];
@@ -778,7 +805,7 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s) {
 
 hipError_t launch_upload_f64(sph_ctx* c, const double* pos, const double* vel, hipStream_t s) {
     if (c->n > 0)
-        upload_f64_kernel<<<blocks_for(c->n, 256), 256, 0, s>>>(c->sc, pos, vel, c->b.posB, c->b.velB, c->b.pidB,
+        (c->sc.n_inst > 1 ? upload_f64_kernel<true> : upload_f64_kernel<false>)<<<blocks_for(c->n, 256), 256, 0, s>>>(c->sc, pos, vel, c->b.posB, c->b.velB, c->b.pidB,
                                                                   c->b.keyI, c->b.valI, c->b.flags);
     return hipGetLastError();
 }
@@ -786,7 +813,7 @@ hipError_t launch_upload_f64(sph_ctx* c, const double* pos, const double* vel, h
 hipError_t launch_upload_f32(sph_ctx* c, const float* x, const float* y, const float* z, const float* vx,
                              const float* vy, const float* vz, hipStream_t s) {
     if (c->n > 0)
-        upload_f32_kernel<<<blocks_for(c->n, 256), 256, 0, s>>>(c->sc, x, y, z, vx, vy, vz, c->b.posB, c->b.velB,
+        (c->sc.n_inst > 1 ? upload_f32_kernel<true> : upload_f32_kernel<false>)<<<blocks_for(c->n, 256), 256, 0, s>>>(c->sc, x, y, z, vx, vy, vz, c->b.posB, c->b.velB,
                                                                   c->b.pidB, c->b.keyI, c->b.valI, c->b.flags);
     return hipGetLastError();
 }
@@ -810,7 +837,7 @@ hipError_t launch_density(sph_ctx* c, hipStream_t s) {
         voxel_density_kernel<<<blocks_for(c->n, bs), bs, 0, s>>>(c->sc, c->b.posA, c->b.velA, c->b.cs, c->b.rhoA,
                                                                    c->b.nbr);
     else
-        naive_density_kernel<<<blocks_for(c->n, bs), bs, 0, s>>>(c->sc, c->b.posA, c->b.velA, c->b.cs, c->b.rhoA,
+        (c->sc.n_inst > 1 ? naive_density_kernel<true> : naive_density_kernel<false>)<<<blocks_for(c->n, bs), bs, 0, s>>>(c->sc, c->b.posA, c->b.velA, c->b.cs, c->b.rhoA,
                                                                    c->b.nl, c->b.nn);
     return hipGetLastError();
 }
@@ -826,7 +853,7 @@ void launch_force_mode(sph_ctx* c, hipStream_t s) {
             c->sc, c->b.posA, c->b.velA, c->b.pidA, c->b.rhoA, c->b.nbr, c->b.posB, c->b.velB, c->b.pidB, c->b.keyI,
             c->b.valI, c->b.rng, c->b.pipe, dP, dV, dF, c->b.flags);
     else
-        naive_force_kernel<MODE><<<blocks_for(c->n, bs), bs, 0, s>>>(
+        (c->sc.n_inst > 1 ? naive_force_kernel<MODE, true> : naive_force_kernel<MODE, false>)<<<blocks_for(c->n, bs), bs, 0, s>>>(
             c->sc, c->b.posA, c->b.velA, c->b.pidA, c->b.rhoA, c->b.cs, c->b.nl, c->b.nn, c->b.posB, c->b.velB,
             c->b.pidB, c->b.keyI, c->b.valI, c->b.rng, c->b.pipe, dP, dV, dF, c->b.flags);
 }

@@ -57,6 +57,7 @@ EXPORTED = [
     "sph_abi_version", "sph_last_error", "sph_create", "sph_destroy", "sph_workspace_bytes", "sph_bind_workspace",
     "sph_upload_state", "sph_upload_state_f32", "sph_step", "sph_download_state", "sph_download_state_f32",
     "sph_run_stage", "sph_buffer_elems", "sph_read_buffer", "sph_diagnostics", "sph_grid_info", "sph_set_launch",
+    "sph_create_batch", "sph_batch_info",
 ]
 
 _lib = None
@@ -90,6 +91,8 @@ def load_library() -> ctypes.CDLL:
         "sph_diagnostics": ([vp, vp, vp], i32),
         "sph_grid_info": ([vp, POINTER(i32), POINTER(i32), POINTER(i32), POINTER(i32)], i32),
         "sph_set_launch": ([vp, i32, i32], i32),
+        "sph_create_batch": ([POINTER(SphParamsC), POINTER(SphConstantsC), i32, POINTER(vp)], i32),
+        "sph_batch_info": ([vp, POINTER(i32), POINTER(i32)], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -121,15 +124,28 @@ def _stream(device) -> c_void_p:
 class NativeContext:
     """One SPH context on one device: owns the workspace tensor and the C handle."""
 
-    def __init__(self, params: SphParamsC, consts: SphConstantsC, device="cuda:0"):
+    def __init__(self, params, consts, device="cuda:0"):
+        """params/consts: one SphParamsC/SphConstantsC for a plain step, or equal-length
+        sequences of them for the batched form (sph_create_batch)."""
         if not torch.cuda.is_available():
             raise RuntimeError("the SPH step runs on a ROCm GPU; none is available")
         self.lib = load_library()
         self.device = torch.device(device)
-        self.n = int(params.particle_count)
         h = c_void_p()
-        check(self.lib.sph_create(ctypes.byref(params), ctypes.byref(consts), ctypes.byref(h)), "sph_create")
+        if isinstance(params, SphParamsC):
+            check(self.lib.sph_create(ctypes.byref(params), ctypes.byref(consts), ctypes.byref(h)), "sph_create")
+        else:
+            count = len(params)
+            parr = (SphParamsC * count)(*params)
+            carr = (SphConstantsC * count)(*consts)
+            check(self.lib.sph_create_batch(parr, carr, count, ctypes.byref(h)), "sph_create_batch")
         self.handle = h
+        cnt = c_int32()
+        self.lib.sph_batch_info(h, ctypes.byref(cnt), None)
+        first = (c_int32 * (cnt.value + 1))()
+        self.lib.sph_batch_info(h, None, first)
+        self.first = list(first)
+        self.n = self.first[-1]
         nbytes = int(self.lib.sph_workspace_bytes(h))
         # 256-byte alignment for the slices: over-allocate and offset
         self._ws = torch.empty(nbytes + 256, dtype=torch.uint8, device=self.device)

@@ -83,6 +83,20 @@ typedef struct sph_ctx sph_ctx;
  * particle exactly as create_xoroshiro128p_states(n, seed) does.  Host only. */
 int sph_create(const sph_params_t* params, const sph_constants_t* consts, sph_ctx** out);
 
+/* Batched form: count independent simulations (each with its own particle count, voxel
+ * grid, h, mu, box and dt) advanced together, one launch per stage for all of them.
+ * params[b] / consts[b] describe instance b; all instances must use SPH_FORM_NAIVE and
+ * SPH_MODE_BOX.  State arrays of the other entry points are the instances' arrays
+ * concatenated in order (instance b starts at particle first[b], see sph_batch_info).
+ * The reference has no counterpart: it advances one simulation per process
+ * (state_generator.py:14-37); this replaces running compute_next_state once per
+ * simulation. */
+int sph_create_batch(const sph_params_t* params, const sph_constants_t* consts, int32_t count, sph_ctx** out);
+
+/* Instance count and the count + 1 particle offsets of a context (1 and {0, N} for a plain
+ * step).  first may be NULL.  Host only. */
+int sph_batch_info(const sph_ctx* ctx, int32_t* count, int32_t* first);
+
 /* Releases the host context.  The workspace belongs to the caller. */
 int sph_destroy(sph_ctx* ctx);
 
