@@ -1,0 +1,253 @@
+// sph_adjoint.hip -- reverse mode of one NAIVE-formulation step in a box (gfx950).
+//
+// The step (sph_kernels.hip), with rho_i = w_norm sum_j u_ij^3, u = 1 - r^2/h^2,
+// p = K (rho - rho0), iota = 1/rho and d = x_i - x_j:
+//   P_i = ps sum_{j != i} (p_i + p_j) iota_j g(r) d,   g(r) = (h - r)^2 / r,  ps = m Gc / 2
+//   V_i = vs sum_j (h - r) iota_j (v_j - v_i),                              vs = m mu Lc
+//   F = f - P + V,  v' = v + F / rho dt,  x' = x + v' dt,  then the box collision.
+// Given the cotangent (gx', gv') of (x', v'), the pullback to (x, v) is, per particle k:
+//   collision   a bounced component's position is constant, its velocity is scaled by
+//               -damp:  gxp = b ? 0 : gx',  gvp = (b ? -damp gv' : gv') + gxp dt
+//   integration Fb = gvp dt iota (the cotangent of F, so Pb = -Fb, Vb = Fb),
+//               rho_b = -dt iota^2 (gvp . F),  gv = gvp + (viscosity), gx = gxp + ...
+// and the pair terms pull back as sums over the same neighbour set, which is symmetric
+// (j is a neighbour of k exactly when k is one of j: |d|^2 is computed bitwise alike), so
+// every transposed sum is a gather over k's own neighbours and no atomics are needed:
+//   gv_k   += vs sum_j (h - r) (iota_k Fb_j - iota_j Fb_k)
+//   pbar_k  = ps sum_j g (iota_k Fb_j.d - iota_j Fb_k.d)
+//   rho_b_k += -vs iota_k^2 sum_j (h - r) Fb_j.(v_k - v_j) - ps iota_k^2 sum_j (p_j + p_k) g Fb_j.d
+//   gx_k   += -vs sum_j (iota_k Fb_j.w - iota_j Fb_k.w) d / r,     w = v_k - v_j
+//             + ps sum_j J(d) (p_k + p_j) (iota_k Fb_j - iota_j Fb_k),
+//               J(d) = g I + g'(r) / r d d^T,  g'(r) = -(h - r)(h + r) / r^2
+//   rt_k    = rho_b_k + K pbar_k                  (total cotangent of rho_k)
+//   gx_k   += -6 w_norm / h^2 sum_j (rt_k + rt_j) u^2 d       (density pass)
+//   dL/dmu += sum_k Fb_k . V_k / mu
+// Kernels, in order, all in the cell order of the step:
+//   adjoint_seed_kernel     the force pass itself (same code, so the step's output is
+//                           written bitwise as sph_step writes it) plus the collision and
+//                           integration pullback
+//   adjoint_pair_kernel     pressure / viscosity pullback: gv final, rt, gx partial
+//   adjoint_density_kernel  density pullback: gx final
+//   adjoint_mu_kernel       dL/dmu of the step, summed in a fixed order (deterministic)
+// The derivation is checked against an FP64 restatement and central finite differences.
+#include "sph_device.h"
+
+namespace sph {
+namespace {
+
+constexpr int kAdjThreads = 256;
+
+// Neighbours of cell-ordered particle i (the particle itself included), from its list or,
+// for a particle with more neighbours than list slots, by rescanning its candidates.
+template <typename F>
+__device__ __forceinline__ void walk_neighbours(const StepConsts& C, int i, float4 p, const float4* __restrict__ pos,
+                                                const uint32_t* __restrict__ cs, const uint32_t* __restrict__ nl,
+                                                const uint32_t* __restrict__ nn, F&& f) {
+    const uint32_t cnt = nn[i];
+    if (cnt <= (uint32_t)kListCap) {
+        const uint32_t* slot = nl + i;
+        const size_t stride = (size_t)C.n;
+        for (uint32_t t = 0; t < cnt; ++t, slot += stride) f(slot[0]);
+    } else {
+        for_each_candidate(p.x, p.y, p.z, C, cs, [&](uint32_t k) {
+            const float4 q = pos[k];
+            const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+            if (dx * dx + dy * dy + dz * dz <= C.h2) f(k);
+        });
+    }
+}
+
+__device__ __forceinline__ float comp(const Vec3& a, int d) { return d == 0 ? a.x : (d == 1 ? a.y : a.z); }
+__device__ __forceinline__ float comp(const float4& a, int d) { return d == 0 ? a.x : (d == 1 ? a.y : a.z); }
+
+__global__ __launch_bounds__(kAdjThreads) void adjoint_seed_kernel(
+    StepConsts C, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
+    const float* __restrict__ rhoA, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ nl,
+    const uint32_t* __restrict__ nn, float4* __restrict__ posOut, float4* __restrict__ velOut,
+    uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
+    uint32_t* __restrict__ flags, const float4* __restrict__ cotx, const float4* __restrict__ cotv,
+    float4* __restrict__ fb, float4* __restrict__ gxs, float4* __restrict__ gvs) {
+    const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= C.n) return;
+    const float4 p = pos[i], v = vel[i];
+    const PairSums S = naive_pair_sums<false>(C, i, C.n, p, v, pos, vel, cs, nl, nn);
+    const float ps = 0.5f * C.mass * C.grad_c;
+    const float vs = C.mass * C.visc * C.lap_c;
+    const Vec3 P{ps * S.ax, ps * S.ay, ps * S.az}, V{vs * S.bx, vs * S.by, vs * S.bz};
+    const float rho = rhoA[i];
+    const uint32_t pid = pidA[i];
+    finish_particle<SPH_MODE_BOX>(C, i, p, v, pid, rho, P, V, posOut, velOut, pidOut, keyI, valI, nullptr, nullptr,
+                                  nullptr, nullptr, nullptr, flags);
+    const Integrated I = integrate_particle(C, p, v, rho, P, V);
+    const float4 gx1 = cotx[pid], gv1 = cotv[pid];
+    float gxp[3], gvp[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const bool b = box_bounced(C, comp(I.x, d), d);
+        gxp[d] = b ? 0.0f : comp(gx1, d);
+        gvp[d] = (b ? -C.damp * comp(gv1, d) : comp(gv1, d)) + gxp[d] * C.dt;
+    }
+    const float iota = v.w;  // 1/rho from the density pass
+    const float s = C.dt * iota;
+    const float fbx = gvp[0] * s, fby = gvp[1] * s, fbz = gvp[2] * s;
+    const float rb = -s * iota * (gvp[0] * I.f.x + gvp[1] * I.f.y + gvp[2] * I.f.z);
+    // dV/dmu = m Lc sum_j (h - r) iota_j (v_j - v_i)
+    const float mu_term = C.mass * C.lap_c * (fbx * S.bx + fby * S.by + fbz * S.bz);
+    fb[i] = make_float4(fbx, fby, fbz, rb);
+    gxs[i] = make_float4(gxp[0], gxp[1], gxp[2], mu_term);
+    gvs[i] = make_float4(gvp[0], gvp[1], gvp[2], 0.0f);
+}
+
+__global__ __launch_bounds__(kAdjThreads) void adjoint_pair_kernel(
+    StepConsts C, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
+    const uint32_t* __restrict__ cs, const uint32_t* __restrict__ nl, const uint32_t* __restrict__ nn,
+    const float4* __restrict__ fb, float4* __restrict__ gxs, const float4* __restrict__ gvs, float* __restrict__ rt,
+    float4* __restrict__ cotv) {
+    const int k = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (k >= C.n) return;
+    const float4 p = pos[k], v = vel[k], fk = fb[k];
+    const float ik = v.w, h = C.h;
+    float gvx = 0.f, gvy = 0.f, gvz = 0.f;  // x vs
+    float gxx = 0.f, gxy = 0.f, gxz = 0.f;  // final units
+    float rbv = 0.f, rbp = 0.f, pb = 0.f;
+    const float ps = 0.5f * C.mass * C.grad_c;
+    const float vs = C.mass * C.visc * C.lap_c;
+    walk_neighbours(C, k, p, pos, cs, nl, nn, [&](uint32_t j) {
+        if (j == (uint32_t)k) return;
+        const float4 q = pos[j], w = vel[j], fj = fb[j];
+        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float rinv = rsqrtf(r2);
+        const float r = r2 * rinv;
+        const float hr = h - r;
+        const float ij = w.w;
+        // viscosity
+        gvx += hr * (ik * fj.x - ij * fk.x);
+        gvy += hr * (ik * fj.y - ij * fk.y);
+        gvz += hr * (ik * fj.z - ij * fk.z);
+        const float wx = v.x - w.x, wy = v.y - w.y, wz = v.z - w.z;
+        const float fjw = fj.x * wx + fj.y * wy + fj.z * wz;
+        const float fkw = fk.x * wx + fk.y * wy + fk.z * wz;
+        rbv += hr * fjw;
+        const float sv = -vs * (ik * fjw - ij * fkw) * rinv;
+        // pressure
+        const float g = hr * hr * rinv;
+        const float fjd = fj.x * dx + fj.y * dy + fj.z * dz;
+        const float fkd = fk.x * dx + fk.y * dy + fk.z * dz;
+        const float ppk = p.w + q.w;
+        pb += g * (ik * fjd - ij * fkd);
+        rbp += ppk * g * fjd;
+        const float bx = ppk * (ik * fj.x - ij * fk.x);
+        const float by = ppk * (ik * fj.y - ij * fk.y);
+        const float bz = ppk * (ik * fj.z - ij * fk.z);
+        const float db = dx * bx + dy * by + dz * bz;
+        const float gp = -hr * (h + r) * rinv * rinv * rinv * db;
+        gxx += sv * dx + ps * (g * bx + gp * dx);
+        gxy += sv * dy + ps * (g * by + gp * dy);
+        gxz += sv * dz + ps * (g * bz + gp * dz);
+    });
+    rt[k] = fk.w - vs * ik * ik * rbv - ps * ik * ik * rbp + C.k * ps * pb;
+    const float4 gx = gxs[k], gv = gvs[k];
+    gxs[k] = make_float4(gx.x + gxx, gx.y + gxy, gx.z + gxz, gx.w);
+    cotv[pidA[k]] = make_float4(gv.x + vs * gvx, gv.y + vs * gvy, gv.z + vs * gvz, 0.0f);
+}
+
+__global__ __launch_bounds__(kAdjThreads) void adjoint_density_kernel(
+    StepConsts C, const float4* __restrict__ pos, const uint32_t* __restrict__ pidA, const uint32_t* __restrict__ cs,
+    const uint32_t* __restrict__ nl, const uint32_t* __restrict__ nn, const float* __restrict__ rt,
+    const float4* __restrict__ gxs, float4* __restrict__ cotx) {
+    const int k = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (k >= C.n) return;
+    const float4 p = pos[k];
+    const float rk = rt[k];
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    walk_neighbours(C, k, p, pos, cs, nl, nn, [&](uint32_t j) {
+        const float4 q = pos[j];
+        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        const float u = fmaxf(1.0f - (dx * dx + dy * dy + dz * dz) * C.inv_h2, 0.0f);
+        const float c = (rk + rt[j]) * u * u;
+        ax += c * dx;
+        ay += c * dy;
+        az += c * dz;
+    });
+    const float s = -6.0f * C.w_norm * C.inv_h2;
+    const float4 gx = gxs[k];
+    cotx[pidA[k]] = make_float4(gx.x + s * ax, gx.y + s * ay, gx.z + s * az, 0.0f);
+}
+
+// One workgroup: each lane sums a strided range in order, then a fixed tree; the step's
+// dL/dmu is added to acc[0].
+__global__ __launch_bounds__(kAdjThreads) void adjoint_mu_kernel(int n, const float4* __restrict__ gxs,
+                                                                  float* __restrict__ acc) {
+    __shared__ float part[kAdjThreads];
+    float s = 0.0f;
+    for (int i = threadIdx.x; i < n; i += kAdjThreads) s += gxs[i].w;
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kAdjThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[0] += part[0];
+}
+
+__global__ __launch_bounds__(kAdjThreads) void adjoint_set_kernel(int n, const float* __restrict__ gx,
+                                                                   const float* __restrict__ gv,
+                                                                   float4* __restrict__ cotx,
+                                                                   float4* __restrict__ cotv, float* __restrict__ acc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) acc[0] = 0.0f;
+    if (i >= n) return;
+    const int64_t o = 3 * (int64_t)i;
+    cotx[i] = gx ? make_float4(gx[o], gx[o + 1], gx[o + 2], 0.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
+    cotv[i] = gv ? make_float4(gv[o], gv[o + 1], gv[o + 2], 0.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(kAdjThreads) void adjoint_get_kernel(int n, const float4* __restrict__ cotx,
+                                                                   const float4* __restrict__ cotv,
+                                                                   const float* __restrict__ acc, float* __restrict__ gx,
+                                                                   float* __restrict__ gv, float* __restrict__ gmu) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && gmu) gmu[0] = acc[0];
+    if (i >= n) return;
+    const int64_t o = 3 * (int64_t)i;
+    if (gx) { const float4 a = cotx[i]; gx[o] = a.x; gx[o + 1] = a.y; gx[o + 2] = a.z; }
+    if (gv) { const float4 a = cotv[i]; gv[o] = a.x; gv[o + 1] = a.y; gv[o + 2] = a.z; }
+}
+
+inline int adj_blocks(int64_t n) { return (int)((n + kAdjThreads - 1) / kAdjThreads); }
+
+}  // namespace
+
+hipError_t launch_adjoint_set(sph_ctx* c, const float* gx, const float* gv, hipStream_t s) {
+    const int64_t n = c->n;
+    adjoint_set_kernel<<<adj_blocks(n > 0 ? n : 1), kAdjThreads, 0, s>>>((int)n, gx, gv, c->b.cotx, c->b.cotv,
+                                                                         c->b.mu_acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_get(sph_ctx* c, float* gx, float* gv, float* gmu, hipStream_t s) {
+    const int64_t n = c->n;
+    adjoint_get_kernel<<<adj_blocks(n > 0 ? n : 1), kAdjThreads, 0, s>>>((int)n, c->b.cotx, c->b.cotv, c->b.mu_acc,
+                                                                         gx, gv, gmu);
+    return hipGetLastError();
+}
+
+// After sort, cell table and density of the step: the seed, pair, density and mu kernels.
+hipError_t launch_adjoint_pullback(sph_ctx* c, hipStream_t s) {
+    const int64_t n = c->n;
+    if (n == 0) return hipGetLastError();
+    const Buffers& b = c->b;
+    const int nb = adj_blocks(n);
+    adjoint_seed_kernel<<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.velA, b.pidA, b.rhoA, b.cs, b.nl, b.nn, b.posB,
+                                                    b.velB, b.pidB, b.keyI, b.valI, b.flags, b.cotx, b.cotv, b.adj_fb,
+                                                    b.adj_gx, b.adj_gv);
+    adjoint_pair_kernel<<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.velA, b.pidA, b.cs, b.nl, b.nn, b.adj_fb, b.adj_gx,
+                                                    b.adj_gv, b.adj_rt, b.cotv);
+    adjoint_density_kernel<<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.pidA, b.cs, b.nl, b.nn, b.adj_rt, b.adj_gx,
+                                                       b.cotx);
+    adjoint_mu_kernel<<<1, kAdjThreads, 0, s>>>((int)n, b.adj_gx, b.mu_acc);
+    return hipGetLastError();
+}
+
+}  // namespace sph

@@ -80,7 +80,8 @@ size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 // Workspace slices, in order.
 enum Slice {
     S_POSA, S_VELA, S_POSB, S_VELB, S_PIDA, S_PIDB, S_RHOA, S_KEYI, S_VALI, S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_CS,
-    S_HIST, S_DTOT, S_RNG, S_DBGP, S_DBGV, S_DBGF, S_NBR, S_NL, S_NN, S_DIAG, S_FLAGS, S_PIPE, S_INST, S_INSTF, S_COUNT
+    S_HIST, S_DTOT, S_RNG, S_DBGP, S_DBGV, S_DBGF, S_NBR, S_NL, S_NN, S_DIAG, S_FLAGS, S_PIPE, S_INST, S_INSTF,
+    S_COTX, S_COTV, S_AFB, S_AGX, S_AGV, S_ART, S_MU, S_COUNT
 };
 
 struct Layout {
@@ -107,6 +108,9 @@ Layout plan_layout(const sph_ctx* c) {
     sz[S_PIPE] = sizeof(sph::PipeTable);
     sz[S_INST] = sizeof(sph::StepConsts) * (c->inst.empty() ? 1 : c->inst.size());
     sz[S_INSTF] = sizeof(int) * (c->inst_first.empty() ? 1 : c->inst_first.size());
+    sz[S_COTX] = sz[S_COTV] = sz[S_AFB] = sz[S_AGX] = sz[S_AGV] = c->adjoint ? 16 * nn : 16;
+    sz[S_ART] = c->adjoint ? 4 * nn : 16;
+    sz[S_MU] = 64;
     Layout L;
     size_t o = 0;
     for (int i = 0; i < S_COUNT; ++i) {
@@ -214,6 +218,9 @@ int plan(sph_ctx* c) {
     if (rc) return rc;
     c->n = c->prm.particle_count;
     c->keep_terms = (c->prm.flags & SPH_FLAG_KEEP_TERMS) ? 1 : 0;
+    c->adjoint = (c->prm.flags & SPH_FLAG_ADJOINT) ? 1 : 0;
+    if (c->adjoint && (c->prm.formulation != SPH_FORM_NAIVE || c->prm.mode != SPH_MODE_BOX))
+        return fail(SPH_E_INVALID, "SPH_FLAG_ADJOINT needs the NAIVE formulation in BOX mode");
     plan_sort_and_layout(c);
     return SPH_OK;
 }
@@ -230,6 +237,8 @@ int plan_batch(sph_ctx* c, const sph_params_t* params, const sph_constants_t* co
     for (int b = 0; b < count; ++b) {
         if (params[b].formulation != SPH_FORM_NAIVE || params[b].mode != SPH_MODE_BOX)
             return fail(SPH_E_INVALID, "batched instances use the NAIVE formulation in BOX mode (instance %d)", b);
+        if (params[b].flags & SPH_FLAG_ADJOINT)
+            return fail(SPH_E_INVALID, "SPH_FLAG_ADJOINT is for plain contexts (instance %d)", b);
         sph::PipeTable unused;
         int rc = plan_instance(params[b], consts[b], c->inst[b], unused);
         if (rc) {
@@ -264,7 +273,8 @@ int plan_batch(sph_ctx* c, const sph_params_t* params, const sph_constants_t* co
     return SPH_OK;
 }
 
-int run_step_stages(sph_ctx* c, hipStream_t s, bool fused) {
+// Sort (with the gather into cell order), cell table and density of a step.
+int run_density_stages(sph_ctx* c, hipStream_t s, bool fused) {
     int rc;
     if (fused && c->passes > 0) {
         if ((rc = hip_check(sph::launch_radix_sort_impl(c, true, s), "sort"))) return rc;
@@ -273,7 +283,12 @@ int run_step_stages(sph_ctx* c, hipStream_t s, bool fused) {
         if ((rc = hip_check(sph::launch_reorder(c, s), "reorder"))) return rc;
     }
     if ((rc = hip_check(sph::launch_cell_table(c, s), "cell table"))) return rc;
-    if ((rc = hip_check(sph::launch_density(c, s), "density"))) return rc;
+    return hip_check(sph::launch_density(c, s), "density");
+}
+
+int run_step_stages(sph_ctx* c, hipStream_t s, bool fused) {
+    int rc;
+    if ((rc = run_density_stages(c, s, fused))) return rc;
     if ((rc = hip_check(sph::launch_force(c, s), "force"))) return rc;
     c->last_stage = SPH_STAGE_FORCE;
     return SPH_OK;
@@ -391,6 +406,13 @@ int sph_bind_workspace(sph_ctx* c, void* dev, size_t bytes, void* stream) {
     b.pipe = (sph::PipeTable*)(base + L.off[S_PIPE]);
     b.inst_table = (sph::StepConsts*)(base + L.off[S_INST]);
     b.inst_first = (int*)(base + L.off[S_INSTF]);
+    b.cotx = (float4*)(base + L.off[S_COTX]);
+    b.cotv = (float4*)(base + L.off[S_COTV]);
+    b.adj_fb = (float4*)(base + L.off[S_AFB]);
+    b.adj_gx = (float4*)(base + L.off[S_AGX]);
+    b.adj_gv = (float4*)(base + L.off[S_AGV]);
+    b.adj_rt = (float*)(base + L.off[S_ART]);
+    b.mu_acc = (float*)(base + L.off[S_MU]);
     c->ws = dev;
     c->sorted_keys = b.keyI;
     c->sorted_vals = b.valI;
@@ -420,6 +442,7 @@ int sph_bind_workspace(sph_ctx* c, void* dev, size_t bytes, void* stream) {
     if ((rc = hip_check(hipStreamSynchronize(s), "bind"))) return rc;
     c->bound = true;
     c->have_state = false;
+    c->adj_seeded = false;
     return SPH_OK;
 }
 
@@ -568,6 +591,55 @@ int sph_set_launch(sph_ctx* c, int32_t density_block, int32_t force_block) {
     c->density_block = density_block;
     c->force_block = force_block;
     return SPH_OK;
+}
+
+int sph_set_constants(sph_ctx* c, const sph_constants_t* k) {
+    if (!c || !k) return fail(SPH_E_INVALID, "null argument");
+    if (!c->inst.empty()) return fail(SPH_E_STATE, "constants of a batch are set per instance at creation");
+    if (k->h != c->cst.h) return fail(SPH_E_INVALID, "h %g differs from the context's %g (it fixes the grid)", k->h, c->cst.h);
+    if (!(k->mass > 0) || !isfinite(k->mass)) return fail(SPH_E_INVALID, "mass must be positive and finite");
+    for (double v : {k->rho0, k->visc, k->k, k->damp})
+        if (!isfinite(v)) return fail(SPH_E_INVALID, "constants must be finite");
+    c->cst = *k;
+    sph::StepConsts& S = c->sc;
+    S.mass = (float)k->mass;
+    S.rho0 = (float)k->rho0;
+    S.k = (float)k->k;
+    S.visc = (float)k->visc;
+    S.damp = (float)k->damp;
+    S.w_norm = (float)(k->mass * 315.0 / (64.0 * M_PI * k->h * k->h * k->h));
+    return SPH_OK;
+}
+
+int sph_adjoint_seed(sph_ctx* c, const float* gx, const float* gv, void* stream) {
+    int rc = ready(c, false);
+    if (rc) return rc;
+    if (!c->adjoint) return fail(SPH_E_STATE, "context created without SPH_FLAG_ADJOINT");
+    if ((rc = hip_check(sph::launch_adjoint_set(c, gx, gv, (hipStream_t)stream), "adjoint seed"))) return rc;
+    c->adj_seeded = true;
+    return SPH_OK;
+}
+
+int sph_adjoint_step(sph_ctx* c, void* stream) {
+    int rc = ready(c, true);
+    if (rc) return rc;
+    if (!c->adjoint) return fail(SPH_E_STATE, "context created without SPH_FLAG_ADJOINT");
+    if (!c->adj_seeded) return fail(SPH_E_STATE, "no cotangent (call sph_adjoint_seed)");
+    if (c->last_stage != 0 && c->last_stage != SPH_STAGE_FORCE)
+        return fail(SPH_E_STATE, "a staged step is in progress (run it to SPH_STAGE_FORCE first)");
+    hipStream_t s = (hipStream_t)stream;
+    if ((rc = run_density_stages(c, s, true))) return rc;
+    if ((rc = hip_check(sph::launch_adjoint_pullback(c, s), "adjoint"))) return rc;
+    c->last_stage = SPH_STAGE_FORCE;
+    return SPH_OK;
+}
+
+int sph_adjoint_read(sph_ctx* c, float* gx, float* gv, float* gmu, void* stream) {
+    int rc = ready(c, false);
+    if (rc) return rc;
+    if (!c->adjoint) return fail(SPH_E_STATE, "context created without SPH_FLAG_ADJOINT");
+    if (!c->adj_seeded) return fail(SPH_E_STATE, "no cotangent (call sph_adjoint_seed)");
+    return hip_check(sph::launch_adjoint_get(c, gx, gv, gmu, (hipStream_t)stream), "adjoint read");
 }
 
 }  // extern "C"

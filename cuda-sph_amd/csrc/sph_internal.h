@@ -11,6 +11,7 @@
 //   keyS/valS  u32[2][N]  radix ping-pong buffers
 //   cs         u32[C+1]   cell start table: cs[c] = first cell-sorted index with key >= c
 //   rng        u64[2N]    xoroshiro128+ stream per particle id (PIPE mode)
+//   cotx/cotv  float4[N]  reverse mode only: cotangent of the state in particle order
 //   nl/nn      u32[96][N], u32[N]  NAIVE neighbour lists of the density pass, read by the
 //                         force pass.  Column-major: slot t of consecutive particles is
 //                         contiguous.  The density pass appends each lane's first 32 hits to
@@ -93,6 +94,14 @@ struct Buffers {
     PipeTable* pipe;
     StepConsts* inst_table;
     int* inst_first;
+    // reverse mode (SPH_FLAG_ADJOINT): cotangent of the state in particle order, and per
+    // cell-ordered particle the force cotangent (.w: rho cotangent of the integration),
+    // position cotangent so far (.w: the particle's dL/dmu term), velocity cotangent of the
+    // integration and the total rho cotangent
+    float4 *cotx, *cotv;
+    float4 *adj_fb, *adj_gx, *adj_gv;
+    float* adj_rt;
+    float* mu_acc;
 };
 
 }  // namespace sph
@@ -112,6 +121,8 @@ struct sph_ctx {
     int num_tiles;
     int density_block, force_block;
     int keep_terms;
+    int adjoint;             // SPH_FLAG_ADJOINT: reverse-mode buffers are part of the workspace
+    bool adj_seeded;
     size_t ws_bytes;
     void* ws;
     sph::Buffers b;
@@ -144,4 +155,8 @@ hipError_t launch_download_f32(sph_ctx* c, float* x, float* y, float* z, float* 
 hipError_t launch_read_buffer(sph_ctx* c, int which, void* out, hipStream_t s);
 hipError_t launch_diagnostics(sph_ctx* c, double* out8, hipStream_t s);
 int radix_num_tiles(int64_t n);
+// sph_adjoint.hip
+hipError_t launch_adjoint_set(sph_ctx* c, const float* gx, const float* gv, hipStream_t s);
+hipError_t launch_adjoint_get(sph_ctx* c, float* gx, float* gv, float* gmu, hipStream_t s);
+hipError_t launch_adjoint_pullback(sph_ctx* c, hipStream_t s);
 }  // namespace sph

@@ -11,105 +11,10 @@
 // reproduces voxel_kernels.py including its 32-neighbour cap and enumeration order.
 #include <math.h>
 
-#include "sph_internal.h"
+#include "sph_device.h"
 
 namespace sph {
 namespace {
-
-// Workgroup -> tile mapping that gives each of the 8 XCDs (one L2 each) a contiguous
-// range of cell-ordered tiles, so neighbour reads of adjacent tiles hit the same L2.
-// Workgroups are dispatched round-robin over XCDs: block b runs on XCD b % 8.
-__device__ __forceinline__ int xcd_tile(int bid, int nb) {
-    if (nb < 2 * kXcds) return bid;
-    const int q = nb / kXcds, r = nb % kXcds;
-    const int x = bid % kXcds, local = bid / kXcds;
-    return x < r ? x * (q + 1) + local : r * (q + 1) + (x - r) * q + local;
-}
-
-// Constants of the instance that owns particle slot i.  A plain step has one instance (the
-// kernel argument itself).  In the batched form every instance owns a contiguous range of
-// particle ids and, because its cell keys form a contiguous range too, the same range of
-// cell-ordered slots; the range is found by binary search over the instance offsets.
-template <bool BATCH>
-__device__ __forceinline__ const StepConsts& inst_consts(const StepConsts& G, int i) {
-    if constexpr (!BATCH) {
-        return G;
-    } else {
-        int lo = 0, hi = G.n_inst;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (G.inst_first[mid] <= i) lo = mid;
-            else hi = mid;
-        }
-        return G.inst_table[lo];
-    }
-}
-
-
-// Raw buffer loads with 32-bit byte offsets: no 64-bit address arithmetic per access, and
-// an offset past num_records reads zero instead of faulting.  Word 3 of the descriptor is
-// the gfx9 value for untyped 32-bit-element buffers.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
-}
-__device__ __forceinline__ float4 ld_f4(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)byte_off, 0, 0);
-    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
-}
-__device__ __forceinline__ uint32_t ld_u32(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
-    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)byte_off, 0, 0);
-}
-
-// ---------------------------------------------------------------------------------------
-// Voxel keys.  Reference semantics (voxel_kernels.py:9-17, 88-105): per axis
-// int32(x / voxel) truncated toward zero, flattened as x + y*Dx + z*Dx*Dy with wrapping
-// int32 arithmetic and no clamping.  Grid semantics: the same but floor and clamped to the
-// grid, which keeps every pair within h in neighbouring cells for any position.  Both are
-// equal for 0 <= x < Dx*voxel, and both are exact: with power-of-two voxels x * (1/voxel)
-// is exact in FP32; otherwise the quotient is formed in FP64 as the reference does.
-// ---------------------------------------------------------------------------------------
-__device__ __forceinline__ int grid_coord(float x, int d, const StepConsts& C) {
-    float q;
-    if (C.pow2) q = floorf(x * C.inv_cell[d]);
-    else q = (float)floor((double)x / C.dcell[d]);
-    q = fminf(fmaxf(q, 0.0f), (float)(C.dims[d] - 1));  // NaN -> 0
-    return (int)q;
-}
-
-__device__ __forceinline__ int32_t ref_coord(float x, int d, const StepConsts& C) {
-    if (C.pow2) return __float2int_rz(x * C.inv_cell[d]);
-    return __double2int_rz((double)x / C.dcell[d]);
-}
-
-__device__ __forceinline__ uint32_t sort_key(float x, float y, float z, const StepConsts& C) {
-    if (C.ref_keys) {
-        uint32_t vx = (uint32_t)ref_coord(x, 0, C), vy = (uint32_t)ref_coord(y, 1, C),
-                 vz = (uint32_t)ref_coord(z, 2, C);
-        uint32_t key = vx + vy * (uint32_t)C.dims[0] + vz * (uint32_t)C.dims[0] * (uint32_t)C.dims[1];
-        return key ^ 0x80000000u;  // signed order as unsigned
-    }
-    return (uint32_t)(C.cell_off + grid_coord(x, 0, C) +
-                      C.dims[0] * (grid_coord(y, 1, C) + C.dims[1] * grid_coord(z, 2, C)));
-}
-
-// bucket of a sort key in the cell table: -1 below the grid, n_cells above it
-__device__ __forceinline__ int key_bucket(uint32_t ukey, const StepConsts& C) {
-    if (!C.ref_keys) return (int)ukey;
-    const int32_t s = (int32_t)(ukey ^ 0x80000000u);
-    return s < 0 ? -1 : (s >= C.n_cells ? C.n_cells : s);
-}
-
-__device__ __forceinline__ bool finite3(float x, float y, float z) {
-    return isfinite(x) && isfinite(y) && isfinite(z);
-}
-
-__device__ __forceinline__ void hash_particle(const StepConsts& C, float x, float y, float z, uint32_t pid,
-                                              uint32_t slot, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
-                                              uint32_t* __restrict__ flags) {
-    keyI[pid] = sort_key(x, y, z, C);
-    valI[pid] = slot;
-    if (!finite3(x, y, z)) atomicAdd(flags, 1u);
-}
 
 // ---------------------------------------------------------------------------------------
 // Upload: float64 (N,3) AoS or float32 SoA in particle order -> posB/velB (slot = id), hash.
@@ -197,42 +102,6 @@ __global__ __launch_bounds__(256) void reorder_kernel(int n, const uint32_t* __r
     pidA[j] = pidB[s];
 }
 
-// ---------------------------------------------------------------------------------------
-// Neighbour candidates of the NAIVE formulation.  For each row of cells (y, z) whose slab
-// comes within h of the particle, the x cells are those overlapping [x - e, x + e] with
-// e = sqrt(h^2 - dy^2 - dz^2); cells of one row are consecutive keys, so the candidates
-// of a row are one contiguous range cs[first]..cs[last+1].  Boundary cells extend to
-// infinity (positions outside the grid are clamped into them).  The search radius carries
-// a small margin; the exact test r^2 <= h^2 is applied to each candidate.
-// ---------------------------------------------------------------------------------------
-__device__ __forceinline__ float slab_dist(float p, int c, int d, const StepConsts& C) {
-    const float lo = c == 0 ? -INFINITY : (float)c * C.cell[d];
-    const float hi = c == C.dims[d] - 1 ? INFINITY : (float)(c + 1) * C.cell[d];
-    return fmaxf(fmaxf(lo - p, p - hi), 0.0f);
-}
-
-template <typename F>
-__device__ __forceinline__ void for_each_candidate(float px, float py, float pz, const StepConsts& C,
-                                                   const uint32_t* __restrict__ cs, F&& f) {
-    const float hs = C.h_search;
-    const float hs2 = hs * hs;
-    const int ylo = grid_coord(py - hs, 1, C), yhi = grid_coord(py + hs, 1, C);
-    const int zlo = grid_coord(pz - hs, 2, C), zhi = grid_coord(pz + hs, 2, C);
-    for (int cz = zlo; cz <= zhi; ++cz) {
-        const float dz = slab_dist(pz, cz, 2, C);
-        for (int cy = ylo; cy <= yhi; ++cy) {
-            const float dy = slab_dist(py, cy, 1, C);
-            const float rem = hs2 - dz * dz - dy * dy;
-            if (rem < 0.0f) continue;
-            const float e = sqrtf(rem);
-            const int xlo = grid_coord(px - e, 0, C), xhi = grid_coord(px + e, 0, C);
-            const int row = C.cell_off + C.dims[0] * (cy + C.dims[1] * cz);
-            const uint32_t k0 = cs[row + xlo], k1 = cs[row + xhi + 1];
-            for (uint32_t k = k0; k < k1; ++k) f(k);
-        }
-    }
-}
-
 // Writes pressure and 1/rho into the .w lanes of posA / velA (4-byte stores; the .w lanes
 // are not read by this kernel, other threads read only x, y, z of their neighbours).
 __device__ __forceinline__ void store_density(const StepConsts& C, int i, float acc, float4* pos, float4* vel,
@@ -312,191 +181,10 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_kernel(StepCons
     store_density(C, i, acc, pos, vel, rhoA);
 }
 
-// ---------------------------------------------------------------------------------------
-// Wall collision.  Box: base_kernels.py:75-98.  Pipe: base_kernels.py:56-72, 101-127 and
-// util_kernels.py:38-204, in FP32, with the particle's own xoroshiro128+ stream.
-// ---------------------------------------------------------------------------------------
-__device__ __forceinline__ void collide_box(const StepConsts& C, float& x, float& v, int d) {
-    bool bounced = false;
-    if (x < 0.0f) { x = C.box_lo_target; bounced = true; }
-    if (x > C.space[d]) { x = C.box_hi_target[d]; bounced = true; }
-    if (bounced) { v *= -1.0f; v *= C.damp; }
-}
-
-struct Vec3 { float x, y, z; };
-__device__ __forceinline__ float len3(Vec3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
-__device__ __forceinline__ float dot3(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-
-__device__ int pipe_find_segment(const PipeTable& T, int rows, float x) {
-    for (int j = 0; j < rows - 1; ++j)
-        if (T.row[j][0] <= x && x < T.row[j + 1][0]) return j;
-    return -1;
-}
-
-__device__ float pipe_radius(const PipeTable& T, float x, int seg) {
-    const float r0 = T.row[seg][3], r1 = T.row[seg + 1][3], len = T.row[seg][4];
-    if (r0 == r1) return r0;
-    if (r0 < r1) {
-        const float delta = x - T.xbeg[seg];
-        const float tl = len * r0 / (r1 - r0);
-        return r0 * (1.0f + delta / tl);
-    }
-    const float delta = T.xbeg[seg] + len - x;
-    const float tl = len * r1 / (r0 - r1);
-    return r1 * (1.0f + delta / tl);
-}
-
-__device__ bool pipe_out(const PipeTable& T, Vec3 p, int seg) {
-    const float yn = p.y - T.row[seg][1], zn = p.z - T.row[seg][2];
-    return sqrtf(yn * yn + zn * zn) > pipe_radius(T, p.x, seg);
-}
-
-__device__ void pipe_solve_collision(const PipeTable& T, Vec3& p, Vec3& v, int seg) {
-    const float sr = T.row[seg][3], er = T.row[seg + 1][3];
-    const float yn = p.y - T.row[seg][1], zn = p.z - T.row[seg][2];
-    const float hh = sqrtf(yn * yn + zn * zn);
-    // calc_edge_vector (util_kernels.py:152-171)
-    Vec3 first{T.row[seg][0], yn / hh * sr, zn / hh * sr};
-    const Vec3 second{T.row[seg + 1][0], yn / hh * er, zn / hh * er};
-    const Vec3 edge{second.x - first.x, second.y - first.y, second.z - first.z};
-    first.y += T.row[seg][1];
-    first.z += T.row[seg][2];
-    // calc_dt (util_kernels.py:113-121) with calc_distance_to_pipe (:124-135)
-    const float sl = len3(v), el = len3(edge);
-    const float cos_a = dot3(v, edge) / (sl * el);
-    const Vec3 pv{first.x - p.x, first.y - p.y, first.z - p.z};
-    const Vec3 cr{edge.y * pv.z - edge.z * pv.y, edge.z * pv.x - edge.x * pv.z, edge.x * pv.y - edge.y * pv.x};
-    const float dist = len3(cr) / el;
-    const float sin_a = sqrtf(1.0f - cos_a * cos_a);
-    const float dt = -(dist / (sin_a + 0.001f)) / sl;
-    const Vec3 cp{p.x + v.x * dt, p.y + v.y * dt, p.z + v.z * dt};
-    // reflection (util_kernels.py:191-199)
-    if (sr == er) {
-        v.y = -v.y;
-        v.z = -v.z;
-    } else {
-        const float c = dot3(edge, v) / (el * sl);
-        const Vec3 sv{2.0f * edge.x / el * c * sl, 2.0f * edge.y / el * c * sl, 2.0f * edge.z / el * c * sl};
-        v = Vec3{sv.x - v.x, sv.y - v.y, sv.z - v.z};
-    }
-    const Vec3 dd{p.x - cp.x, p.y - cp.y, p.z - cp.z};
-    const float way = len3(dd);
-    const float svl = len3(v);
-    p = Vec3{cp.x + v.x * way / svl, cp.y + v.y * way / svl, cp.z + v.z * way / svl};
-}
-
-// xoroshiro128+ (Blackman & Vigna 2016; rotations 55, 14, 36) and the 53-bit uniform of
-// numba.cuda.random.xoroshiro128p_uniform_float64, rounded once to FP32.
-__device__ __forceinline__ uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
-__device__ __forceinline__ uint64_t xoro_next(uint64_t& s0, uint64_t& s1) {
-    const uint64_t a = s0;
-    uint64_t b = s1;
-    const uint64_t result = a + b;
-    b ^= a;
-    s0 = rotl64(a, 55) ^ b ^ (b << 14);
-    s1 = rotl64(b, 36);
-    return result;
-}
-__device__ __forceinline__ float xoro_uniform(uint64_t& s0, uint64_t& s1) {
-    return (float)(double)(xoro_next(s0, s1) >> 11) * 0x1p-53f;
-}
-
-__device__ void collide_pipe(const StepConsts& C, const PipeTable& T, Vec3& p, Vec3& v, uint64_t* __restrict__ rng,
-                             uint32_t pid) {
-    const int rows = C.pipe_rows;
-    const int seg = pipe_find_segment(T, rows, p.x);
-    if (seg == -1) {
-        // put_particle_at_pipe_begin (base_kernels.py:115-127)
-        if (p.x < 0.0f) {
-            p.x = -p.x;
-            v.x = -v.x;
-            if (pipe_out(T, p, 0)) pipe_solve_collision(T, p, v, 0);
-        } else {
-            p.x = 0.0f;
-            // rand_position_inside_pipe (base_kernels.py:101-112); a pipe that does not start
-            // at x <= 0 has no segment here and the reference reads past its table, so this
-            // uses segment 0 instead
-            int s0g = pipe_find_segment(T, rows, p.x);
-            if (s0g < 0) s0g = 0;
-            const float R = pipe_radius(T, p.x, s0g);
-            uint64_t a = rng[2 * (uint64_t)pid], b = rng[2 * (uint64_t)pid + 1];
-            const float r = R * sqrtf(xoro_uniform(a, b));
-            const float theta = xoro_uniform(a, b) * 2.0f * 3.14159265358979f;
-            rng[2 * (uint64_t)pid] = a;
-            rng[2 * (uint64_t)pid + 1] = b;
-            p.y = T.row[0][1] + r * cosf(theta);
-            p.z = T.row[0][2] + r * sinf(theta);
-        }
-    } else if (pipe_out(T, p, seg)) {
-        pipe_solve_collision(T, p, v, seg);
-    }
-}
-
-// Symplectic Euler (base_kernels.py:30-53), collision, output in the same sorted slot of
-// posB/velB, hash of the new position for the next step.
-template <int MODE>
-__device__ __forceinline__ void finish_particle(const StepConsts& C, int i, float4 p, float4 v, uint32_t pid,
-                                                float rho, Vec3 P, Vec3 V, float4* __restrict__ posOut,
-                                                float4* __restrict__ velOut, uint32_t* __restrict__ pidOut,
-                                                uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
-                                                uint64_t* __restrict__ rng, const PipeTable* __restrict__ pipe,
-                                                float4* __restrict__ dbgP, float4* __restrict__ dbgV,
-                                                float4* __restrict__ dbgF, uint32_t* __restrict__ flags) {
-    const float fx = C.ext[0] + -P.x + V.x;
-    const float fy = C.ext[1] + -P.y + V.y;
-    const float fz = C.ext[2] + -P.z + V.z;
-    Vec3 vv{v.x + fx / rho * C.dt, v.y + fy / rho * C.dt, v.z + fz / rho * C.dt};
-    Vec3 pp{p.x + vv.x * C.dt, p.y + vv.y * C.dt, p.z + vv.z * C.dt};
-    if (MODE == SPH_MODE_PIPE) {
-        collide_pipe(C, *pipe, pp, vv, rng, pid);
-    } else {
-        collide_box(C, pp.x, vv.x, 0);
-        collide_box(C, pp.y, vv.y, 1);
-        collide_box(C, pp.z, vv.z, 2);
-    }
-    posOut[i] = make_float4(pp.x, pp.y, pp.z, rho);
-    velOut[i] = make_float4(vv.x, vv.y, vv.z, 0.0f);
-    pidOut[i] = pid;
-    if (dbgP) {
-        dbgP[i] = make_float4(P.x, P.y, P.z, 0.0f);
-        dbgV[i] = make_float4(V.x, V.y, V.z, 0.0f);
-        dbgF[i] = make_float4(fx, fy, fz, 0.0f);
-    }
-    hash_particle(C, pp.x, pp.y, pp.z, pid, (uint32_t)i, keyI, valI, flags);
-}
-
-// Pressure and viscosity sums of one pair (naive_kernels.py:38-52, :70-77) without the
-// constant factors.  q = (x, y, z, p) and vj = (vx, vy, vz, 1/rho) of the neighbour.
-struct PairSums {
-    float ax = 0.f, ay = 0.f, az = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
-    // self = the pair is the particle itself: 1/r is taken as 0, which makes both terms zero
-    // (the pressure term carries a factor 1/r, the viscosity term a factor v_j - v_i = 0)
-    __device__ __forceinline__ void add(const StepConsts& C, float4 p, float4 v, float4 q, float4 vj,
-                                        bool self = false) {
-        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        const float rinv = self ? 0.0f : rsqrtf(r2);
-        const float hr = C.h - r2 * rinv;
-        const float cp = (p.w + q.w) * vj.w * hr * hr * rinv;
-        ax += cp * dx;
-        ay += cp * dy;
-        az += cp * dz;
-        const float cv = hr * vj.w;
-        bx += cv * (vj.x - v.x);
-        by += cv * (vj.y - v.y);
-        bz += cv * (vj.z - v.z);
-    }
-};
-
 // Force pass (naive_kernels.py:24-80) fused with integration and collision: each lane
 // walks its particle's neighbour list (one coalesced load per slot, kForceUnroll slots and
 // their gathers in flight); a particle with more neighbours than list slots rescans its
 // candidates instead.  MODE selects the wall collision at compile time.
-#ifndef SPH_FORCE_UNROLL
-#define SPH_FORCE_UNROLL 4
-#endif
-constexpr int kForceUnroll = SPH_FORCE_UNROLL;
-
 template <int MODE, bool BATCH>
 __global__ __launch_bounds__(256) void naive_force_kernel(
     StepConsts G, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
@@ -508,41 +196,8 @@ __global__ __launch_bounds__(256) void naive_force_kernel(
     const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= G.n) return;
     const StepConsts& C = inst_consts<BATCH>(G, i);
-    const __amdgpu_buffer_rsrc_t rpos = make_rsrc(pos, (uint32_t)G.n * 16u);
-    const __amdgpu_buffer_rsrc_t rvel = make_rsrc(vel, (uint32_t)G.n * 16u);
     const float4 p = pos[i], v = vel[i];
-    PairSums S;
-    const uint32_t cnt = nn[i];
-    if (cnt <= (uint32_t)kListCap) {
-        const uint32_t* slot = nl + i;
-        const size_t stride = (size_t)G.n;
-        uint32_t t = 0;
-        for (; t + kForceUnroll <= cnt; t += kForceUnroll) {
-            uint32_t k[kForceUnroll];
-            float4 q[kForceUnroll], w[kForceUnroll];
-#pragma unroll
-            for (int u = 0; u < kForceUnroll; ++u) k[u] = slot[u * stride];
-            slot += kForceUnroll * stride;
-#pragma unroll
-            for (int u = 0; u < kForceUnroll; ++u) {
-                q[u] = ld_f4(rpos, k[u] * 16u);
-                w[u] = ld_f4(rvel, k[u] * 16u);
-            }
-#pragma unroll
-            for (int u = 0; u < kForceUnroll; ++u) S.add(C, p, v, q[u], w[u], k[u] == (uint32_t)i);
-        }
-        for (; t < cnt; ++t) {
-            const uint32_t k = slot[0];
-            slot += stride;
-            S.add(C, p, v, ld_f4(rpos, k * 16u), ld_f4(rvel, k * 16u), k == (uint32_t)i);
-        }
-    } else {
-        for_each_candidate(p.x, p.y, p.z, C, cs, [&](uint32_t k) {
-            const float4 q = pos[k];
-            const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-            if (dx * dx + dy * dy + dz * dz <= C.h2 && k != (uint32_t)i) S.add(C, p, v, q, vel[k]);
-        });
-    }
+    const PairSums S = naive_pair_sums<BATCH>(C, i, G.n, p, v, pos, vel, cs, nl, nn);
     const float ps = 0.5f * C.mass * C.grad_c;   // naive_kernels.py:40-52: m * (pi+pj)/(2 rho_j) * grad W
     const float vs = C.mass * C.visc * C.lap_c;  // naive_kernels.py:71-77: m * mu * (vj-vi)/rho_j * lap W
     finish_particle<MODE>(C, i, p, v, pidA[i], rhoA[i], Vec3{ps * S.ax, ps * S.ay, ps * S.az},

@@ -8,7 +8,8 @@ from .data_classes import Pipe, Segment, SimulationParameters, SimulationState
 
 __all__ = [
     "SPHConstants", "MAX_NEIGHBOURS", "RNG_SEED", "Pipe", "Segment", "SimulationParameters", "SimulationState",
-    "SPHStrategy", "NaiveSPHStrategy", "VoxelSPHStrategy", "make_strategy", "StateGenerator",
+    "SPHStrategy", "NaiveSPHStrategy", "VoxelSPHStrategy", "make_strategy", "StateGenerator", "BatchedSPH",
+    "DifferentiableSPH", "rollout",
 ]
 
 
@@ -19,4 +20,10 @@ def __getattr__(name):
     if name == "StateGenerator":
         from .state_generator import StateGenerator
         return StateGenerator
+    if name == "BatchedSPH":
+        from .batched import BatchedSPH
+        return BatchedSPH
+    if name in ("DifferentiableSPH", "rollout"):
+        from . import differentiable
+        return getattr(differentiable, name)
     raise AttributeError(name)

@@ -19,6 +19,7 @@ from ._build import LIB_PATH
 SPH_MODE_BOX, SPH_MODE_PIPE = 0, 1
 SPH_FORM_NAIVE, SPH_FORM_VOXEL = 0, 1
 SPH_FLAG_KEEP_TERMS = 1
+SPH_FLAG_ADJOINT = 2
 
 STAGE_SORT, STAGE_TABLE, STAGE_REORDER, STAGE_DENSITY, STAGE_FORCE = 1, 2, 3, 4, 5
 
@@ -57,7 +58,8 @@ EXPORTED = [
     "sph_abi_version", "sph_last_error", "sph_create", "sph_destroy", "sph_workspace_bytes", "sph_bind_workspace",
     "sph_upload_state", "sph_upload_state_f32", "sph_step", "sph_download_state", "sph_download_state_f32",
     "sph_run_stage", "sph_buffer_elems", "sph_read_buffer", "sph_diagnostics", "sph_grid_info", "sph_set_launch",
-    "sph_create_batch", "sph_batch_info",
+    "sph_create_batch", "sph_batch_info", "sph_set_constants", "sph_adjoint_seed", "sph_adjoint_step",
+    "sph_adjoint_read",
 ]
 
 _lib = None
@@ -93,6 +95,10 @@ def load_library() -> ctypes.CDLL:
         "sph_set_launch": ([vp, i32, i32], i32),
         "sph_create_batch": ([POINTER(SphParamsC), POINTER(SphConstantsC), i32, POINTER(vp)], i32),
         "sph_batch_info": ([vp, POINTER(i32), POINTER(i32)], i32),
+        "sph_set_constants": ([vp, POINTER(SphConstantsC)], i32),
+        "sph_adjoint_seed": ([vp, vp, vp, vp], i32),
+        "sph_adjoint_step": ([vp, vp], i32),
+        "sph_adjoint_read": ([vp, vp, vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -197,9 +203,13 @@ class NativeContext:
                                           _stream(self.device)), "sph_download_state")
         return (pos, vel, rho, f) if force else (pos, vel, rho)
 
-    def download_f32(self):
-        """Returns seven float32 device tensors x, y, z, vx, vy, vz, rho."""
-        out = [torch.empty((self.n,), dtype=torch.float32, device=self.device) for _ in range(7)]
+    def download_f32(self, out=None):
+        """Returns seven float32 device tensors x, y, z, vx, vy, vz, rho; `out` may give the
+        destination tensors (each (N,) contiguous, or None to skip that field)."""
+        if out is None:
+            out = [torch.empty((self.n,), dtype=torch.float32, device=self.device) for _ in range(7)]
+        for t in out:
+            assert t is None or (t.dtype == torch.float32 and t.shape == (self.n,) and t.is_contiguous())
         check(self.lib.sph_download_state_f32(self.handle, *[_ptr(t) for t in out], _stream(self.device)),
               "sph_download_state_f32")
         return out
@@ -227,6 +237,29 @@ class NativeContext:
 
     def set_launch(self, density_block: int, force_block: int):
         check(self.lib.sph_set_launch(self.handle, density_block, force_block), "sph_set_launch")
+
+    def set_constants(self, consts: SphConstantsC):
+        check(self.lib.sph_set_constants(self.handle, ctypes.byref(consts)), "sph_set_constants")
+
+    # -- reverse mode (SPH_FLAG_ADJOINT) --------------------------------------------------
+    def adjoint_seed(self, gx: torch.Tensor = None, gv: torch.Tensor = None):
+        """gx, gv: float32 (N, 3) contiguous device tensors in particle order, or None (zero)."""
+        for t in (gx, gv):
+            assert t is None or (t.dtype == torch.float32 and t.shape == (self.n, 3) and t.is_contiguous())
+        check(self.lib.sph_adjoint_seed(self.handle, _ptr(gx), _ptr(gv), _stream(self.device)), "sph_adjoint_seed")
+
+    def adjoint_step(self):
+        check(self.lib.sph_adjoint_step(self.handle, _stream(self.device)), "sph_adjoint_step")
+
+    def adjoint_read(self):
+        """Returns float32 device tensors gx (N, 3), gv (N, 3) and dL/dmu (1,)."""
+        kw = dict(dtype=torch.float32, device=self.device)
+        gx = torch.empty((self.n, 3), **kw)
+        gv = torch.empty((self.n, 3), **kw)
+        gmu = torch.empty((1,), **kw)
+        check(self.lib.sph_adjoint_read(self.handle, _ptr(gx), _ptr(gv), _ptr(gmu), _stream(self.device)),
+              "sph_adjoint_read")
+        return gx, gv, gmu
 
 
 def make_params(n, external_force, dt, space_size, voxel_size, mode, formulation, pipe_table=None, flags=0,

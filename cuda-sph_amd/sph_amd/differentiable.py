@@ -1,0 +1,129 @@
+"""Differentiable rollout of the NAIVE box step: gradients of a loss on the state after T
+steps with respect to the initial velocities, the initial positions and the viscosity.
+
+The reference has no gradients; the forward map is exactly its step (sim/src/sph/
+strategies/naive_sph_strategy.py with the box collision of base_kernels.py:75-98), run in
+FP32 by libsph_amd.  The backward pass is hand-written (libsph_amd's sph_adjoint_* kernels):
+the rollout keeps the state every `checkpoint_every` steps, and for each segment, last to
+first, recomputes the segment's states from its checkpoint and pulls the cotangent back
+through the steps one at a time.  Memory: (T / k + k) states of 24 bytes per particle.
+
+    sim = DifferentiableSPH(params, constants, steps=200, checkpoint_every=20)
+    xT, vT = rollout(sim, x0, v0, mu)          # x0, v0 (N, 3) float32 cuda; mu a tensor
+    loss = ((xT - target) ** 2).sum(1).mean()
+    loss.backward()                            # fills v0.grad, mu.grad (and x0.grad)
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import replace
+from typing import Optional, Tuple
+
+import torch
+
+from . import _native as N
+from .constants import SPHConstants
+from .data_classes import SimulationParameters
+
+
+class Tape:
+    """What the backward pass needs of one forward rollout."""
+
+    def __init__(self, checkpoints: torch.Tensor, visc: float, steps: int):
+        self.checkpoints = checkpoints  # (n_ck, 6, N) float32: x, y, z, vx, vy, vz rows
+        self.visc = visc
+        self.steps = steps
+
+
+class DifferentiableSPH:
+    def __init__(self, params: SimulationParameters, constants: SPHConstants, steps: int, checkpoint_every: int = 20,
+                 device="cuda:0"):
+        if steps < 1 or checkpoint_every < 1:
+            raise ValueError("steps and checkpoint_every must be positive")
+        self.params = params
+        self.constants = constants
+        self.steps = int(steps)
+        self.every = int(checkpoint_every)
+        p = N.make_params(params.particle_count, params.external_force, 1.0 / params.fps, params.space_size,
+                          params.voxel_size, N.SPH_MODE_BOX, N.SPH_FORM_NAIVE, flags=N.SPH_FLAG_ADJOINT)
+        self.ctx = N.NativeContext(p, N.make_consts(constants), device)
+        self.device = self.ctx.device
+        self.n = self.ctx.n
+        self._seg = torch.empty((self.every, 6, self.n), dtype=torch.float32, device=self.device)
+
+    @property
+    def num_checkpoints(self) -> int:
+        return math.ceil(self.steps / self.every)
+
+    def set_viscosity(self, mu: float) -> None:
+        self.ctx.set_constants(N.make_consts(replace(self.constants, visc=float(mu))))
+
+    # -- state rows --------------------------------------------------------------------
+    def _save(self, rows: torch.Tensor) -> None:
+        self.ctx.download_f32([rows[0], rows[1], rows[2], rows[3], rows[4], rows[5], None])
+
+    def _load(self, rows: torch.Tensor) -> None:
+        self.ctx.upload_f32(rows[0], rows[1], rows[2], rows[3], rows[4], rows[5])
+
+    def _rows(self, x: torch.Tensor, v: torch.Tensor) -> torch.Tensor:
+        x = x.detach().to(self.device, torch.float32).reshape(self.n, 3)
+        v = v.detach().to(self.device, torch.float32).reshape(self.n, 3)
+        return torch.cat([x.t(), v.t()]).contiguous()
+
+    # -- passes ------------------------------------------------------------------------
+    def forward(self, x0: torch.Tensor, v0: torch.Tensor, mu: float) -> Tuple[torch.Tensor, torch.Tensor, Tape]:
+        """Runs the rollout; returns x_T, v_T (float32 (N, 3), particle order) and the tape."""
+        self.set_viscosity(mu)
+        ck = torch.empty((self.num_checkpoints, 6, self.n), dtype=torch.float32, device=self.device)
+        ck[0] = self._rows(x0, v0)
+        self._load(ck[0])
+        for t in range(1, self.steps):
+            self.ctx.step(1)
+            if t % self.every == 0:
+                self._save(ck[t // self.every])
+        self.ctx.step(1)
+        out = torch.empty((6, self.n), dtype=torch.float32, device=self.device)
+        self._save(out)
+        return out[:3].t().contiguous(), out[3:].t().contiguous(), Tape(ck, float(mu), self.steps)
+
+    def backward(self, tape: Tape, gx: Optional[torch.Tensor], gv: Optional[torch.Tensor]):
+        """Cotangents of x_T, v_T (float32 (N, 3) or None) -> (dL/dx0, dL/dv0, dL/dmu)."""
+        self.set_viscosity(tape.visc)
+        f32 = dict(dtype=torch.float32, device=self.device)
+        gx = None if gx is None else gx.detach().to(**f32).reshape(self.n, 3).contiguous()
+        gv = None if gv is None else gv.detach().to(**f32).reshape(self.n, 3).contiguous()
+        self.ctx.adjoint_seed(gx, gv)
+        for seg in reversed(range(self.num_checkpoints)):
+            t0 = seg * self.every
+            length = min(self.every, tape.steps - t0)
+            self._load(tape.checkpoints[seg])
+            for a in range(1, length):
+                self.ctx.step(1)
+                self._save(self._seg[a])
+            for a in reversed(range(length)):
+                self._load(tape.checkpoints[seg] if a == 0 else self._seg[a])
+                self.ctx.adjoint_step()
+        return self.ctx.adjoint_read()
+
+
+class _Rollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x0, v0, mu, sim):
+        xT, vT, tape = sim.forward(x0, v0, float(mu))
+        ctx.sim, ctx.tape = sim, tape
+        ctx.meta = [(t.dtype, t.device, t.shape) for t in (x0, v0, mu)]
+        return xT, vT
+
+    @staticmethod
+    def backward(ctx, gxT, gvT):
+        gx0, gv0, gmu = ctx.sim.backward(ctx.tape, gxT, gvT)
+        (dx, devx, sx), (dv, devv, sv), (dm, devm, sm) = ctx.meta
+        return (gx0.to(devx, dx).reshape(sx), gv0.to(devv, dv).reshape(sv), gmu.to(devm, dm).reshape(sm), None)
+
+
+def rollout(sim: DifferentiableSPH, x0: torch.Tensor, v0: torch.Tensor, mu: torch.Tensor):
+    """x_T, v_T after sim.steps steps from (x0, v0) at viscosity mu, differentiable with
+    respect to x0, v0 and mu (a tensor; a 0-d or one-element float tensor)."""
+    if not torch.is_tensor(mu):
+        mu = torch.tensor(float(mu), dtype=torch.float64)
+    return _Rollout.apply(x0, v0, mu, sim)

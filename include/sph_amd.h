@@ -74,6 +74,9 @@ typedef struct {
 /* Keep the per-particle pressure term, viscosity term and result force of the last step
  * (readable with sph_read_buffer).  Off by default: they cost 48 bytes per particle. */
 #define SPH_FLAG_KEEP_TERMS 1
+/* Allocate the adjoint buffers (sph_adjoint_*): 64 bytes per particle more.  Plain
+ * NAIVE-formulation contexts in BOX mode only. */
+#define SPH_FLAG_ADJOINT 2
 
 typedef struct sph_ctx sph_ctx;
 
@@ -195,6 +198,34 @@ int sph_grid_info(const sph_ctx* ctx, int32_t* dims3, int32_t* stencil3, int32_t
 
 /* Launch shapes of the neighbour kernels (threads per block), for the autotuner. */
 int sph_set_launch(sph_ctx* ctx, int32_t density_block, int32_t force_block);
+
+/* Replaces the physics constants of a plain context between steps (the reference freezes
+ * them at JIT time, config.py:18-23).  mass, rho0, visc, k and damp may change; h must be
+ * the one the context was created with (it fixes the voxel grid).  Host only. */
+int sph_set_constants(sph_ctx* ctx, const sph_constants_t* consts);
+
+/* ---------------------------------------------------------------------------------------
+ * Reverse mode (no reference counterpart: the reference has no gradients).  For a context
+ * created with SPH_FLAG_ADJOINT the library holds one cotangent (gx, gv) of a state, in
+ * particle order, and pulls it back through steps one at a time:
+ *   sph_adjoint_seed   sets the cotangent of the state the rollout ended in (dL/dx_T,
+ *                      dL/dv_T) and clears the viscosity gradient
+ *   sph_adjoint_step   with the state x_t resident (uploaded, e.g. from a checkpoint),
+ *                      advances it to x_{t+1} exactly as sph_step does and, in the same
+ *                      pass, maps the cotangent of x_{t+1} to that of x_t and adds the
+ *                      step's dL/dmu
+ *   sph_adjoint_read   the current cotangent and the accumulated dL/dmu
+ * A rollout of T steps is differentiated by visiting t = T-1 .. 0 with x_t re-uploaded
+ * before each call (states recomputed from checkpoints).  The pullback is the exact
+ * derivative of the FP32 step, wall bounces included (a bounced component's position is
+ * constant and its velocity is scaled by -damp).
+ * --------------------------------------------------------------------------------------- */
+/* dev_gx, dev_gv: float32 (N, 3) row-major in particle order; NULL means zero. */
+int sph_adjoint_seed(sph_ctx* ctx, const float* dev_gx, const float* dev_gv, void* stream);
+int sph_adjoint_step(sph_ctx* ctx, void* stream);
+/* dev_gx, dev_gv: float32 (N, 3) row-major in particle order; dev_gmu: one float32.  Any
+ * pointer may be NULL. */
+int sph_adjoint_read(sph_ctx* ctx, float* dev_gx, float* dev_gv, float* dev_gmu, void* stream);
 
 /* Message of the last error on this thread (always NUL-terminated). */
 int sph_last_error(char* buf, size_t len);

@@ -21,7 +21,7 @@ __all__ = [
     "voxel_begin", "voxel_neighbours", "rng_init", "rng_next", "rng_jump", "rng_uniform", "splitmix64",
     "find_segment", "x_at_segment_beginning", "radius_in_position", "is_out_of_pipe", "vector_length",
     "distance_between_points", "solve_collision", "collide_box", "collide_pipe", "integrate", "w", "grad_w",
-    "lap_w", "set_threads", "num_threads", "voxel_forces",
+    "lap_w", "set_threads", "num_threads", "voxel_forces", "rollout", "rollout_adjoint", "adjoint_step",
 ]
 
 
@@ -86,6 +86,10 @@ def lib() -> ctypes.CDLL:
             "oc_voxel_forces": ([D, D, c_int64, I32, I32, POINTER(OracleConsts), D, D, D], None),
             "oc_step": ([POINTER(OracleParams), POINTER(OracleConsts), D, D, D, D, U64], c_int),
             "oc_grid_naive_terms": ([D, D, c_int64, POINTER(OracleConsts), D, D, D], None),
+            "oc_rollout": ([POINTER(OracleParams), POINTER(OracleConsts), D, D, c_int, D, D, c_void_p, c_int], c_int),
+            "oc_adjoint_step": ([POINTER(OracleParams), POINTER(OracleConsts), D, D, D, D, D, D, D], c_int),
+            "oc_rollout_adjoint": ([POINTER(OracleParams), POINTER(OracleConsts), D, D, c_int, D, c_int, D, D, D],
+                                   c_int),
             "oc_num_threads": ([], c_int),
             "oc_set_num_threads": ([c_int], None),
         }
@@ -342,3 +346,66 @@ def grad_w(consts, a, b):
 
 def lap_w(consts, a, b) -> float:
     return float(lib().oc_lap_w(ctypes.byref(consts_of(consts)), _d(_f64(a)), _d(_f64(b))))
+
+
+# -- rollout and its adjoint (NAIVE, box walls) ------------------------------------------
+def _box_params(n, ext, dt, space) -> OracleParams:
+    p = OracleParams()
+    p.n = n
+    p.ext[:] = [float(v) for v in ext]
+    p.dt = float(dt)
+    p.space[:] = [float(v) for v in space]
+    p.voxel[:] = [1.0, 1.0, 1.0]
+    p.mode, p.formulation, p.pipe_rows, p.use_grid = 0, 0, 0, 1
+    return p
+
+
+def rollout(pos, vel, *, consts, ext, dt, space, steps, branches=None, branch_mode=0):
+    """Positions and velocities after `steps` NAIVE box steps (FP64).  `branches`, a uint8
+    array of steps * 3N, records (branch_mode 1) or replays (2) the wall-bounce decisions."""
+    pos = _f64(pos, (-1, 3))
+    vel = _f64(vel, (-1, 3))
+    n = pos.shape[0]
+    xT = np.zeros_like(pos)
+    vT = np.zeros_like(vel)
+    c = consts_of(consts)
+    bp = None
+    if branches is not None:
+        assert branches.dtype == np.uint8 and branches.flags.c_contiguous and branches.size >= steps * 3 * n
+        bp = branches.ctypes.data_as(c_void_p)
+    lib().oc_rollout(ctypes.byref(_box_params(n, ext, dt, space)), ctypes.byref(c), _d(pos), _d(vel), int(steps),
+                     _d(xT), _d(vT), bp, int(branch_mode) if bp is not None else 0)
+    return xT, vT
+
+
+def adjoint_step(pos, vel, gx1, gv1, *, consts, ext, dt, space):
+    """Adjoint of one NAIVE box step at (pos, vel): returns (gx0, gv0, dL/dmu) for the
+    cotangents (gx1, gv1) of the step's output."""
+    pos = _f64(pos, (-1, 3))
+    vel = _f64(vel, (-1, 3))
+    gx1 = _f64(gx1, (-1, 3))
+    gv1 = _f64(gv1, (-1, 3))
+    n = pos.shape[0]
+    gx0 = np.zeros_like(pos)
+    gv0 = np.zeros_like(vel)
+    gmu = c_double(0.0)
+    lib().oc_adjoint_step(ctypes.byref(_box_params(n, ext, dt, space)), ctypes.byref(consts_of(consts)), _d(pos),
+                          _d(vel), _d(gx1), _d(gv1), _d(gx0), _d(gv0), ctypes.byref(gmu))
+    return gx0, gv0, gmu.value
+
+
+def rollout_adjoint(pos, vel, *, consts, ext, dt, space, steps, target, every=20):
+    """Loss (1/N) sum |x_T - target|^2 and its gradients with respect to the initial
+    velocities (N, 3) and the viscosity, by the FP64 reverse pass with checkpoints every
+    `every` steps.  Returns (loss, grad_v0, grad_mu)."""
+    pos = _f64(pos, (-1, 3))
+    vel = _f64(vel, (-1, 3))
+    target = _f64(target, (-1, 3))
+    n = pos.shape[0]
+    g = np.zeros_like(vel)
+    loss = c_double(0.0)
+    gmu = c_double(0.0)
+    c = consts_of(consts)
+    lib().oc_rollout_adjoint(ctypes.byref(_box_params(n, ext, dt, space)), ctypes.byref(c), _d(pos), _d(vel),
+                             int(steps), _d(target), int(every), ctypes.byref(loss), _d(g), ctypes.byref(gmu))
+    return loss.value, g, gmu.value

@@ -814,3 +814,317 @@ OC_API void oc_set_num_threads(int t) {
     (void)t;
 #endif
 }
+
+/* ------------------------------------------------------------------------------------
+ * Rollout and its adjoint (NAIVE formulation, box walls), FP64.  No reference
+ * counterpart: the reference has no gradients.  This is the reference gradient for the
+ * GPU adjoint kernels and is itself checked against central finite differences.
+ *
+ * Loss = (1/N) sum_i |x_T,i - target_i|^2; gradients with respect to the initial
+ * velocities and the viscosity mu.  The step is the one of oc_step (NAIVE, BOX):
+ *   rho_i = m sum_j W_ij (self included), p = K (rho - rho0),
+ *   P_i = m sum_j (p_i + p_j) / (2 rho_j) G(x_i - x_j), G(d) = Gc (h - r)^2 d / r,
+ *   V_i = m mu sum_j (v_j - v_i) / rho_j L(r_ij),   L(r) = Lc (h - r),
+ *   F = f - P + V, v' = v + F / rho dt, x' = x + v' dt, then the box collision.
+ * Every transposed sum of the adjoint is a sum over the (symmetric) neighbour set.
+ * ---------------------------------------------------------------------------------- */
+typedef struct {
+    int64_t* start; /* n + 1 */
+    int64_t* idx;
+} oc_nlist_t;
+
+static void build_nlist(const double* pos, int64_t n, double h, oc_nlist_t* L) {
+    oc_grid_t G;
+    grid_build(&G, pos, n, h);
+    double h2 = h * h;
+    int64_t* cnt = (int64_t*)calloc((size_t)(n + 1), sizeof(int64_t));
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t found = 0; /* the macro's own locals are x, y, z, c, e, j */
+        GRID_FOR_NEIGHBOURS(&G, pos, i, h2, { if (j != i) ++found; });
+        cnt[i] = found;
+    }
+    L->start = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    L->start[0] = 0;
+    for (int64_t i = 0; i < n; ++i) L->start[i + 1] = L->start[i] + cnt[i];
+    L->idx = (int64_t*)malloc(sizeof(int64_t) * (size_t)(L->start[n] + 1));
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t o = L->start[i];
+        GRID_FOR_NEIGHBOURS(&G, pos, i, h2, { if (j != i) L->idx[o++] = j; });
+    }
+    free(cnt);
+    grid_free(&G);
+}
+
+static void free_nlist(oc_nlist_t* L) { free(L->start); free(L->idx); }
+
+typedef struct {
+    double *rho, *P, *V, *F; /* n, 3n, 3n, 3n */
+    oc_nlist_t nl;
+} oc_fwd_t;
+
+/* Box collision of base_kernels.py:75-98 with its branch per component exposed:
+ * 0 = no wall, 1 = below 0 (x -> 1e-3), 2 = above space (x -> space - 1e-3).  bmode 0 decides
+ * as the reference does, 1 decides and records, 2 replays recorded decisions (the same
+ * smooth piece of the rollout map, for finite differences that do not step across a
+ * change of bounce step). */
+static void collide_box_branch(double* pos, double* vel, int64_t n, const double* space, double damp, uint8_t* br,
+                               int bmode) {
+    for (int64_t i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) {
+            size_t k = 3 * (size_t)i + d;
+            double x = pos[k];
+            uint8_t b;
+            if (bmode == 2) b = br[k];
+            else b = x < 0 ? (1e-3 > space[d] ? 2 : 1) : (x > space[d] ? 2 : 0);
+            if (bmode == 1) br[k] = b;
+            if (b == 1) pos[k] = 1e-3;
+            if (b == 2) pos[k] = space[d] - 1e-3;
+            if (b) { vel[k] *= -1; vel[k] *= damp; }
+        }
+}
+
+/* one step with intermediates; x1/v1 receive the new state */
+static void fwd_step(const oc_params_t* prm, const oc_consts_t* C, const double* x, const double* v, double* x1,
+                     double* v1, oc_fwd_t* S, uint8_t* br, int bmode) {
+    int64_t n = prm->n;
+    oc_derived_t D = derive(C);
+    build_nlist(x, n, C->h, &S->nl);
+    double w0 = D.w_const * D.h2 * D.h2 * D.h2;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; ++i) {
+        double acc = w0;
+        for (int64_t e = S->nl.start[i]; e < S->nl.start[i + 1]; ++e)
+            acc += compute_w(&D, x + 3 * i, x + 3 * S->nl.idx[e]);
+        S->rho[i] = acc * C->mass;
+    }
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; ++i) {
+        double ap[3] = {0, 0, 0}, av[3] = {0, 0, 0};
+        double pi = C->k * (S->rho[i] - C->rho0);
+        for (int64_t e = S->nl.start[i]; e < S->nl.start[i + 1]; ++e) {
+            int64_t j = S->nl.idx[e];
+            double pj = C->k * (S->rho[j] - C->rho0);
+            double g[3];
+            compute_grad_w(C, &D, x + 3 * i, x + 3 * j, g);
+            double lap = compute_lap_w(C, &D, x + 3 * i, x + 3 * j);
+            for (int d = 0; d < 3; ++d) {
+                ap[d] += (pi + pj) / (2 * S->rho[j]) * g[d];
+                av[d] += (v[3 * j + d] - v[3 * i + d]) / S->rho[j] * lap * C->visc * C->mass;
+            }
+        }
+        for (int d = 0; d < 3; ++d) { S->P[3 * i + d] = ap[d] * C->mass; S->V[3 * i + d] = av[d]; }
+    }
+    memcpy(x1, x, sizeof(double) * 3 * (size_t)n);
+    memcpy(v1, v, sizeof(double) * 3 * (size_t)n);
+    oc_integrate(S->F, x1, v1, prm->ext, S->rho, S->P, S->V, n, prm->dt);
+    collide_box_branch(x1, v1, n, prm->space, C->damp, br, bmode);
+}
+
+/* adjoint of one step: (gx1, gv1) on the output -> (gx0, gv0) on the input; adds dL/dmu */
+static void adj_step(const oc_params_t* prm, const oc_consts_t* C, const double* x, const double* v,
+                     const oc_fwd_t* S, const double* gx1, const double* gv1, double* gx0, double* gv0, double* gmu) {
+    int64_t n = prm->n;
+    oc_derived_t D = derive(C);
+    double dt = prm->dt, m = C->mass, mu = C->visc;
+    double* Pb = (double*)malloc(sizeof(double) * 3 * (size_t)(n + 1));
+    double* Vb = (double*)malloc(sizeof(double) * 3 * (size_t)(n + 1));
+    double* rb = (double*)malloc(sizeof(double) * (size_t)(n + 1));
+    double mu_acc = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        rb[i] = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            size_t k = 3 * (size_t)i + d;
+            /* recompute the pre-collision state (base_kernels.py:46-53) */
+            double vpre = v[k] + S->F[k] / S->rho[i] * dt;
+            double xpre = x[k] + vpre * dt;
+            int bounced = (xpre < 0) || ((xpre < 0 ? 1e-3 : xpre) > prm->space[d]);
+            double gxp = bounced ? 0.0 : gx1[k];
+            double gvp = (bounced ? -C->damp * gv1[k] : gv1[k]) + gxp * dt;
+            gx0[k] = gxp;
+            gv0[k] = gvp;
+            double Fb = gvp * dt / S->rho[i];
+            rb[i] += -gvp * S->F[k] * dt / (S->rho[i] * S->rho[i]);
+            Pb[k] = -Fb;
+            Vb[k] = Fb;
+            mu_acc += Fb * S->V[k] / mu;
+        }
+    }
+    *gmu += mu_acc;
+    double* rtot = (double*)malloc(sizeof(double) * (size_t)(n + 1));
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t k = 0; k < n; ++k) {
+        double pk = C->k * (S->rho[k] - C->rho0), rk = S->rho[k];
+        double pbar = 0.0, rbar = rb[k];
+        double gx[3] = {0, 0, 0}, gv[3] = {0, 0, 0};
+        for (int64_t e = S->nl.start[k]; e < S->nl.start[k + 1]; ++e) {
+            int64_t j = S->nl.idx[e];
+            double pj = C->k * (S->rho[j] - C->rho0), rj = S->rho[j];
+            double d3[3], r2 = 0.0;
+            for (int d = 0; d < 3; ++d) { d3[d] = x[3 * k + d] - x[3 * j + d]; r2 += d3[d] * d3[d]; }
+            double r = sqrt(r2), hr = C->h - r;
+            double L = D.lap_w_const * hr;
+            const double *Vk = Vb + 3 * k, *Vj = Vb + 3 * j, *Pk = Pb + 3 * k, *Pj = Pb + 3 * j;
+            /* viscosity */
+            double akj = 0.0, ajk = 0.0, gvk = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                gv[d] += m * mu * (L / rk * Vj[d] - L / rj * Vk[d]);
+                akj += Vk[d] * (v[3 * j + d] - v[3 * k + d]);
+                ajk += Vj[d] * (v[3 * k + d] - v[3 * j + d]);
+            }
+            gvk = ajk;
+            rbar += -(m * mu / (rk * rk)) * gvk * L;
+            akj *= m * mu / rj;
+            ajk *= m * mu / rk;
+            for (int d = 0; d < 3; ++d) gx[d] += -D.lap_w_const * (akj + ajk) * d3[d] / r;
+            /* pressure */
+            double g = hr * hr / r, gp = -hr * (C->h + r) / (r * r);
+            double PkG = 0.0, PjG = 0.0;
+            for (int d = 0; d < 3; ++d) { PkG += Pk[d] * D.grad_w_const * g * d3[d]; PjG += Pj[d] * D.grad_w_const * g * (-d3[d]); }
+            pbar += m * (PkG / (2 * rj) + PjG / (2 * rk));
+            rbar += -m / (2 * rk * rk) * (pj + pk) * PjG;
+            double ckj = (pk + pj) / (2 * rj), cjk = (pj + pk) / (2 * rk);
+            double b[3], db = 0.0;
+            for (int d = 0; d < 3; ++d) { b[d] = m * (ckj * Pk[d] - cjk * Pj[d]); db += d3[d] * b[d]; }
+            for (int d = 0; d < 3; ++d) gx[d] += D.grad_w_const * (g * b[d] + gp * db * d3[d] / r);
+        }
+        rtot[k] = rbar + C->k * pbar;
+        for (int d = 0; d < 3; ++d) { gx0[3 * k + d] += gx[d]; gv0[3 * k + d] += gv[d]; }
+    }
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t k = 0; k < n; ++k) {
+        double gx[3] = {0, 0, 0};
+        for (int64_t e = S->nl.start[k]; e < S->nl.start[k + 1]; ++e) {
+            int64_t j = S->nl.idx[e];
+            double d3[3], r2 = 0.0;
+            for (int d = 0; d < 3; ++d) { d3[d] = x[3 * k + d] - x[3 * j + d]; r2 += d3[d] * d3[d]; }
+            double t = D.h2 - r2;
+            double s = m * (rtot[k] + rtot[j]) * (-6.0 * D.w_const * t * t);
+            for (int d = 0; d < 3; ++d) gx[d] += s * d3[d];
+        }
+        for (int d = 0; d < 3; ++d) gx0[3 * k + d] += gx[d];
+    }
+    free(Pb); free(Vb); free(rb); free(rtot);
+}
+
+/* x_T of a rollout (NAIVE, BOX).  branches (steps x 3n bytes) with bmode 1 records the
+ * bounce decisions, with bmode 2 replays them; NULL / 0 for the plain rollout. */
+OC_API int oc_rollout(const oc_params_t* prm, const oc_consts_t* C, const double* x0, const double* v0, int steps,
+                      double* xT, double* vT, uint8_t* branches, int bmode) {
+    int64_t n = prm->n;
+    double* x = (double*)malloc(sizeof(double) * 3 * (size_t)(n + 1));
+    double* v = (double*)malloc(sizeof(double) * 3 * (size_t)(n + 1));
+    double* x1 = (double*)malloc(sizeof(double) * 3 * (size_t)(n + 1));
+    double* v1 = (double*)malloc(sizeof(double) * 3 * (size_t)(n + 1));
+    oc_fwd_t S;
+    S.rho = (double*)malloc(sizeof(double) * (size_t)(n + 1));
+    S.P = (double*)malloc(sizeof(double) * 3 * (size_t)(n + 1));
+    S.V = (double*)malloc(sizeof(double) * 3 * (size_t)(n + 1));
+    S.F = (double*)malloc(sizeof(double) * 3 * (size_t)(n + 1));
+    memcpy(x, x0, sizeof(double) * 3 * (size_t)n);
+    memcpy(v, v0, sizeof(double) * 3 * (size_t)n);
+    if (!branches) bmode = 0;
+    for (int t = 0; t < steps; ++t) {
+        fwd_step(prm, C, x, v, x1, v1, &S, bmode ? branches + (size_t)t * 3 * (size_t)n : NULL, bmode);
+        free_nlist(&S.nl);
+        double* tmp = x; x = x1; x1 = tmp;
+        tmp = v; v = v1; v1 = tmp;
+    }
+    memcpy(xT, x, sizeof(double) * 3 * (size_t)n);
+    if (vT) memcpy(vT, v, sizeof(double) * 3 * (size_t)n);
+    free(x); free(v); free(x1); free(v1); free(S.rho); free(S.P); free(S.V); free(S.F);
+    return 0;
+}
+
+/* loss and gradients; states are kept every `every` steps and recomputed in between */
+OC_API int oc_rollout_adjoint(const oc_params_t* prm, const oc_consts_t* C, const double* x0, const double* v0,
+                              int steps, const double* target, int every, double* loss, double* g_v0, double* g_mu) {
+    int64_t n = prm->n;
+    size_t sz = sizeof(double) * 3 * (size_t)(n + 1);
+    if (every < 1) every = 1;
+    int nck = (steps + every - 1) / every + 1;
+    double* ckx = (double*)malloc(sz * (size_t)nck);
+    double* ckv = (double*)malloc(sz * (size_t)nck);
+    double* xs = (double*)malloc(sz * (size_t)(every + 1));
+    double* vs = (double*)malloc(sz * (size_t)(every + 1));
+    double* gx = (double*)malloc(sz);
+    double* gv = (double*)malloc(sz);
+    double* gx0 = (double*)malloc(sz);
+    double* gv0 = (double*)malloc(sz);
+    oc_fwd_t S;
+    S.rho = (double*)malloc(sizeof(double) * (size_t)(n + 1));
+    S.P = (double*)malloc(sz);
+    S.V = (double*)malloc(sz);
+    S.F = (double*)malloc(sz);
+    /* forward, keeping checkpoints */
+    memcpy(xs, x0, sizeof(double) * 3 * (size_t)n);
+    memcpy(vs, v0, sizeof(double) * 3 * (size_t)n);
+    memcpy(ckx, x0, sizeof(double) * 3 * (size_t)n);
+    memcpy(ckv, v0, sizeof(double) * 3 * (size_t)n);
+    for (int t = 0; t < steps; ++t) {
+        fwd_step(prm, C, xs, vs, xs + 3 * (n + 1), vs + 3 * (n + 1), &S, NULL, 0);
+        free_nlist(&S.nl);
+        memcpy(xs, xs + 3 * (n + 1), sizeof(double) * 3 * (size_t)n);
+        memcpy(vs, vs + 3 * (n + 1), sizeof(double) * 3 * (size_t)n);
+        if ((t + 1) % every == 0) {
+            memcpy(ckx + 3 * (n + 1) * ((t + 1) / every), xs, sizeof(double) * 3 * (size_t)n);
+            memcpy(ckv + 3 * (n + 1) * ((t + 1) / every), vs, sizeof(double) * 3 * (size_t)n);
+        }
+    }
+    double l = 0.0;
+    for (int64_t i = 0; i < 3 * n; ++i) {
+        double d = xs[i] - target[i];
+        l += d * d;
+        gx[i] = 2.0 * d / (double)n;
+        gv[i] = 0.0;
+    }
+    *loss = l / (double)n;
+    double gm = 0.0;
+    /* backward over segments */
+    for (int seg = (steps - 1) / every; seg >= 0; --seg) {
+        int t0 = seg * every, t1 = t0 + every < steps ? t0 + every : steps;
+        memcpy(xs, ckx + 3 * (n + 1) * seg, sizeof(double) * 3 * (size_t)n);
+        memcpy(vs, ckv + 3 * (n + 1) * seg, sizeof(double) * 3 * (size_t)n);
+        for (int t = t0; t < t1; ++t) {
+            int a = t - t0;
+            fwd_step(prm, C, xs + 3 * (n + 1) * a, vs + 3 * (n + 1) * a, xs + 3 * (n + 1) * (a + 1),
+                     vs + 3 * (n + 1) * (a + 1), &S, NULL, 0);
+            free_nlist(&S.nl);
+        }
+        for (int t = t1 - 1; t >= t0; --t) {
+            int a = t - t0;
+            fwd_step(prm, C, xs + 3 * (n + 1) * a, vs + 3 * (n + 1) * a, gx0, gv0, &S, NULL, 0); /* intermediates */
+            adj_step(prm, C, xs + 3 * (n + 1) * a, vs + 3 * (n + 1) * a, &S, gx, gv, gx0, gv0, &gm);
+            free_nlist(&S.nl);
+            memcpy(gx, gx0, sizeof(double) * 3 * (size_t)n);
+            memcpy(gv, gv0, sizeof(double) * 3 * (size_t)n);
+        }
+    }
+    memcpy(g_v0, gv, sizeof(double) * 3 * (size_t)n);
+    *g_mu = gm;
+    free(ckx); free(ckv); free(xs); free(vs); free(gx); free(gv); free(gx0); free(gv0);
+    free(S.rho); free(S.P); free(S.V); free(S.F);
+    return 0;
+}
+
+/* adjoint of one step at state (x, v): (gx1, gv1) on the step's output -> (gx0, gv0) on its
+ * input, and the step's dL/dmu in *gmu */
+OC_API int oc_adjoint_step(const oc_params_t* prm, const oc_consts_t* C, const double* x, const double* v,
+                           const double* gx1, const double* gv1, double* gx0, double* gv0, double* gmu) {
+    int64_t n = prm->n;
+    size_t sz = sizeof(double) * 3 * (size_t)(n + 1);
+    double* x1 = (double*)malloc(sz);
+    double* v1 = (double*)malloc(sz);
+    oc_fwd_t S;
+    S.rho = (double*)malloc(sizeof(double) * (size_t)(n + 1));
+    S.P = (double*)malloc(sz);
+    S.V = (double*)malloc(sz);
+    S.F = (double*)malloc(sz);
+    fwd_step(prm, C, x, v, x1, v1, &S, NULL, 0);
+    *gmu = 0.0;
+    adj_step(prm, C, x, v, &S, gx1, gv1, gx0, gv0, gmu);
+    free_nlist(&S.nl);
+    free(x1); free(v1); free(S.rho); free(S.P); free(S.V); free(S.F);
+    return 0;
+}

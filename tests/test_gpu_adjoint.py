@@ -1,0 +1,163 @@
+"""GPU reverse mode (sph_adjoint_*) against the FP64 oracle's adjoint and finite differences.
+
+Bars (FP32 kernels against FP64 references on the same FP32-rounded inputs):
+  one step           max |g_gpu - g_ref| <= 1e-5 * max |g_ref| for the position and the
+                     velocity cotangent; dL/dmu within 1e-5 relative (measured <= 7e-7)
+  30-step rollout    relative L2 error of dL/dv0 <= 5e-5 (measured 6e-6), dL/dmu within
+                     2e-5 (measured 5e-7), loss within 1e-6
+  FD, 40 steps       16,384 particles: 8 velocity components within 5e-4 (measured 5e-5) and
+                     mu within 1e-4 (measured 7e-7) of FP64 central differences, eps 1e-6
+The adjoint step also performs the step: its state output must equal sph_step's bitwise,
+which shows that the recomputed forward (and so every bounce decision) is the step's own.
+Beyond ~120 steps of the 16,384-particle box the rollout map becomes ill-conditioned
+(particles piling on the floor); gradients there are compared in bench.py, not here.
+"""
+from dataclasses import replace
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from sph_amd import workloads as W
+from sph_amd.differentiable import DifferentiableSPH, rollout
+
+pytestmark = pytest.mark.gpu
+
+
+def f32(a):
+    return np.asarray(a, dtype=np.float64).astype(np.float32).astype(np.float64)
+
+
+def case(name):
+    """(workload, x, v) with FP32-representable values."""
+    if name == "box":
+        w = W.box_uniform(4096, seed=21, steps=1)
+        rng = np.random.default_rng(22)
+        return w, f32(w.state.position), f32(rng.normal(0.0, 0.3, size=(4096, 3)))
+    if name == "walls":
+        # dense slab against the floor and the x = 0 wall, moving into them: bounces, and more
+        # neighbours than list slots (the rescan path)
+        w = W.box_uniform(3000, seed=23, steps=1)
+        rng = np.random.default_rng(24)
+        x = rng.uniform([0.0, 0.0, 0.0], [0.25, 0.03, 0.25], size=(3000, 3))
+        v = rng.normal(0.0, 0.3, size=(3000, 3))
+        v[:, 1] = -np.abs(v[:, 1]) * 3
+        v[:, 0] -= 0.5
+        return w, f32(x), f32(v)
+    raise KeyError(name)
+
+
+def kw(w):
+    p = w.params
+    return dict(ext=p.external_force, dt=1.0 / p.fps, space=p.space_size)
+
+
+def sim_for(w, steps=1, every=20, gpu="cuda:0"):
+    return DifferentiableSPH(w.params, w.constants, steps=steps, checkpoint_every=every, device=gpu)
+
+
+def dev(a, gpu):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(gpu)
+
+
+@pytest.mark.parametrize("name", ["box", "walls"])
+def test_adjoint_step_writes_the_step_bitwise(gpu, name):
+    w, x, v = case(name)
+    sim = sim_for(w, gpu=gpu)
+    rows = sim._rows(dev(x, gpu), dev(v, gpu))
+    sim._load(rows)
+    sim.ctx.step(1)
+    a = torch.stack(sim.ctx.download_f32())
+    sim._load(rows)
+    sim.ctx.adjoint_seed(None, None)
+    sim.ctx.adjoint_step()
+    b = torch.stack(sim.ctx.download_f32())
+    assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("name", ["box", "walls"])
+def test_adjoint_step_matches_oracle(gpu, name):
+    w, x, v = case(name)
+    rng = np.random.default_rng(5)
+    gx1 = f32(rng.normal(size=x.shape))
+    gv1 = f32(rng.normal(size=x.shape))
+    sim = sim_for(w, gpu=gpu)
+    sim._load(sim._rows(dev(x, gpu), dev(v, gpu)))
+    sim.ctx.adjoint_seed(dev(gx1, gpu), dev(gv1, gpu))
+    sim.ctx.adjoint_step()
+    gx, gv, gmu = [t.cpu().numpy().astype(np.float64) for t in sim.ctx.adjoint_read()]
+    rx, rv, rmu = oracle.adjoint_step(x, v, gx1, gv1, consts=w.constants, **kw(w))
+    ex = np.abs(gx - rx).max() / np.abs(rx).max()
+    ev = np.abs(gv - rv).max() / np.abs(rv).max()
+    em = abs(gmu[0] - rmu) / abs(rmu)
+    print(f"{name}: gx {ex:.2e} gv {ev:.2e} gmu {em:.2e} (|gx| {np.abs(rx).max():.3e}, mu {rmu:.4e})")
+    assert ex <= 1e-5 and ev <= 1e-5 and em <= 1e-5
+
+
+def test_rollout_gradient_matches_oracle(gpu):
+    steps, every = 30, 7  # segments of 7, 7, 7, 7, 2 steps
+    w = W.box_uniform(4096, seed=31, steps=steps)
+    x0, v0 = f32(w.state.position), f32(np.random.default_rng(32).normal(0.0, 0.1, size=(4096, 3)))
+    xt, _ = oracle.rollout(x0, v0, consts=replace(w.constants, visc=5.0), steps=steps, **kw(w))
+    target = xt + np.random.default_rng(33).normal(0.0, 0.01, size=xt.shape)
+    loss_ref, g_ref, gmu_ref = oracle.rollout_adjoint(x0, v0, consts=w.constants, steps=steps, target=target,
+                                                      every=every, **kw(w))
+    sim = sim_for(w, steps=steps, every=every, gpu=gpu)
+    xv = dev(x0, gpu).requires_grad_(True)
+    vv = dev(v0, gpu).requires_grad_(True)
+    mu = torch.tensor(w.constants.visc, dtype=torch.float64, requires_grad=True)
+    xT, _ = rollout(sim, xv, vv, mu)
+    loss = ((xT.double() - torch.from_numpy(target).to(gpu)) ** 2).sum(1).mean()
+    loss.backward()
+    g = vv.grad.cpu().numpy().astype(np.float64)
+    e2 = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
+    em = abs(mu.grad.item() - gmu_ref) / abs(gmu_ref)
+    el = abs(loss.item() - loss_ref) / loss_ref
+    print(f"rollout: loss {el:.2e} dv0 L2 {e2:.2e} dmu {em:.2e} ({gmu_ref:.4e})")
+    assert el <= 1e-6 and e2 <= 5e-5 and em <= 2e-5
+    assert xv.grad is not None and torch.isfinite(xv.grad).all()
+
+
+def test_backward_is_deterministic(gpu):
+    w, x, v = case("box")
+    sim = sim_for(w, steps=6, every=4, gpu=gpu)
+    xT, vT, tape = sim.forward(dev(x, gpu), dev(v, gpu), 3.5)
+    g = torch.ones_like(xT)
+    a = sim.backward(tape, g, None)
+    b = sim.backward(tape, g, None)
+    for s, t in zip(a, b):
+        assert torch.equal(s, t)
+
+
+def test_16k_finite_differences_smooth_horizon(gpu):
+    """The differentiable workload (16,384 particles, seeded box) at 40 steps, where the
+    rollout map is smooth: 8 velocity components and mu against FP64 central differences."""
+    steps, eps = 40, 1e-6
+    w = W.adjoint_16k()
+    c = w.constants
+    x0, v0 = f32(w.state.position), f32(w.state.velocity)
+    xt, _ = oracle.rollout(x0, v0, consts=replace(c, visc=5.0), steps=steps, **kw(w))
+    target = xt + np.random.default_rng(41).normal(0.0, 0.01, size=xt.shape)
+    sim = sim_for(w, steps=steps, every=20, gpu=gpu)
+    vv = dev(v0, gpu).requires_grad_(True)
+    mu = torch.tensor(c.visc, dtype=torch.float64, requires_grad=True)
+    xT, _ = rollout(sim, dev(x0, gpu), vv, mu)
+    ((xT.double() - torch.from_numpy(target).to(gpu)) ** 2).sum(1).mean().backward()
+    g = vv.grad.cpu().numpy().astype(np.float64).ravel()
+
+    def L(vel, cc):
+        xT_, _ = oracle.rollout(x0, vel, consts=cc, steps=steps, **kw(w))
+        return np.mean(np.sum((xT_ - target) ** 2, axis=1))
+
+    worst = 0.0
+    for k in np.random.default_rng(42).choice(v0.size, 8, replace=False):
+        e = np.zeros(v0.size)
+        e[k] = eps
+        fd = (L(v0 + e.reshape(v0.shape), c) - L(v0 - e.reshape(v0.shape), c)) / (2 * eps)
+        worst = max(worst, abs(g[k] - fd) / abs(fd))
+        print(f"v[{k}] adjoint {g[k]:.6e} fd {fd:.6e}")
+    fd = (L(v0, replace(c, visc=c.visc + eps)) - L(v0, replace(c, visc=c.visc - eps))) / (2 * eps)
+    emu = abs(mu.grad.item() - fd) / abs(fd)
+    print(f"mu adjoint {mu.grad.item():.6e} fd {fd:.6e}; worst velocity {worst:.2e}")
+    assert worst <= 5e-4 and emu <= 1e-4
