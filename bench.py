@@ -63,6 +63,8 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--batched", choices=["on", "off"], default="on",
                     help="also measure the batched form (64 box simulations, 300 steps, HIP graph)")
     ap.add_argument("--batch-steps", type=int, default=300)
+    ap.add_argument("--adjoint", choices=["on", "off"], default="on",
+                    help="also measure the differentiable form (16,384 particles, 200 steps, forward + backward)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain device copy of known size (for counter calibration)")
     return ap.parse_args()
@@ -204,6 +206,53 @@ def bench_batched(steps: int, warmup: int = 3):
             "speedup_vs_sequential": round(t_seq / t_batch, 3)}
 
 
+def bench_adjoint(steps: int = 200, every: int = 20):
+    """The differentiable form on the adjoint_16k workload: loss = mean |x_T - target|^2 with
+    target = the rollout at mu = 5 plus N(0, 0.01^2) noise (both from the GPU step), gradients
+    with respect to the initial velocities and mu through the adjoint kernels, checkpoints
+    every `every` steps.  Rate = N * steps / (forward + backward time)."""
+    import numpy as np
+    import torch
+
+    from sph_amd import workloads
+    from sph_amd.differentiable import DifferentiableSPH
+
+    w = workloads.adjoint_16k()
+    n = w.params.particle_count
+    sim = DifferentiableSPH(w.params, w.constants, steps=steps, checkpoint_every=every)
+    x0 = torch.from_numpy(w.state.position.astype(np.float32)).cuda()
+    v0 = torch.from_numpy(w.state.velocity.astype(np.float32)).cuda()
+    x5, _, _ = sim.forward(x0, v0, 5.0)
+    noise = torch.from_numpy(np.random.default_rng(44).normal(0.0, 0.01, size=(n, 3)).astype(np.float32)).cuda()
+    target = x5 + noise
+    mu = float(w.constants.visc)
+
+    def run():
+        xT, _, tape = sim.forward(x0, v0, mu)
+        b.record()
+        gx = 2.0 * (xT - target) / n
+        loss = ((xT.double() - target.double()) ** 2).sum(1).mean()
+        c.record()
+        grads = sim.backward(tape, gx, None)
+        d.record()
+        return loss, grads
+
+    a, b, c, d = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    run()  # warm-up
+    torch.cuda.synchronize()
+    a.record()
+    loss, (gx0, gv0, gmu) = run()
+    torch.cuda.synchronize()
+    t_f, t_b = a.elapsed_time(b) / 1e3, c.elapsed_time(d) / 1e3
+    log(f"adjoint: forward {t_f * 1e3:.1f} ms, backward {t_b * 1e3:.1f} ms for {steps} steps of {n} particles")
+    return {"workload": "adjoint_16k", "particles": n, "steps": steps, "checkpoint_every": every,
+            "value": round(n * steps / (t_f + t_b), 1), "unit": "particle-updates/s",
+            "ms_forward": round(t_f * 1e3, 3), "ms_backward": round(t_b * 1e3, 3),
+            "ms_per_adjoint_step": round(t_b * 1e3 / steps, 4),
+            "loss": float(loss), "grad_mu": float(gmu.item()), "grad_v0_norm": float(gv0.double().norm()),
+            "finite": bool(torch.isfinite(gv0).all() and torch.isfinite(gmu).all())}
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -281,6 +330,7 @@ def main() -> int:
         log("copy calibration: 5 copies of 256 MiB")
 
     batched = bench_batched(args.batch_steps) if args.batched == "on" else None
+    adjoint = bench_adjoint() if args.adjoint == "on" else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -298,6 +348,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "stages_ms": {k: round(v, 4) for k, v in st.items()},
         "batched": batched,
+        "adjoint": adjoint,
         "wall_ms_per_step": round(wall * 1e3 / max(args.steps, 1), 5),
         "final_state": {"non_finite": diag["non_finite"], "max_speed": float(diag["max_speed"]),
                         "mean_density": float(diag["mean_density"])},
