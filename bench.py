@@ -8,11 +8,19 @@ integration, collision.  W warm-up steps, then K timed steps bracketed by device
 synchronisation and HIP events on the stream the kernels run on.
 
 Also measured in the same run:
-  stage timing  events around each stage of a few staged steps (the force kernel is one
-                launch, so its event time is that kernel's duration);
-  roofline      algorithmic bytes of the force kernel per launch / its event time;
-  cpu_baseline  the FP64 CPU oracle (grid search, OpenMP over the host cores) on a bounded
-                sample of the same workload: the first steps until about 10 s have elapsed.
+  tuning        the autotuner's choice of grid cell edge and block sizes (before timing)
+  stage timing  events around each stage of the same K steps run stage by stage (each
+                stage's kernels are one or a few launches, so event time = kernel time);
+  roofline      algorithmic bytes per launch / event time of the dominant kernel (and of
+                the force kernel, and per stage and per step in "kernels"/"step_roofline");
+  repeats       the timed region repeated five times, and the median;
+  pipe          the pipe workload (524,288 particles, 1000 steps);
+  box_4k        the reference's CPU-sized case (4,096 particles, 100 steps) on the GPU;
+  batched       64 simulations per launch (HIP graph) against running them one by one;
+  adjoint       the differentiable form: forward + backward of a 200-step rollout;
+  cpu_baseline  the FP64 CPU oracle (OpenMP over the host cores) on a bounded sample of
+                the same workload (the first steps until about 10 s have elapsed), and on
+                box_4k with the O(N^2) pair loops for all 100 steps.
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -36,8 +44,16 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 #   of the new position): read x, v (24 B), rho and p (8 B), particle id (4 B); write x, v
 #   (24 B), rho (4 B), id (4 B), sort key and slot (8 B) -> 76 B.
 FORCE_BYTES_PER_PARTICLE = 76
-#   density kernel: read x (12 B), write rho, p, 1/rho (12 B) -> 24 B.
-DENSITY_BYTES_PER_PARTICLE = 24
+#   density kernel: read x (12 B), write rho and p (8 B) -> 20 B (1/rho is derived).
+DENSITY_BYTES_PER_PARTICLE = 20
+#   radix sort, per 8-bit pass: histogram read of the key (4 B), key + slot read and written
+#   (16 B) -> 20 B; reorder: slot (4 B), x, v and id gathered (28 B) and written (28 B) -> 60 B;
+#   cell table: sorted key (4 B) plus 4 B per cell of the table.
+SORT_BYTES_PER_PASS = 20
+REORDER_BYTES_PER_PARTICLE = 60
+# FP32 operations per density candidate: 3 sub, 3 for r^2, 2 for (1 - r^2/h^2) clamped,
+# 2 for the cube, 1 add.
+DENSITY_OPS_PER_CANDIDATE = 11
 # FP32 operations per candidate pair test (3 sub, 3 mul/fma, compare) and per interacting
 # pair in the force kernel (rsqrt, 2 pressure / viscosity coefficient chains, 6 fma, 3 sub).
 FLOPS_PER_CANDIDATE = 9
@@ -63,6 +79,10 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--batched", choices=["on", "off"], default="on",
                     help="also measure the batched form (64 box simulations, 300 steps, HIP graph)")
     ap.add_argument("--batch-steps", type=int, default=300)
+    ap.add_argument("--tune", choices=["on", "off"], default="on",
+                    help="autotune grid cell edge and block sizes before timing (skipped with --voxel/--block)")
+    ap.add_argument("--pipe", choices=["on", "off"], default="on",
+                    help="also measure the pipe workload (524,288 particles, 1000 steps)")
     ap.add_argument("--adjoint", choices=["on", "off"], default="on",
                     help="also measure the differentiable form (16,384 particles, 200 steps, forward + backward)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -152,15 +172,87 @@ def cpu_baseline(w, budget_s: float):
         if el >= budget_s or steps >= 200:
             break
     n = len(pos)
-    return {"value": n * steps / el, "unit": "particle-updates/s", "cores": threads, "kind": "port",
-            "sample": f"first {steps} steps of {w.name} ({n} particles), FP64 C restatement of the reference "
-                      f"step (NAIVE formulation through a grid search), OpenMP, {el:.1f} s"}
+    out = {"value": n * steps / el, "unit": "particle-updates/s", "cores": threads, "kind": "port",
+           "sample": f"first {steps} steps of {w.name} ({n} particles), FP64 C restatement of the reference "
+                     f"step (NAIVE formulation through a grid search), OpenMP, {el:.1f} s"}
+    # the reference's own CPU-sized case (box_4k, 100 steps) with the O(N^2) pair loops of
+    # naive_kernels.py, every step
+    from sph_amd import workloads
+
+    s = workloads.box_4k()
+    p, v = s.state.position, s.state.velocity
+    t0 = time.perf_counter()
+    for _ in range(s.steps):
+        p, v, _, _, _ = oracle.step(p, v, consts=s.constants, ext=s.params.external_force, dt=1.0 / s.params.fps,
+                                    space=s.params.space_size, voxel=s.params.voxel_size, use_grid=False)
+    el = time.perf_counter() - t0
+    out["box_4k"] = {"value": round(len(p) * s.steps / el, 1), "unit": "particle-updates/s", "cores": threads,
+                     "kind": "port", "sample": f"all {s.steps} steps of box_4k, O(N^2) pair loops, {el:.2f} s"}
+    log(f"cpu baseline box_4k: {el:.2f} s")
+    return out
 
 
-def bench_batched(steps: int, warmup: int = 3):
+def bench_small(warmup: int = 3):
+    """The reference's CPU-sized case (box_4k: 4,096 particles, 100 steps) on the GPU."""
+    import numpy as np
+    import torch
+
+    from sph_amd import workloads
+    from sph_amd.strategy import NaiveSPHStrategy
+
+    w = workloads.box_4k()
+    n = w.params.particle_count
+    strat = NaiveSPHStrategy(w.params, constants=w.constants)
+    pos = torch.from_numpy(np.ascontiguousarray(w.state.position)).cuda()
+    vel = torch.from_numpy(np.ascontiguousarray(w.state.velocity)).cuda()
+    strat.ctx.upload(pos, vel)
+    strat.ctx.step(warmup)
+    strat.ctx.upload(pos, vel)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    strat.ctx.step(w.steps)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b)
+    return {"workload": "box_4k", "particles": n, "steps": w.steps, "value": round(n * w.steps / (ms / 1e3), 1),
+            "unit": "particle-updates/s", "ms_per_step": round(ms / w.steps, 5)}
+
+
+def bench_pipe(steps: int = 1000, warmup: int = 5):
+    """The pipe workload (pipe_512k: 8 segments, inflow, re-injection through the seeded
+    xoroshiro128+ streams): particle-updates/s over its 1000 steps."""
+    import numpy as np
+    import torch
+
+    from sph_amd import workloads
+    from sph_amd.strategy import NaiveSPHStrategy
+
+    w = workloads.pipe_flow()
+    n = w.params.particle_count
+    strat = NaiveSPHStrategy(w.params, constants=w.constants, mode="PIPE")
+    strat.ctx.upload(torch.from_numpy(np.ascontiguousarray(w.state.position)).cuda(),
+                     torch.from_numpy(np.ascontiguousarray(w.state.velocity)).cuda())
+    strat.ctx.step(warmup)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    strat.ctx.step(steps)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b)
+    d = strat.diagnostics()
+    log(f"pipe: {steps} steps of {n} particles in {ms:.1f} ms")
+    return {"workload": "pipe_512k", "particles": n, "steps": steps, "value": round(n * steps / (ms / 1e3), 1),
+            "unit": "particle-updates/s", "ms_per_step": round(ms / steps, 5), "h": w.constants.h,
+            "non_finite": d["non_finite"]}
+
+
+def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
     """Aggregate particle-updates/s of the batch_64 workload: one launch per stage for all 64
     instances, the timed steps replayed as one HIP graph; against the same 64 simulations
-    advanced one after another with the plain step (same kernels, one context each)."""
+    advanced one after another with the plain step (same kernels, grid and block sizes, one
+    context each).  With tune, the grid cell scale and block sizes come from the autotuner."""
     import numpy as np
     import torch
 
@@ -171,7 +263,19 @@ def bench_batched(steps: int, warmup: int = 3):
     bw = workloads.batch_64(steps=steps)
     total = bw.total_particles
     states = [w.state for w in bw.instances]
-    batch = BatchedSPH([(w.params, w.constants) for w in bw.instances])
+    inst = [(w.params, w.constants) for w in bw.instances]
+    scale, launch, tuning = None, None, None
+    if tune:
+        from sph_amd.autotune import tune_batch
+
+        log("autotuning the batch")
+        res = tune_batch(inst, states, log=log)
+        scale, launch = res.cell_scale, res.launch
+        tuning = {"cell_scale": scale, "density_block": res.density_block, "force_block": res.force_block,
+                  "trials": len(res.trials), "seconds": round(res.seconds, 2),
+                  "candidate_ms_per_step": [min(t[3] for t in res.trials), max(t[3] for t in res.trials)]}
+        log(f"batch tuned: {tuning}")
+    batch = BatchedSPH(inst, cell_scale=scale, launch=launch)
     batch.upload(states)
     graph = batch.capture(steps)  # recorded, not run
     batch.upload(states)
@@ -184,26 +288,34 @@ def bench_batched(steps: int, warmup: int = 3):
     torch.cuda.synchronize()
     t_batch = e0.elapsed_time(e1) / 1e3
     log(f"batched: {steps} steps of {total} particles in {t_batch * 1e3:.1f} ms")
+    # the baseline gets its best case: each instance alone, timed with the plain step's
+    # default grid and blocks and with the batch's tuned ones, the faster counted
+    configs = [(None, None)] + ([(scale, launch)] if tune and (scale, launch) != (None, None) else [])
     t_seq = 0.0
     for w in bw.instances:
-        strat = NaiveSPHStrategy(w.params, constants=w.constants)
-        strat.ctx.upload(torch.from_numpy(np.ascontiguousarray(w.state.position)).cuda(),
-                         torch.from_numpy(np.ascontiguousarray(w.state.velocity)).cuda())
-        strat.ctx.step(warmup)
-        torch.cuda.synchronize()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record()
-        strat.ctx.step(steps)
-        b.record()
-        torch.cuda.synchronize()
-        t_seq += a.elapsed_time(b) / 1e3
-        del strat
+        best = None
+        for sc, la in configs:
+            gv = None if sc is None else workloads.voxel_for(w.constants.h) * sc
+            strat = NaiveSPHStrategy(w.params, constants=w.constants, grid_voxel=gv, launch=la)
+            strat.ctx.upload(torch.from_numpy(np.ascontiguousarray(w.state.position)).cuda(),
+                             torch.from_numpy(np.ascontiguousarray(w.state.velocity)).cuda())
+            strat.ctx.step(warmup)
+            torch.cuda.synchronize()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            strat.ctx.step(steps)
+            b.record()
+            torch.cuda.synchronize()
+            t = a.elapsed_time(b) / 1e3
+            best = t if best is None else min(best, t)
+            del strat
+        t_seq += best
     log(f"sequential plain runs: {t_seq * 1e3:.1f} ms")
     return {"workload": "batch_64", "instances": len(bw.instances), "particles": total, "steps": steps,
             "value": round(total * steps / t_batch, 1), "unit": "particle-updates/s",
             "ms_per_step": round(t_batch * 1e3 / steps, 4), "graph": True,
             "sequential_plain_value": round(total * steps / t_seq, 1),
-            "speedup_vs_sequential": round(t_seq / t_batch, 3)}
+            "speedup_vs_sequential": round(t_seq / t_batch, 3), "tuning": tuning}
 
 
 def bench_adjoint(steps: int = 200, every: int = 20):
@@ -273,11 +385,26 @@ def main() -> int:
         vox = args.voxel * 3 if len(args.voxel) == 1 else args.voxel
         w.params = dataclasses.replace(w.params, voxel_size=np.asarray(vox, dtype=np.float64))
     n = w.params.particle_count
-    strat = NaiveSPHStrategy(w.params, constants=w.constants, mode=w.mode, device=dev)
+    tuning = None
+    launch = tuple(args.block) if args.block else None
+    if args.tune == "on" and not args.voxel and not args.block:
+        import dataclasses
+
+        from sph_amd.autotune import tune_strategy
+
+        log("autotuning grid and block sizes")
+        res = tune_strategy(w.params, w.constants, w.state, mode=w.mode, log=log)
+        gv = res.grid_voxel(w.constants.h)
+        if gv is not None:
+            w.params = dataclasses.replace(w.params, voxel_size=np.asarray([gv] * 3))
+        launch = res.launch
+        tuning = {"cell_scale": res.cell_scale, "grid_voxel": gv, "density_block": res.density_block,
+                  "force_block": res.force_block, "trials": len(res.trials), "seconds": round(res.seconds, 2),
+                  "candidate_ms_per_step": [min(t[3] for t in res.trials), max(t[3] for t in res.trials)]}
+        log(f"tuned: {tuning}")
+    strat = NaiveSPHStrategy(w.params, constants=w.constants, mode=w.mode, device=dev, launch=launch)
     pos = torch.from_numpy(np.ascontiguousarray(w.state.position)).to(dev)
     vel = torch.from_numpy(np.ascontiguousarray(w.state.velocity)).to(dev)
-    if args.block:
-        strat.ctx.set_launch(*args.block)
     strat.ctx.upload(pos, vel)
     log(f"warmup {args.warmup} steps")
     strat.ctx.step(args.warmup)
@@ -305,21 +432,58 @@ def main() -> int:
     strat.ctx.step(args.warmup)
     st = stage_times(strat, args.steps)
     cand, pairs = neighbour_stats(w)
-    force_s = st["force"] / 1e3
-    force_bytes = FORCE_BYTES_PER_PARTICLE * n
-    achieved = force_bytes / force_s / 1e9
-    flops = n * (cand * FLOPS_PER_CANDIDATE + pairs * FLOPS_PER_PAIR)
-    roofline = {"bound": "hbm", "kernel": "naive_force_kernel", "achieved": round(achieved, 2),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": None, "bytes_per_particle": FORCE_BYTES_PER_PARTICLE,
-                "kernel_ms": round(st["force"], 4),
-                "valu": {"achieved_tflops": round(flops / force_s / 1e12, 2), "peak_tflops": FP32_VECTOR_PEAK_TFLOPS,
-                         "candidates_per_particle": round(cand, 1), "pairs_per_particle": round(pairs, 1)}}
+    grid = strat.ctx.grid_info()
+    n_cells = int(np.prod(grid["dims"]))
+    per_particle = {"sort": SORT_BYTES_PER_PASS * grid["passes"], "cell_table": 4 + 4 * n_cells / n,
+                    "reorder": REORDER_BYTES_PER_PARTICLE, "density": DENSITY_BYTES_PER_PARTICLE,
+                    "force": FORCE_BYTES_PER_PARTICLE}
+    kernels = {}
+    for k, ms in st.items():
+        gbs = per_particle[k] * n / (ms / 1e3) / 1e9
+        kernels[k] = {"ms": round(ms, 4), "bytes_per_particle": round(per_particle[k], 2),
+                      "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    step_bytes = sum(per_particle.values())
+    step_gbs = step_bytes * n / (ms_per_step / 1e3) / 1e9
+    step_roofline = {"bytes_per_particle": round(step_bytes, 2), "ms": round(ms_per_step, 5),
+                     "achieved_gbs": round(step_gbs, 1), "frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                     "hbm_floor_ms": round(step_bytes * n / HBM_PEAK_GBS / 1e6, 4)}
+    traffic = {}
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(traffic_file):
-        tr = json.load(open(traffic_file)).get(args.workload, {}).get("naive_force_kernel")
-        if tr:
-            roofline["traffic"] = tr.get("bytes_per_launch")
+        traffic = json.load(open(traffic_file)).get(args.workload, {})
+    valu_ops = {"density": n * cand * DENSITY_OPS_PER_CANDIDATE,
+                "force": n * (cand * FLOPS_PER_CANDIDATE + pairs * FLOPS_PER_PAIR)}
+
+    def kernel_roofline(stage, name):
+        ms = st[stage]
+        achieved = per_particle[stage] * n / (ms / 1e3) / 1e9
+        return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": (traffic.get(name) or {}).get("bytes_per_launch"),
+                "bytes_per_particle": per_particle[stage], "kernel_ms": round(ms, 4),
+                "valu": {"achieved_tflops": round(valu_ops[stage] / (ms / 1e3) / 1e12, 2),
+                         "peak_tflops": FP32_VECTOR_PEAK_TFLOPS, "candidates_per_particle": round(cand, 1),
+                         "pairs_per_particle": round(pairs, 1)}}
+
+    # the roofline object is the dominant (longest) kernel's; the force kernel is reported
+    # beside it as well
+    dominant = max(("density", "force"), key=lambda k: st[k])
+    names = {"density": "naive_density_kernel", "force": "naive_force_kernel"}
+    roofline = kernel_roofline(dominant, names[dominant])
+    roofline_force = kernel_roofline("force", "naive_force_kernel")
+
+    # repeats of the timed region (upload, warm-up, K steps), for the median
+    repeats = []
+    for _ in range(5):
+        strat.ctx.upload(pos, vel)
+        strat.ctx.step(args.warmup)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        strat.ctx.step(args.steps)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        repeats.append(round(e0.elapsed_time(e1) / max(args.steps, 1), 5))
+    log(f"repeats (ms/step): {repeats}")
 
     if args.calibrate_copy:
         src = torch.empty(256 * 2 ** 20, dtype=torch.uint8, device=dev)
@@ -329,7 +493,9 @@ def main() -> int:
         torch.cuda.synchronize()
         log("copy calibration: 5 copies of 256 MiB")
 
-    batched = bench_batched(args.batch_steps) if args.batched == "on" else None
+    pipe = bench_pipe() if args.pipe == "on" else None
+    small = bench_small()
+    batched = bench_batched(args.batch_steps, tune=args.tune == "on") if args.batched == "on" else None
     adjoint = bench_adjoint() if args.adjoint == "on" else None
 
     cpu = None
@@ -345,7 +511,15 @@ def main() -> int:
                    "mode": w.mode, "h": w.constants.h, "voxel": [float(v) for v in np.asarray(w.params.voxel_size)],
                    "parallelism": "single GPU"},
         "roofline": roofline,
+        "roofline_force": roofline_force,
+        "kernels": kernels,
+        "step_roofline": step_roofline,
+        "repeat_ms_per_step": repeats,
+        "median_ms_per_step": float(np.median(repeats)),
         "cpu_baseline": cpu,
+        "tuning": tuning,
+        "pipe": pipe,
+        "box_4k": small,
         "stages_ms": {k: round(v, 4) for k, v in st.items()},
         "batched": batched,
         "adjoint": adjoint,

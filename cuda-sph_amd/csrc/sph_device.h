@@ -191,11 +191,15 @@ __device__ void pipe_solve_collision(const PipeTable& T, Vec3& p, Vec3& v, int s
     first.z += T.row[seg][2];
     // calc_dt (util_kernels.py:113-121) with calc_distance_to_pipe (:124-135)
     const float sl = len3(v), el = len3(edge);
-    const float cos_a = dot3(v, edge) / (sl * el);
     const Vec3 pv{first.x - p.x, first.y - p.y, first.z - p.z};
     const Vec3 cr{edge.y * pv.z - edge.z * pv.y, edge.z * pv.x - edge.x * pv.z, edge.x * pv.y - edge.y * pv.x};
     const float dist = len3(cr) / el;
-    const float sin_a = sqrtf(1.0f - cos_a * cos_a);
+    // sin of the angle between v and the wall line.  The reference writes sqrt(1 - cos^2)
+    // (util_kernels.py:118); for a particle sliding along a conical wall cos is within 1e-8
+    // of 1, which FP32 rounds above 1 (sqrt of a negative: NaN).  |v x edge| / (|v| |edge|)
+    // is the same quantity and stays accurate at small angles.
+    const Vec3 ve{v.y * edge.z - v.z * edge.y, v.z * edge.x - v.x * edge.z, v.x * edge.y - v.y * edge.x};
+    const float sin_a = len3(ve) / (sl * el);
     const float dt = -(dist / (sin_a + 0.001f)) / sl;
     const Vec3 cp{p.x + v.x * dt, p.y + v.y * dt, p.z + v.z * dt};
     // reflection (util_kernels.py:191-199)

@@ -17,14 +17,23 @@ import torch
 from . import _native as N
 from .constants import SPHConstants
 from .data_classes import SimulationParameters, SimulationState
+from .workloads import voxel_for
 
 
 class BatchedSPH:
     def __init__(self, instances: Sequence[Tuple[SimulationParameters, SPHConstants]], device="cuda:0",
-                 launch: Optional[Tuple[int, int]] = None):
+                 launch: Optional[Tuple[int, int]] = None, cell_scale: Optional[float] = None):
+        """launch: (density, force) threads per workgroup.  cell_scale: when given, instance
+        b searches a grid of edge voxel_for(h_b) * cell_scale instead of its voxel_size
+        (the NAIVE sums do not depend on the grid; see autotune.py)."""
         self.instances = list(instances)
-        params = [N.make_params(p.particle_count, p.external_force, 1.0 / p.fps, p.space_size, p.voxel_size,
-                                N.SPH_MODE_BOX, N.SPH_FORM_NAIVE) for p, _ in self.instances]
+        self.cell_scale = cell_scale
+
+        def grid(p, c):
+            return p.voxel_size if cell_scale is None else [voxel_for(c.h) * cell_scale] * 3
+
+        params = [N.make_params(p.particle_count, p.external_force, 1.0 / p.fps, p.space_size, grid(p, c),
+                                N.SPH_MODE_BOX, N.SPH_FORM_NAIVE) for p, c in self.instances]
         consts = [N.make_consts(c) for _, c in self.instances]
         self.ctx = N.NativeContext(params, consts, device)
         self.device = self.ctx.device
@@ -42,7 +51,13 @@ class BatchedSPH:
         vel = np.concatenate([np.asarray(s.velocity, dtype=np.float64).reshape(-1, 3) for s in states])
         if pos.shape[0] != self.n:
             raise ValueError(f"states hold {pos.shape[0]} particles, the batch has {self.n}")
-        self.ctx.upload(torch.from_numpy(pos).to(self.device), torch.from_numpy(vel).to(self.device))
+        self._pos = torch.from_numpy(pos).to(self.device)
+        self._vel = torch.from_numpy(vel).to(self.device)
+        self.ctx.upload(self._pos, self._vel)
+
+    def ctx_state_tensors(self) -> Tuple[torch.Tensor, torch.Tensor]:
+        """The last uploaded state as float64 (N, 3) device tensors."""
+        return self._pos, self._vel
 
     def download(self) -> List[SimulationState]:
         pos, vel, rho = [t.cpu().numpy() for t in self.ctx.download()]

@@ -16,7 +16,7 @@ Differences in behaviour at the boundary:
 from __future__ import annotations
 
 import logging
-from typing import Optional
+from typing import Optional, Tuple
 
 import numpy as np
 import torch
@@ -40,9 +40,15 @@ class SPHStrategy:
               "_integrate", "_collide", "_finalize_computation")
 
     def __init__(self, params: SimulationParameters, constants: Optional[SPHConstants] = None, mode: str = "BOX",
-                 device="cuda:0", keep_terms: bool = False, seed: int = RNG_SEED):
+                 device="cuda:0", keep_terms: bool = False, seed: int = RNG_SEED, grid_voxel: Optional[float] = None,
+                 launch: Optional[Tuple[int, int]] = None):
+        """grid_voxel: NAIVE formulation only, the edge of the search grid's cells in place
+        of params.voxel_size (the NAIVE sums do not depend on it; see autotune.py).
+        launch: (density, force) threads per workgroup."""
         if mode not in _MODES:
             raise Exception(f"Wrong simulation mode! ({mode})")
+        if grid_voxel is not None and self.formulation != N.SPH_FORM_NAIVE:
+            raise ValueError("grid_voxel applies to the NAIVE formulation; VOXEL uses params.voxel_size")
         self.params = params
         self.constants = constants or SPHConstants()
         self.mode = mode
@@ -52,12 +58,15 @@ class SPHStrategy:
         n = int(params.particle_count)
         self.result_force = np.zeros((n, 3), dtype=np.float64)
         table = params.pipe.to_numpy() if mode == "PIPE" else None
-        self._params_c = N.make_params(n, params.external_force, self.dt, params.space_size, params.voxel_size,
+        voxel = params.voxel_size if grid_voxel is None else [float(grid_voxel)] * 3
+        self._params_c = N.make_params(n, params.external_force, self.dt, params.space_size, voxel,
                                        _MODES[mode], self.formulation, table,
                                        N.SPH_FLAG_KEEP_TERMS if keep_terms else 0, seed)
         self.keep_terms = keep_terms
         self.ctx = N.NativeContext(self._params_c, N.make_consts(self.constants), device)
         self.device = self.ctx.device
+        if launch:
+            self.ctx.set_launch(*launch)
         logger.info("SPH strategy on %s: %s", self.device, self.ctx.grid_info())
 
     # -- reference entry point -----------------------------------------------------------

@@ -1,0 +1,56 @@
+"""Autotuner host logic (no GPU): cache round trip, signatures, the search's choice."""
+import numpy as np
+
+from sph_amd import autotune as T
+from sph_amd import workloads as W
+
+
+def test_cache_round_trip(tmp_path):
+    cache = T.TuneCache(str(tmp_path / "tune.json"))
+    res = T.TuneResult(0.5, 128, 256, 0.123, ((1.0, 64, 64, 0.2), (0.5, 128, 256, 0.123)), 1.5, False)
+    cache.put("k", res)
+    got = cache.get("k")
+    assert got.cached and got.seconds == 0.0
+    assert (got.cell_scale, got.density_block, got.force_block, got.ms_per_step) == (0.5, 128, 256, 0.123)
+    assert got.trials == res.trials
+    assert cache.get("other") is None
+
+
+def test_signature_tracks_workload_and_order():
+    a = W.box_uniform(512, seed=1)
+    b = W.box_uniform(512, seed=2, h=0.05)
+    ka = T.signature("plain", [(a.params, a.constants, a.state)], "BOX", "NAIVE", 10, "cpu")
+    kb = T.signature("plain", [(b.params, b.constants, b.state)], "BOX", "NAIVE", 10, "cpu")
+    assert ka != kb
+    assert ka == T.signature("plain", [(a.params, a.constants, a.state)], "BOX", "NAIVE", 10, "cpu")
+    kab = T.signature("batch", [(a.params, a.constants, a.state), (b.params, b.constants, b.state)], "BOX", "NAIVE",
+                      10, "cpu")
+    kba = T.signature("batch", [(b.params, b.constants, b.state), (a.params, a.constants, a.state)], "BOX", "NAIVE",
+                      10, "cpu")
+    assert kab != kba
+
+
+def test_search_picks_fastest_and_skips_rejected_grids(monkeypatch):
+    cost = {(1.0, 64, 64): 3.0, (1.0, 64, 128): 2.0, (1.0, 128, 64): 4.0, (1.0, 128, 128): 5.0}
+    state = {}
+    closed = []
+
+    def make(scale):
+        if scale == 0.25:
+            raise RuntimeError("grid too fine")
+
+        def set_launch(db, fb):
+            state["cfg"] = (scale, db, fb)
+        return set_launch, lambda k: None, lambda: None, lambda: closed.append(scale)
+
+    monkeypatch.setattr(T, "_time_steps", lambda step, reset, steps, warmup: cost[state["cfg"]])
+    res = T._search(make, [1.0, 0.25], (64, 128), 5, 1)
+    assert (res.cell_scale, res.density_block, res.force_block, res.ms_per_step) == (1.0, 64, 128, 2.0)
+    assert len(res.trials) == 4 and closed == [1.0]
+
+
+def test_grid_voxel_is_power_of_two():
+    res = T.TuneResult(0.5, 64, 64, 1.0)
+    g = res.grid_voxel(0.0457)
+    assert g == 2.0 ** -6 and np.log2(g) == int(np.log2(g))
+    assert T.TuneResult(None, 64, 64, 1.0).grid_voxel(0.0457) is None
