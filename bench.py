@@ -16,6 +16,8 @@ Also measured in the same run:
   repeats       the timed region repeated five times, and the median;
   pipe          the pipe workload (524,288 particles, 1000 steps);
   box_4k        the reference's CPU-sized case (4,096 particles, 100 steps) on the GPU;
+  frames        frame output to disk in the reference's Saver format, synchronous
+                (the reference's loop) against the asynchronous writer;
   batched       64 simulations per launch (HIP graph) against running them one by one;
   adjoint       the differentiable form: forward + backward of a 200-step rollout;
   cpu_baseline  the FP64 CPU oracle (OpenMP over the host cores) on a bounded sample of
@@ -81,6 +83,8 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--batch-steps", type=int, default=300)
     ap.add_argument("--tune", choices=["on", "off"], default="on",
                     help="autotune grid cell edge and block sizes before timing (skipped with --voxel/--block)")
+    ap.add_argument("--frames", choices=["on", "off"], default="on",
+                    help="also measure frame output to disk in the reference's format")
     ap.add_argument("--pipe", choices=["on", "off"], default="on",
                     help="also measure the pipe workload (524,288 particles, 1000 steps)")
     ap.add_argument("--adjoint", choices=["on", "off"], default="on",
@@ -190,6 +194,43 @@ def cpu_baseline(w, budget_s: float):
                      "kind": "port", "sample": f"all {s.steps} steps of box_4k, O(N^2) pair loops, {el:.2f} s"}
     log(f"cpu baseline box_4k: {el:.2f} s")
     return out
+
+
+def bench_frames(n: int = 65536, frames: int = 50):
+    """Frame output in the reference's file format (a Saver folder of .npy frames), every
+    step saved: the reference's loop (StateGenerator + Saver: one synchronous step, copy
+    and write per frame) against run_to_disk (frames gathered on the GPU, copied on a
+    second stream and written by a thread while the GPU steps on)."""
+    import itertools
+    import shutil
+    import tempfile
+
+    from sph_amd import workloads
+    from sph_amd.serializer import Saver, run_to_disk
+    from sph_amd.state_generator import StateGenerator
+    from sph_amd.strategy import NaiveSPHStrategy
+
+    w = workloads.box_uniform(n, seed=5)
+    params = w.params  # dt = 1 / fps = 1e-3; the first `frames` frames are written
+    root = tempfile.mkdtemp(prefix="sph_frames_")
+    try:
+        gen = StateGenerator(w.state, params, constants=w.constants)
+        saver = Saver("sync", params, root=root)
+        t0 = time.perf_counter()
+        for state in itertools.islice(gen, frames):
+            saver.save_next_state(state)
+        t_sync = time.perf_counter() - t0
+        strat = NaiveSPHStrategy(params, constants=w.constants)
+        t0 = time.perf_counter()
+        run_to_disk(strat, w.state, params, "async", root=root, frames=frames)
+        t_async = time.perf_counter() - t0
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+    mb = n * 56 / 1e6
+    log(f"frames: sync {t_sync:.2f} s, async {t_async:.2f} s for {frames} frames of {mb:.1f} MB")
+    return {"workload": f"box_uniform_{n}", "particles": n, "frames": frames, "mb_per_frame": round(mb, 2),
+            "sync_frames_per_s": round(frames / t_sync, 1), "async_frames_per_s": round(frames / t_async, 1),
+            "speedup": round(t_sync / t_async, 2)}
 
 
 def bench_small(warmup: int = 3):
@@ -494,6 +535,7 @@ def main() -> int:
         log("copy calibration: 5 copies of 256 MiB")
 
     pipe = bench_pipe() if args.pipe == "on" else None
+    frames = bench_frames() if args.frames == "on" else None
     small = bench_small()
     batched = bench_batched(args.batch_steps, tune=args.tune == "on") if args.batched == "on" else None
     adjoint = bench_adjoint() if args.adjoint == "on" else None
@@ -520,6 +562,7 @@ def main() -> int:
         "tuning": tuning,
         "pipe": pipe,
         "box_4k": small,
+        "frames": frames,
         "stages_ms": {k: round(v, 4) for k, v in st.items()},
         "batched": batched,
         "adjoint": adjoint,

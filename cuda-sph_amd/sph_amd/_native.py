@@ -192,12 +192,19 @@ class NativeContext:
     def run_stage(self, stage: int):
         check(self.lib.sph_run_stage(self.handle, int(stage), _stream(self.device)), "sph_run_stage")
 
-    def download(self, force: bool = False):
-        """Returns float64 device tensors pos (N,3), vel (N,3), rho (N,) [, force (N,3)]."""
+    def download(self, force: bool = False, out=None):
+        """Returns float64 device tensors pos (N,3), vel (N,3), rho (N,) [, force (N,3)];
+        `out` may give (pos, vel, rho) destinations (contiguous float64 on the device)."""
         kw = dict(dtype=torch.float64, device=self.device)
-        pos = torch.empty((self.n, 3), **kw)
-        vel = torch.empty((self.n, 3), **kw)
-        rho = torch.empty((self.n,), **kw)
+        if out is not None:
+            pos, vel, rho = out
+            for t, shape in ((pos, (self.n, 3)), (vel, (self.n, 3)), (rho, (self.n,))):
+                assert t.dtype == torch.float64 and tuple(t.shape) == shape and t.is_contiguous()
+                assert t.device == self.device
+        else:
+            pos = torch.empty((self.n, 3), **kw)
+            vel = torch.empty((self.n, 3), **kw)
+            rho = torch.empty((self.n,), **kw)
         f = torch.empty((self.n, 3), **kw) if force else None
         check(self.lib.sph_download_state(self.handle, _ptr(pos), _ptr(vel), _ptr(rho), _ptr(f),
                                           _stream(self.device)), "sph_download_state")
