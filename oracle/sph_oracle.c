@@ -28,6 +28,8 @@
  *   naive rho / P / V       independent NumPy brute force, pair identities, invariances
  *   voxel keys / sort       closed form (trunc, NumPy stable argsort)
  *   voxel formulation       brute-force cap-32 enumeration in NumPy on small inputs
+ *   rollout adjoint         directional central differences of this file's own forward map
+ *                           (tests/test_oracle_adjoint.py), wall bounces replayed
  */
 #include <math.h>
 #include <stdint.h>
