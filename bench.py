@@ -93,6 +93,28 @@ def parse() -> argparse.Namespace:
     return ap.parse_args()
 
 
+def tune_workload(w, mode="BOX", scales=(1.0, 0.5), enabled=True):
+    """Autotuned (params with the chosen grid, (density, force) block sizes, lanes per
+    particle, summary) for a workload; the untuned defaults when disabled."""
+    import dataclasses
+
+    import numpy as np
+
+    if not enabled:
+        return w.params, None, 1, None
+    from sph_amd.autotune import tune_strategy
+
+    res = tune_strategy(w.params, w.constants, w.state, mode=mode, scales=scales, log=log)
+    gv = res.grid_voxel(w.constants.h)
+    params = w.params if gv is None else dataclasses.replace(w.params, voxel_size=np.asarray([gv] * 3))
+    info = {"cell_scale": res.cell_scale, "grid_voxel": gv, "density_block": res.density_block,
+            "force_block": res.force_block, "lanes": res.lanes, "trials": len(res.trials),
+            "seconds": round(res.seconds, 2),
+            "candidate_ms_per_step": [min(t[4] for t in res.trials), max(t[4] for t in res.trials)]}
+    log(f"tuned {w.name}: {info}")
+    return params, res.launch, res.lanes, info
+
+
 def make_workload(name):
     from sph_amd import workloads
 
@@ -252,7 +274,7 @@ def bench_frames(n: int = 65536, frames: int = 50):
             "speedup": round(t_sync / t_async, 2)}
 
 
-def bench_small(warmup: int = 3):
+def bench_small(warmup: int = 3, tune: bool = True):
     """The reference's CPU-sized case (box_4k: 4,096 particles, 100 steps) on the GPU."""
     import numpy as np
     import torch
@@ -262,24 +284,26 @@ def bench_small(warmup: int = 3):
 
     w = workloads.box_4k()
     n = w.params.particle_count
-    strat = NaiveSPHStrategy(w.params, constants=w.constants)
+    params, launch, lanes, tuning = tune_workload(w, enabled=tune)
+    strat = NaiveSPHStrategy(params, constants=w.constants, launch=launch, lanes=lanes)
     pos = torch.from_numpy(np.ascontiguousarray(w.state.position)).cuda()
     vel = torch.from_numpy(np.ascontiguousarray(w.state.velocity)).cuda()
     strat.ctx.upload(pos, vel)
     strat.ctx.step(warmup)
+    graph = strat.capture(w.steps)  # the 100 steps as one HIP graph (recorded, not run)
     strat.ctx.upload(pos, vel)
     torch.cuda.synchronize()
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     a.record()
-    strat.ctx.step(w.steps)
+    graph.replay()
     b.record()
     torch.cuda.synchronize()
     ms = a.elapsed_time(b)
     return {"workload": "box_4k", "particles": n, "steps": w.steps, "value": round(n * w.steps / (ms / 1e3), 1),
-            "unit": "particle-updates/s", "ms_per_step": round(ms / w.steps, 5)}
+            "unit": "particle-updates/s", "ms_per_step": round(ms / w.steps, 5), "graph": True, "tuning": tuning}
 
 
-def bench_pipe(steps: int = 1000, warmup: int = 5):
+def bench_pipe(steps: int = 1000, warmup: int = 5, tune: bool = True):
     """The pipe workload (pipe_512k: 8 segments, inflow, re-injection through the seeded
     xoroshiro128+ streams): particle-updates/s over its 1000 steps."""
     import numpy as np
@@ -290,7 +314,8 @@ def bench_pipe(steps: int = 1000, warmup: int = 5):
 
     w = workloads.pipe_flow()
     n = w.params.particle_count
-    strat = NaiveSPHStrategy(w.params, constants=w.constants, mode="PIPE")
+    params, launch, lanes, tuning = tune_workload(w, "PIPE", enabled=tune)
+    strat = NaiveSPHStrategy(params, constants=w.constants, mode="PIPE", launch=launch, lanes=lanes)
     strat.ctx.upload(torch.from_numpy(np.ascontiguousarray(w.state.position)).cuda(),
                      torch.from_numpy(np.ascontiguousarray(w.state.velocity)).cuda())
     strat.ctx.step(warmup)
@@ -305,7 +330,7 @@ def bench_pipe(steps: int = 1000, warmup: int = 5):
     log(f"pipe: {steps} steps of {n} particles in {ms:.1f} ms")
     return {"workload": "pipe_512k", "particles": n, "steps": steps, "value": round(n * steps / (ms / 1e3), 1),
             "unit": "particle-updates/s", "ms_per_step": round(ms / steps, 5), "h": w.constants.h,
-            "non_finite": d["non_finite"]}
+            "non_finite": d["non_finite"], "tuning": tuning}
 
 
 def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
@@ -324,18 +349,18 @@ def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
     total = bw.total_particles
     states = [w.state for w in bw.instances]
     inst = [(w.params, w.constants) for w in bw.instances]
-    scale, launch, tuning = None, None, None
+    scale, launch, lanes, tuning = None, None, 1, None
     if tune:
         from sph_amd.autotune import tune_batch
 
         log("autotuning the batch")
         res = tune_batch(inst, states, log=log)
-        scale, launch = res.cell_scale, res.launch
+        scale, launch, lanes = res.cell_scale, res.launch, res.lanes
         tuning = {"cell_scale": scale, "density_block": res.density_block, "force_block": res.force_block,
-                  "trials": len(res.trials), "seconds": round(res.seconds, 2),
-                  "candidate_ms_per_step": [min(t[3] for t in res.trials), max(t[3] for t in res.trials)]}
+                  "lanes": lanes, "trials": len(res.trials), "seconds": round(res.seconds, 2),
+                  "candidate_ms_per_step": [min(t[4] for t in res.trials), max(t[4] for t in res.trials)]}
         log(f"batch tuned: {tuning}")
-    batch = BatchedSPH(inst, cell_scale=scale, launch=launch)
+    batch = BatchedSPH(inst, cell_scale=scale, launch=launch, lanes=lanes)
     batch.upload(states)
     graph = batch.capture(steps)  # recorded, not run
     batch.upload(states)
@@ -350,13 +375,13 @@ def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
     log(f"batched: {steps} steps of {total} particles in {t_batch * 1e3:.1f} ms")
     # the baseline gets its best case: each instance alone, timed with the plain step's
     # default grid and blocks and with the batch's tuned ones, the faster counted
-    configs = [(None, None)] + ([(scale, launch)] if tune and (scale, launch) != (None, None) else [])
+    configs = [(None, None, 1)] + ([(scale, launch, lanes)] if tune else [])
     t_seq = 0.0
     for w in bw.instances:
         best = None
-        for sc, la in configs:
+        for sc, la, ln in configs:
             gv = None if sc is None else workloads.voxel_for(w.constants.h) * sc
-            strat = NaiveSPHStrategy(w.params, constants=w.constants, grid_voxel=gv, launch=la)
+            strat = NaiveSPHStrategy(w.params, constants=w.constants, grid_voxel=gv, launch=la, lanes=ln)
             strat.ctx.upload(torch.from_numpy(np.ascontiguousarray(w.state.position)).cuda(),
                              torch.from_numpy(np.ascontiguousarray(w.state.velocity)).cuda())
             strat.ctx.step(warmup)
@@ -378,7 +403,7 @@ def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
             "speedup_vs_sequential": round(t_seq / t_batch, 3), "tuning": tuning}
 
 
-def bench_adjoint(steps: int = 200, every: int = 20):
+def bench_adjoint(steps: int = 200, every: int = 20, tune: bool = True):
     """The differentiable form on the adjoint_16k workload: loss = mean |x_T - target|^2 with
     target = the rollout at mu = 5 plus N(0, 0.01^2) noise (both from the GPU step), gradients
     with respect to the initial velocities and mu through the adjoint kernels, checkpoints
@@ -391,7 +416,8 @@ def bench_adjoint(steps: int = 200, every: int = 20):
 
     w = workloads.adjoint_16k()
     n = w.params.particle_count
-    sim = DifferentiableSPH(w.params, w.constants, steps=steps, checkpoint_every=every)
+    _, launch, lanes, tuning = tune_workload(w, scales=(1.0,), enabled=tune)  # the grid stays the workload's
+    sim = DifferentiableSPH(w.params, w.constants, steps=steps, checkpoint_every=every, launch=launch, lanes=lanes)
     x0 = torch.from_numpy(w.state.position.astype(np.float32)).cuda()
     v0 = torch.from_numpy(w.state.velocity.astype(np.float32)).cuda()
     x5, _, _ = sim.forward(x0, v0, 5.0)
@@ -422,7 +448,7 @@ def bench_adjoint(steps: int = 200, every: int = 20):
             "ms_forward": round(t_f * 1e3, 3), "ms_backward": round(t_b * 1e3, 3),
             "ms_per_adjoint_step": round(t_b * 1e3 / steps, 4),
             "loss": float(loss), "grad_mu": float(gmu.item()), "grad_v0_norm": float(gv0.double().norm()),
-            "finite": bool(torch.isfinite(gv0).all() and torch.isfinite(gmu).all())}
+            "finite": bool(torch.isfinite(gv0).all() and torch.isfinite(gmu).all()), "tuning": tuning}
 
 
 def main() -> int:
@@ -445,24 +471,11 @@ def main() -> int:
         vox = args.voxel * 3 if len(args.voxel) == 1 else args.voxel
         w.params = dataclasses.replace(w.params, voxel_size=np.asarray(vox, dtype=np.float64))
     n = w.params.particle_count
-    tuning = None
-    launch = tuple(args.block) if args.block else None
-    if args.tune == "on" and not args.voxel and not args.block:
-        import dataclasses
-
-        from sph_amd.autotune import tune_strategy
-
-        log("autotuning grid and block sizes")
-        res = tune_strategy(w.params, w.constants, w.state, mode=w.mode, log=log)
-        gv = res.grid_voxel(w.constants.h)
-        if gv is not None:
-            w.params = dataclasses.replace(w.params, voxel_size=np.asarray([gv] * 3))
-        launch = res.launch
-        tuning = {"cell_scale": res.cell_scale, "grid_voxel": gv, "density_block": res.density_block,
-                  "force_block": res.force_block, "trials": len(res.trials), "seconds": round(res.seconds, 2),
-                  "candidate_ms_per_step": [min(t[3] for t in res.trials), max(t[3] for t in res.trials)]}
-        log(f"tuned: {tuning}")
-    strat = NaiveSPHStrategy(w.params, constants=w.constants, mode=w.mode, device=dev, launch=launch)
+    tune = args.tune == "on" and not args.voxel and not args.block
+    w.params, launch, lanes, tuning = tune_workload(w, w.mode, enabled=tune)
+    if args.block:
+        launch = tuple(args.block)
+    strat = NaiveSPHStrategy(w.params, constants=w.constants, mode=w.mode, device=dev, launch=launch, lanes=lanes)
     pos = torch.from_numpy(np.ascontiguousarray(w.state.position)).to(dev)
     vel = torch.from_numpy(np.ascontiguousarray(w.state.velocity)).to(dev)
     strat.ctx.upload(pos, vel)
@@ -552,11 +565,11 @@ def main() -> int:
         torch.cuda.synchronize()
         log("copy calibration: 5 copies of 256 MiB")
 
-    pipe = bench_pipe() if args.pipe == "on" else None
+    pipe = bench_pipe(tune=args.tune == "on") if args.pipe == "on" else None
     frames = bench_frames() if args.frames == "on" else None
-    small = bench_small()
+    small = bench_small(tune=args.tune == "on")
     batched = bench_batched(args.batch_steps, tune=args.tune == "on") if args.batched == "on" else None
-    adjoint = bench_adjoint() if args.adjoint == "on" else None
+    adjoint = bench_adjoint(tune=args.tune == "on") if args.adjoint == "on" else None
 
     cpu = None
     if not args.no_cpu_baseline:

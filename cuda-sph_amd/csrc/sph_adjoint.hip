@@ -60,6 +60,7 @@ __device__ __forceinline__ void walk_neighbours(const StepConsts& C, int i, floa
 __device__ __forceinline__ float comp(const Vec3& a, int d) { return d == 0 ? a.x : (d == 1 ? a.y : a.z); }
 __device__ __forceinline__ float comp(const float4& a, int d) { return d == 0 ? a.x : (d == 1 ? a.y : a.z); }
 
+template <int L>
 __global__ __launch_bounds__(kAdjThreads) void adjoint_seed_kernel(
     StepConsts C, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
     const float* __restrict__ rhoA, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ nl,
@@ -67,10 +68,13 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_seed_kernel(
     uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
     uint32_t* __restrict__ flags, const float4* __restrict__ cotx, const float4* __restrict__ cotv,
     float4* __restrict__ fb, float4* __restrict__ gxs, float4* __restrict__ gvs) {
-    const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    // L lanes per particle, as the step's force pass uses, so the recomputed sums are its own
+    const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int i = (int)(gt / L);
     if (i >= C.n) return;
     const float4 p = pos[i], v = vel[i];
-    const PairSums S = naive_pair_sums<false>(C, i, C.n, p, v, pos, vel, cs, nl, nn);
+    const PairSums S = naive_pair_sums_lanes<false, L>(C, i, gt & (L - 1), C.n, p, v, pos, vel, cs, nl, nn);
+    if (L > 1 && (gt & (L - 1)) != 0) return;
     const float ps = 0.5f * C.mass * C.grad_c;
     const float vs = C.mass * C.visc * C.lap_c;
     const Vec3 P{ps * S.ax, ps * S.ay, ps * S.az}, V{vs * S.bx, vs * S.by, vs * S.bz};
@@ -239,9 +243,17 @@ hipError_t launch_adjoint_pullback(sph_ctx* c, hipStream_t s) {
     if (n == 0) return hipGetLastError();
     const Buffers& b = c->b;
     const int nb = adj_blocks(n);
-    adjoint_seed_kernel<<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.velA, b.pidA, b.rhoA, b.cs, b.nl, b.nn, b.posB,
-                                                    b.velB, b.pidB, b.keyI, b.valI, b.flags, b.cotx, b.cotv, b.adj_fb,
-                                                    b.adj_gx, b.adj_gv);
+    auto seed = adjoint_seed_kernel<1>;
+    switch (c->lanes) {
+        case 2: seed = adjoint_seed_kernel<2>; break;
+        case 4: seed = adjoint_seed_kernel<4>; break;
+        case 8: seed = adjoint_seed_kernel<8>; break;
+        case 16: seed = adjoint_seed_kernel<16>; break;
+        default: break;
+    }
+    seed<<<adj_blocks(n * c->lanes), kAdjThreads, 0, s>>>(c->sc, b.posA, b.velA, b.pidA, b.rhoA, b.cs, b.nl, b.nn,
+                                                            b.posB, b.velB, b.pidB, b.keyI, b.valI, b.flags, b.cotx,
+                                                            b.cotv, b.adj_fb, b.adj_gx, b.adj_gv);
     adjoint_pair_kernel<<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.velA, b.pidA, b.cs, b.nl, b.nn, b.adj_fb, b.adj_gx,
                                                     b.adj_gv, b.adj_rt, b.cotv);
     adjoint_density_kernel<<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.pidA, b.cs, b.nl, b.nn, b.adj_rt, b.adj_gx,

@@ -210,6 +210,7 @@ void plan_sort_and_layout(sph_ctx* c) {
     c->num_tiles = sph::radix_num_tiles(c->n);
     if (c->density_block <= 0) c->density_block = 256;
     if (c->force_block <= 0) c->force_block = 256;
+    if (c->lanes <= 0) c->lanes = 1;
     c->ws_bytes = plan_layout(c).total;
 }
 
@@ -590,6 +591,18 @@ int sph_set_launch(sph_ctx* c, int32_t density_block, int32_t force_block) {
         if (b != 64 && b != 128 && b != 256) return fail(SPH_E_INVALID, "block size must be 64, 128 or 256");
     c->density_block = density_block;
     c->force_block = force_block;
+    return SPH_OK;
+}
+
+int sph_set_lanes(sph_ctx* c, int32_t lanes) {
+    if (!c) return fail(SPH_E_INVALID, "null context");
+    if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16)
+        return fail(SPH_E_INVALID, "lanes per particle must be 1, 2, 4, 8 or 16");
+    if (lanes > 1 && c->prm.formulation != SPH_FORM_NAIVE)
+        return fail(SPH_E_INVALID, "the VOXEL formulation walks its capped list with one lane per particle");
+    if (c->n * (int64_t)lanes >= (int64_t(1) << 31))
+        return fail(SPH_E_INVALID, "%lld particles x %d lanes exceed 2^31 threads", (long long)c->n, lanes);
+    c->lanes = lanes;
     return SPH_OK;
 }
 

@@ -389,5 +389,64 @@ __device__ __forceinline__ PairSums naive_pair_sums(const StepConsts& C, int i, 
     return S;
 }
 
+// The same sums with L lanes per particle (L = 1 is naive_pair_sums): lane l of the group
+// walks list slots l, l + L, ... (or, past the list, every L-th candidate of each row), and
+// the partial sums are added across the group; every lane of the group returns the total.
+template <bool BATCH, int L>
+__device__ __forceinline__ PairSums naive_pair_sums_lanes(const StepConsts& C, int i, uint32_t l, int n, float4 p,
+                                                          float4 v, const float4* __restrict__ pos,
+                                                          const float4* __restrict__ vel,
+                                                          const uint32_t* __restrict__ cs,
+                                                          const uint32_t* __restrict__ nl,
+                                                          const uint32_t* __restrict__ nn) {
+    if constexpr (L == 1) {
+        return naive_pair_sums<BATCH>(C, i, n, p, v, pos, vel, cs, nl, nn);
+    } else {
+        const __amdgpu_buffer_rsrc_t rpos = make_rsrc(pos, (uint32_t)n * 16u);
+        const __amdgpu_buffer_rsrc_t rvel = make_rsrc(vel, (uint32_t)n * 16u);
+        PairSums S;
+        const uint32_t cnt = nn[i];
+        if (cnt <= (uint32_t)kListCap) {
+            const size_t stride = (size_t)n;
+            for (uint32_t t = l; t < cnt; t += L) {
+                const uint32_t k = nl[t * stride + i];
+                S.add(C, p, v, ld_f4(rpos, k * 16u), ld_f4(rvel, k * 16u), k == (uint32_t)i);
+            }
+        } else {
+            const float hs = C.h_search;
+            const float hs2 = hs * hs;
+            const int ylo = grid_coord(p.y - hs, 1, C), yhi = grid_coord(p.y + hs, 1, C);
+            const int zlo = grid_coord(p.z - hs, 2, C), zhi = grid_coord(p.z + hs, 2, C);
+            for (int cz = zlo; cz <= zhi; ++cz) {
+                const float dz0 = slab_dist(p.z, cz, 2, C);
+                for (int cy = ylo; cy <= yhi; ++cy) {
+                    const float dy0 = slab_dist(p.y, cy, 1, C);
+                    const float rem = hs2 - dz0 * dz0 - dy0 * dy0;
+                    if (rem < 0.0f) continue;
+                    const float e = sqrtf(rem);
+                    const int xlo = grid_coord(p.x - e, 0, C), xhi = grid_coord(p.x + e, 0, C);
+                    const int row = C.cell_off + C.dims[0] * (cy + C.dims[1] * cz);
+                    const uint32_t k1 = cs[row + xhi + 1];
+                    for (uint32_t k = cs[row + xlo] + l; k < k1; k += L) {
+                        const float4 q = pos[k];
+                        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+                        if (dx * dx + dy * dy + dz * dz <= C.h2 && k != (uint32_t)i) S.add(C, p, v, q, vel[k]);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int o = 1; o < L; o <<= 1) {
+            S.ax += __shfl_xor(S.ax, o);
+            S.ay += __shfl_xor(S.ay, o);
+            S.az += __shfl_xor(S.az, o);
+            S.bx += __shfl_xor(S.bx, o);
+            S.by += __shfl_xor(S.by, o);
+            S.bz += __shfl_xor(S.bz, o);
+        }
+        return S;
+    }
+}
+
 }  // namespace
 }  // namespace sph

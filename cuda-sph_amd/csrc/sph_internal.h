@@ -120,6 +120,7 @@ struct sph_ctx {
     int pass_bits[4];
     int num_tiles;
     int density_block, force_block;
+    int lanes;               // NAIVE formulation: lanes of a wavefront per particle in density / force (1..16)
     int keep_terms;
     int adjoint;             // SPH_FLAG_ADJOINT: reverse-mode buffers are part of the workspace
     bool adj_seeded;

@@ -181,11 +181,82 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_kernel(StepCons
     store_density(C, i, acc, pos, vel, rhoA);
 }
 
+// Density with L lanes per particle (L = 2, 4, 8 or 16; a group of L consecutive lanes of
+// one wavefront shares a particle).  The group walks the same rows and takes each row's
+// candidates L at a time (lane l tests candidate k0 + l + L m), so a particle with many
+// candidates is spread over L lanes: small or dense problems get L times the parallelism.
+// Hits are appended in candidate order (a group ballot ranks them), so the neighbour list is
+// the one the one-lane kernel builds; the density is summed per lane, then across the group.
+template <bool BATCH, int L>
+__global__ __launch_bounds__(kDensityThreads) void naive_density_group_kernel(StepConsts G, float4* pos, float4* vel,
+                                                                               const uint32_t* __restrict__ cs,
+                                                                               float* __restrict__ rhoA,
+                                                                               uint32_t* __restrict__ nl,
+                                                                               uint32_t* __restrict__ nn) {
+    static_assert(L >= 2 && L <= 16 && (L & (L - 1)) == 0, "lanes per particle: 2, 4, 8 or 16");
+    __shared__ uint32_t s_list[kLdsListCap][kDensityThreads];
+    const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int i = (int)(gt / L);
+    if (i >= G.n) return;  // groups are aligned, so a whole group leaves together
+    const uint32_t l = gt & (L - 1);
+    const uint32_t col = threadIdx.x / L;
+    const uint32_t gbase = (threadIdx.x & 63u) & ~(uint32_t)(L - 1);
+    const StepConsts& C = inst_consts<BATCH>(G, i);
+    const __amdgpu_buffer_rsrc_t rpos = make_rsrc(pos, (uint32_t)G.n * 16u);
+    const __amdgpu_buffer_rsrc_t rcs = make_rsrc(cs, ((uint32_t)G.cs_cells + 1u) * 4u);
+    const float4 p = pos[i];
+    const size_t stride = (size_t)G.n;
+    float acc = 0.0f;
+    uint32_t cnt = 0;
+    const float hs = C.h_search;
+    const float hs2 = hs * hs;
+    const int ylo = grid_coord(p.y - hs, 1, C), yhi = grid_coord(p.y + hs, 1, C);
+    const int zlo = grid_coord(p.z - hs, 2, C), zhi = grid_coord(p.z + hs, 2, C);
+    for (int cz = zlo; cz <= zhi; ++cz) {
+        const float ez = slab_dist(p.z, cz, 2, C);
+        for (int cy = ylo; cy <= yhi; ++cy) {
+            const float ey = slab_dist(p.y, cy, 1, C);
+            const float rem = hs2 - ez * ez - ey * ey;
+            if (rem < 0.0f) continue;
+            const float e = __builtin_amdgcn_sqrtf(rem);
+            const int xlo = grid_coord(p.x - e, 0, C), xhi = grid_coord(p.x + e, 0, C);
+            const uint32_t row = (uint32_t)(C.cell_off + C.dims[0] * (cy + C.dims[1] * cz));
+            const uint32_t k0 = ld_u32(rcs, (row + (uint32_t)xlo) * 4u);
+            const uint32_t k1 = ld_u32(rcs, (row + (uint32_t)xhi + 1u) * 4u);
+            for (uint32_t base = k0; base < k1; base += L) {
+                const uint32_t k = base + l;
+                const bool valid = k < k1;
+                const float4 q = ld_f4(rpos, (valid ? k : k0) * 16u);
+                const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+                const float r2 = valid ? dx * dx + dy * dy + dz * dz : INFINITY;
+                const float t = fmaxf(1.0f - r2 * C.inv_h2, 0.0f);
+                acc += t * t * t;
+                const bool hit = r2 <= C.h2;
+                const uint32_t gm = (uint32_t)(__ballot(hit) >> gbase) & ((1u << L) - 1u);
+                if (hit) {
+                    const uint32_t slot = cnt + (uint32_t)__popc(gm & ((1u << l) - 1u));
+                    if (slot < (uint32_t)kLdsListCap) s_list[slot][col] = k;
+                    else if (slot < (uint32_t)kListCap) nl[slot * stride + i] = k;
+                }
+                cnt += (uint32_t)__popc(gm);
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < L; o <<= 1) acc += __shfl_xor(acc, o);
+    const uint32_t staged = cnt < (uint32_t)kLdsListCap ? cnt : (uint32_t)kLdsListCap;
+    for (uint32_t t = l; t < staged; t += L) nl[t * stride + i] = s_list[t][col];
+    if (l == 0) {
+        nn[i] = cnt;
+        store_density(C, i, acc, pos, vel, rhoA);
+    }
+}
+
 // Force pass (naive_kernels.py:24-80) fused with integration and collision: each lane
 // walks its particle's neighbour list (one coalesced load per slot, kForceUnroll slots and
 // their gathers in flight); a particle with more neighbours than list slots rescans its
 // candidates instead.  MODE selects the wall collision at compile time.
-template <int MODE, bool BATCH>
+template <int MODE, bool BATCH, int L = 1>
 __global__ __launch_bounds__(256) void naive_force_kernel(
     StepConsts G, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
     const float* __restrict__ rhoA, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ nl,
@@ -193,11 +264,13 @@ __global__ __launch_bounds__(256) void naive_force_kernel(
     uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
     uint64_t* __restrict__ rng, const PipeTable* __restrict__ pipe, float4* __restrict__ dbgP,
     float4* __restrict__ dbgV, float4* __restrict__ dbgF, uint32_t* __restrict__ flags) {
-    const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int i = (int)(gt / L);
     if (i >= G.n) return;
     const StepConsts& C = inst_consts<BATCH>(G, i);
     const float4 p = pos[i], v = vel[i];
-    const PairSums S = naive_pair_sums<BATCH>(C, i, G.n, p, v, pos, vel, cs, nl, nn);
+    const PairSums S = naive_pair_sums_lanes<BATCH, L>(C, i, gt & (L - 1), G.n, p, v, pos, vel, cs, nl, nn);
+    if (L > 1 && (gt & (L - 1)) != 0) return;
     const float ps = 0.5f * C.mass * C.grad_c;   // naive_kernels.py:40-52: m * (pi+pj)/(2 rho_j) * grad W
     const float vs = C.mass * C.visc * C.lap_c;  // naive_kernels.py:71-77: m * mu * (vj-vi)/rho_j * lap W
     finish_particle<MODE>(C, i, p, v, pidA[i], rhoA[i], Vec3{ps * S.ax, ps * S.ay, ps * S.az},
@@ -485,16 +558,42 @@ hipError_t launch_reorder(sph_ctx* c, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <int L>
+void launch_density_lanes(sph_ctx* c, hipStream_t s) {
+    const int bs = c->density_block;
+    const Buffers& b = c->b;
+    if constexpr (L == 1)
+        (c->sc.n_inst > 1 ? naive_density_kernel<true> : naive_density_kernel<false>)<<<blocks_for(c->n, bs), bs, 0, s>>>(
+            c->sc, b.posA, b.velA, b.cs, b.rhoA, b.nl, b.nn);
+    else
+        (c->sc.n_inst > 1 ? naive_density_group_kernel<true, L> : naive_density_group_kernel<false, L>)<<<blocks_for(c->n * L, bs), bs, 0, s>>>(
+            c->sc, b.posA, b.velA, b.cs, b.rhoA, b.nl, b.nn);
+}
+
 hipError_t launch_density(sph_ctx* c, hipStream_t s) {
     if (c->n == 0) return hipGetLastError();
     const int bs = c->density_block;
-    if (c->sc.formulation == SPH_FORM_VOXEL)
+    if (c->sc.formulation == SPH_FORM_VOXEL) {
         voxel_density_kernel<<<blocks_for(c->n, bs), bs, 0, s>>>(c->sc, c->b.posA, c->b.velA, c->b.cs, c->b.rhoA,
                                                                    c->b.nbr);
-    else
-        (c->sc.n_inst > 1 ? naive_density_kernel<true> : naive_density_kernel<false>)<<<blocks_for(c->n, bs), bs, 0, s>>>(c->sc, c->b.posA, c->b.velA, c->b.cs, c->b.rhoA,
-                                                                   c->b.nl, c->b.nn);
+        return hipGetLastError();
+    }
+    switch (c->lanes) {
+        case 2: launch_density_lanes<2>(c, s); break;
+        case 4: launch_density_lanes<4>(c, s); break;
+        case 8: launch_density_lanes<8>(c, s); break;
+        case 16: launch_density_lanes<16>(c, s); break;
+        default: launch_density_lanes<1>(c, s); break;
+    }
     return hipGetLastError();
+}
+
+template <int MODE, int L>
+void launch_force_lanes(sph_ctx* c, hipStream_t s, float4* dP, float4* dV, float4* dF) {
+    const int bs = c->force_block;
+    (c->sc.n_inst > 1 ? naive_force_kernel<MODE, true, L> : naive_force_kernel<MODE, false, L>)<<<blocks_for(c->n * L, bs), bs, 0, s>>>(
+        c->sc, c->b.posA, c->b.velA, c->b.pidA, c->b.rhoA, c->b.cs, c->b.nl, c->b.nn, c->b.posB, c->b.velB,
+        c->b.pidB, c->b.keyI, c->b.valI, c->b.rng, c->b.pipe, dP, dV, dF, c->b.flags);
 }
 
 template <int MODE>
@@ -503,14 +602,19 @@ void launch_force_mode(sph_ctx* c, hipStream_t s) {
     float4* dP = c->keep_terms ? c->b.dbgP : nullptr;
     float4* dV = c->keep_terms ? c->b.dbgV : nullptr;
     float4* dF = c->keep_terms ? c->b.dbgF : nullptr;
-    if (c->sc.formulation == SPH_FORM_VOXEL)
+    if (c->sc.formulation == SPH_FORM_VOXEL) {
         voxel_force_kernel<MODE><<<blocks_for(c->n, bs), bs, 0, s>>>(
             c->sc, c->b.posA, c->b.velA, c->b.pidA, c->b.rhoA, c->b.nbr, c->b.posB, c->b.velB, c->b.pidB, c->b.keyI,
             c->b.valI, c->b.rng, c->b.pipe, dP, dV, dF, c->b.flags);
-    else
-        (c->sc.n_inst > 1 ? naive_force_kernel<MODE, true> : naive_force_kernel<MODE, false>)<<<blocks_for(c->n, bs), bs, 0, s>>>(
-            c->sc, c->b.posA, c->b.velA, c->b.pidA, c->b.rhoA, c->b.cs, c->b.nl, c->b.nn, c->b.posB, c->b.velB,
-            c->b.pidB, c->b.keyI, c->b.valI, c->b.rng, c->b.pipe, dP, dV, dF, c->b.flags);
+        return;
+    }
+    switch (c->lanes) {
+        case 2: launch_force_lanes<MODE, 2>(c, s, dP, dV, dF); break;
+        case 4: launch_force_lanes<MODE, 4>(c, s, dP, dV, dF); break;
+        case 8: launch_force_lanes<MODE, 8>(c, s, dP, dV, dF); break;
+        case 16: launch_force_lanes<MODE, 16>(c, s, dP, dV, dF); break;
+        default: launch_force_lanes<MODE, 1>(c, s, dP, dV, dF); break;
+    }
 }
 
 hipError_t launch_force(sph_ctx* c, hipStream_t s) {

@@ -58,7 +58,7 @@ EXPORTED = [
     "sph_abi_version", "sph_last_error", "sph_create", "sph_destroy", "sph_workspace_bytes", "sph_bind_workspace",
     "sph_upload_state", "sph_upload_state_f32", "sph_step", "sph_download_state", "sph_download_state_f32",
     "sph_run_stage", "sph_buffer_elems", "sph_read_buffer", "sph_diagnostics", "sph_grid_info", "sph_set_launch",
-    "sph_create_batch", "sph_batch_info", "sph_set_constants", "sph_adjoint_seed", "sph_adjoint_step",
+    "sph_create_batch", "sph_batch_info", "sph_set_constants", "sph_set_lanes", "sph_adjoint_seed", "sph_adjoint_step",
     "sph_adjoint_read",
 ]
 
@@ -96,6 +96,7 @@ def load_library() -> ctypes.CDLL:
         "sph_create_batch": ([POINTER(SphParamsC), POINTER(SphConstantsC), i32, POINTER(vp)], i32),
         "sph_batch_info": ([vp, POINTER(i32), POINTER(i32)], i32),
         "sph_set_constants": ([vp, POINTER(SphConstantsC)], i32),
+        "sph_set_lanes": ([vp, i32], i32),
         "sph_adjoint_seed": ([vp, vp, vp, vp], i32),
         "sph_adjoint_step": ([vp, vp], i32),
         "sph_adjoint_read": ([vp, vp, vp, vp, vp], i32),
@@ -244,6 +245,9 @@ class NativeContext:
 
     def set_launch(self, density_block: int, force_block: int):
         check(self.lib.sph_set_launch(self.handle, density_block, force_block), "sph_set_launch")
+
+    def set_lanes(self, lanes: int):
+        check(self.lib.sph_set_lanes(self.handle, int(lanes)), "sph_set_lanes")
 
     def set_constants(self, consts: SphConstantsC):
         check(self.lib.sph_set_constants(self.handle, ctypes.byref(consts)), "sph_set_constants")

@@ -1,8 +1,10 @@
 """Autotuner for the neighbour kernels' launch shapes and the grid they walk.
 
-Two knobs decide how the step maps onto the GPU:
+Three knobs decide how the step maps onto the GPU:
   - the workgroup sizes of the density and force passes (64, 128 or 256 threads: one to
-    four wavefronts over consecutive cell-ordered particles); and
+    four wavefronts over consecutive cell-ordered particles);
+  - the lanes per particle of those passes (1 to 16 lanes of a wavefront share one
+    particle's candidates: small or dense problems gain parallelism); and
   - the grid cell edge, which sets how many cells a workgroup's particles cover ("cells
     per tile") and how many candidates each particle tests.  Smaller cells test fewer
     candidates outside the h-sphere but walk more cell rows.  Only the NAIVE formulation
@@ -34,6 +36,7 @@ from .workloads import voxel_for
 
 BLOCKS = (64, 128, 256)
 CELL_SCALES = (2.0, 1.0, 0.5, 0.25)
+LANES = (1, 2, 4, 8, 16)
 
 
 @dataclass(frozen=True)
@@ -42,9 +45,10 @@ class TuneResult:
     density_block: int
     force_block: int
     ms_per_step: float
-    trials: Tuple[Tuple[Optional[float], int, int, float], ...] = field(default=())
+    trials: Tuple[Tuple[Optional[float], int, int, int, float], ...] = field(default=())  # scale, blocks, lanes, ms
     seconds: float = 0.0          # time the search took (0 when read from the cache)
     cached: bool = False
+    lanes: int = 1                # lanes per particle of the density and force passes
 
     @property
     def launch(self) -> Tuple[int, int]:
@@ -61,7 +65,7 @@ class TuneResult:
     @staticmethod
     def from_json(d: dict) -> "TuneResult":
         return TuneResult(d["cell_scale"], int(d["density_block"]), int(d["force_block"]), float(d["ms_per_step"]),
-                          tuple(tuple(t) for t in d.get("trials", ())), 0.0, True)
+                          tuple(tuple(t) for t in d.get("trials", ())), 0.0, True, int(d.get("lanes", 1)))
 
 
 def default_cache_path() -> str:
@@ -137,38 +141,51 @@ def _time_steps(step, reset, steps: int, warmup: int) -> float:
 
 
 def _search(make, scales: Sequence[Optional[float]], blocks: Sequence[int], steps: int, warmup: int,
-            log=None) -> TuneResult:
-    """make(scale) -> (set_launch(db, fb), step(k), reset(), close()) for one grid; every
-    block pair is timed on that grid."""
+            log=None, lanes: Sequence[int] = (1,)) -> TuneResult:
+    """make(scale) -> (configure(db, fb, lanes), step(k), reset(), close()) for one grid.
+    First every (grid, lanes) pair is timed with 256-thread workgroups, then every block
+    pair on the fastest of them."""
     t0 = time.perf_counter()
-    trials: List[Tuple[Optional[float], int, int, float]] = []
-    for scale in scales:
+    trials: List[Tuple[Optional[float], int, int, int, float]] = []
+
+    def run(scale, pairs):
         try:
-            set_launch, step, reset, close = make(scale)
+            configure, step, reset, close = make(scale)
         except RuntimeError as e:  # e.g. a grid too fine for the planner's limits
             if log:
                 log(f"autotune: cell scale {scale} rejected: {e}")
-            continue
+            return
         try:
-            for db in blocks:
-                for fb in blocks:
-                    set_launch(db, fb)
-                    ms = _time_steps(step, reset, steps, warmup)
-                    trials.append((scale, db, fb, round(ms, 5)))
+            for db, fb, ln in pairs:
+                try:
+                    configure(db, fb, ln)
+                except RuntimeError as e:  # e.g. too many threads for this lane count
                     if log:
-                        log(f"autotune: scale {scale} blocks {db}/{fb}: {ms:.4f} ms/step")
+                        log(f"autotune: lanes {ln} rejected: {e}")
+                    continue
+                ms = _time_steps(step, reset, steps, warmup)
+                trials.append((scale, db, fb, ln, round(ms, 5)))
+                if log:
+                    log(f"autotune: scale {scale} blocks {db}/{fb} lanes {ln}: {ms:.4f} ms/step")
         finally:
             close()
+
+    top = max(blocks)
+    for scale in scales:
+        run(scale, [(top, top, ln) for ln in lanes])
     if not trials:
         raise RuntimeError("autotune: no candidate could be planned")
-    best = min(trials, key=lambda t: t[3])
-    return TuneResult(best[0], best[1], best[2], best[3], tuple(trials), time.perf_counter() - t0, False)
+    s_best, _, _, l_best, _ = min(trials, key=lambda t: t[4])
+    run(s_best, [(db, fb, l_best) for db in blocks for fb in blocks if (db, fb) != (top, top)])
+    best = min(trials, key=lambda t: t[4])
+    return TuneResult(best[0], best[1], best[2], best[4], tuple(trials), time.perf_counter() - t0, False, best[3])
 
 
 def tune_strategy(params: SimulationParameters, constants: SPHConstants, state: SimulationState, mode: str = "BOX",
-                  formulation: str = "NAIVE", steps: int = 10, warmup: int = 2,
+                  formulation: str = "NAIVE", steps: int = 20, warmup: int = 3,
                   scales: Sequence[float] = (1.0, 0.5), blocks: Sequence[int] = BLOCKS,
-                  cache: Optional[TuneCache] = None, device="cuda:0", log=None) -> TuneResult:
+                  lanes: Sequence[int] = LANES, cache: Optional[TuneCache] = None, device="cuda:0",
+                  log=None) -> TuneResult:
     """Tunes the plain step for one simulation.  VOXEL: block sizes only."""
     from .strategy import NaiveSPHStrategy, VoxelSPHStrategy
 
@@ -185,18 +202,22 @@ def tune_strategy(params: SimulationParameters, constants: SPHConstants, state: 
     def make(scale):
         gv = None if scale is None else voxel_for(constants.h) * scale
         strat = cls(params, constants=constants, mode=mode, device=device, grid_voxel=gv)
-        return (strat.ctx.set_launch, strat.ctx.step, lambda: strat.ctx.upload(pos, vel), strat.ctx.close)
 
-    res = _search(make, grid_scales, blocks, steps, warmup, log)
+        def configure(db, fb, ln):
+            strat.ctx.set_launch(db, fb)
+            strat.ctx.set_lanes(ln)
+        return configure, strat.ctx.step, lambda: strat.ctx.upload(pos, vel), strat.ctx.close
+
+    res = _search(make, grid_scales, blocks, steps, warmup, log, lanes if formulation == "NAIVE" else (1,))
     if cache is not None:
         cache.put(key, res)
     return res
 
 
 def tune_batch(instances: Sequence[Tuple[SimulationParameters, SPHConstants]], states: Sequence[SimulationState],
-               steps: int = 10, warmup: int = 2, scales: Sequence[float] = CELL_SCALES[:3],
-               blocks: Sequence[int] = BLOCKS, cache: Optional[TuneCache] = None, device="cuda:0",
-               log=None) -> TuneResult:
+               steps: int = 20, warmup: int = 3, scales: Sequence[float] = CELL_SCALES[:3],
+               blocks: Sequence[int] = BLOCKS, lanes: Sequence[int] = LANES, cache: Optional[TuneCache] = None,
+               device="cuda:0", log=None) -> TuneResult:
     """Tunes the batched form: one cell scale for all instances (each instance's grid edge
     is voxel_for(h_b) times it) and one block pair for the shared launches."""
     from .batched import BatchedSPH
@@ -211,9 +232,13 @@ def tune_batch(instances: Sequence[Tuple[SimulationParameters, SPHConstants]], s
         b = BatchedSPH(instances, device=device, cell_scale=scale)
         b.upload(states)
         pos, vel = b.ctx_state_tensors()
-        return (b.ctx.set_launch, b.step, lambda: b.ctx.upload(pos, vel), b.ctx.close)
 
-    res = _search(make, list(scales), blocks, steps, warmup, log)
+        def configure(db, fb, ln):
+            b.ctx.set_launch(db, fb)
+            b.ctx.set_lanes(ln)
+        return configure, b.step, lambda: b.ctx.upload(pos, vel), b.ctx.close
+
+    res = _search(make, list(scales), blocks, steps, warmup, log, lanes)
     if cache is not None:
         cache.put(key, res)
     return res

@@ -22,7 +22,7 @@ from .workloads import voxel_for
 
 class BatchedSPH:
     def __init__(self, instances: Sequence[Tuple[SimulationParameters, SPHConstants]], device="cuda:0",
-                 launch: Optional[Tuple[int, int]] = None, cell_scale: Optional[float] = None):
+                 launch: Optional[Tuple[int, int]] = None, cell_scale: Optional[float] = None, lanes: int = 1):
         """launch: (density, force) threads per workgroup.  cell_scale: when given, instance
         b searches a grid of edge voxel_for(h_b) * cell_scale instead of its voxel_size
         (the NAIVE sums do not depend on the grid; see autotune.py)."""
@@ -41,6 +41,8 @@ class BatchedSPH:
         self.n = self.ctx.n
         if launch:
             self.ctx.set_launch(*launch)
+        if lanes != 1:
+            self.ctx.set_lanes(lanes)
         self._graphs = {}
 
     # -- state ---------------------------------------------------------------------------

@@ -36,20 +36,35 @@ class Tape:
 
 
 class DifferentiableSPH:
+    """graphs: replay the forward rollout and each backward segment as HIP graphs (one
+    launch instead of hundreds; captured once per viscosity value, since the step's
+    constants are kernel arguments)."""
+
     def __init__(self, params: SimulationParameters, constants: SPHConstants, steps: int, checkpoint_every: int = 20,
-                 device="cuda:0"):
+                 device="cuda:0", graphs: bool = True, launch: Optional[Tuple[int, int]] = None, lanes: int = 1):
         if steps < 1 or checkpoint_every < 1:
             raise ValueError("steps and checkpoint_every must be positive")
         self.params = params
         self.constants = constants
         self.steps = int(steps)
         self.every = int(checkpoint_every)
+        self.graphs = bool(graphs)
         p = N.make_params(params.particle_count, params.external_force, 1.0 / params.fps, params.space_size,
                           params.voxel_size, N.SPH_MODE_BOX, N.SPH_FORM_NAIVE, flags=N.SPH_FLAG_ADJOINT)
         self.ctx = N.NativeContext(p, N.make_consts(constants), device)
         self.device = self.ctx.device
         self.n = self.ctx.n
-        self._seg = torch.empty((self.every, 6, self.n), dtype=torch.float32, device=self.device)
+        if launch:
+            self.ctx.set_launch(*launch)
+        if lanes != 1:
+            self.ctx.set_lanes(lanes)
+        f32 = dict(dtype=torch.float32, device=self.device)
+        self._seg = torch.empty((self.every, 6, self.n), **f32)       # states of the segment being reversed
+        self._in = torch.empty((6, self.n), **f32)                    # rollout input (x0, v0 rows)
+        self._ck = torch.empty((self.num_checkpoints, 6, self.n), **f32)
+        self._out = torch.empty((6, self.n), **f32)
+        self._fwd_graphs = {}
+        self._seg_graphs = {}
 
     @property
     def num_checkpoints(self) -> int:
@@ -71,20 +86,50 @@ class DifferentiableSPH:
         return torch.cat([x.t(), v.t()]).contiguous()
 
     # -- passes ------------------------------------------------------------------------
-    def forward(self, x0: torch.Tensor, v0: torch.Tensor, mu: float) -> Tuple[torch.Tensor, torch.Tensor, Tape]:
-        """Runs the rollout; returns x_T, v_T (float32 (N, 3), particle order) and the tape."""
-        self.set_viscosity(mu)
-        ck = torch.empty((self.num_checkpoints, 6, self.n), dtype=torch.float32, device=self.device)
-        ck[0] = self._rows(x0, v0)
-        self._load(ck[0])
+    def _forward_body(self) -> None:
+        """_in -> _ck (every `every` steps, _ck[0] = _in) and _out."""
+        self._load(self._in)
         for t in range(1, self.steps):
             self.ctx.step(1)
             if t % self.every == 0:
-                self._save(ck[t // self.every])
+                self._save(self._ck[t // self.every])
         self.ctx.step(1)
-        out = torch.empty((6, self.n), dtype=torch.float32, device=self.device)
-        self._save(out)
-        return out[:3].t().contiguous(), out[3:].t().contiguous(), Tape(ck, float(mu), self.steps)
+        self._save(self._out)
+
+    def _segment_body(self, length: int) -> None:
+        """Reverses one segment whose first state is in _seg[0]: recomputes its states, then
+        pulls the cotangent back through its steps, last to first."""
+        self._load(self._seg[0])
+        for a in range(1, length):
+            self.ctx.step(1)
+            self._save(self._seg[a])
+        for a in reversed(range(length)):
+            self._load(self._seg[a])
+            self.ctx.adjoint_step()
+
+    def _run(self, cache: dict, key, body) -> None:
+        if not self.graphs:
+            body()
+            return
+        g = cache.get(key)
+        if g is None:
+            g = torch.cuda.CUDAGraph()
+            torch.cuda.synchronize(self.device)
+            with torch.cuda.graph(g):
+                body()
+            cache[key] = g
+        g.replay()
+
+    def forward(self, x0: torch.Tensor, v0: torch.Tensor, mu: float) -> Tuple[torch.Tensor, torch.Tensor, Tape]:
+        """Runs the rollout; returns x_T, v_T (float32 (N, 3), particle order) and the tape."""
+        mu = float(mu)
+        self.set_viscosity(mu)
+        self._in.copy_(self._rows(x0, v0))
+        self._run(self._fwd_graphs, mu, self._forward_body)
+        ck = self._ck.clone()
+        ck[0] = self._in
+        out = self._out
+        return out[:3].t().contiguous(), out[3:].t().contiguous(), Tape(ck, mu, self.steps)
 
     def backward(self, tape: Tape, gx: Optional[torch.Tensor], gv: Optional[torch.Tensor]):
         """Cotangents of x_T, v_T (float32 (N, 3) or None) -> (dL/dx0, dL/dv0, dL/dmu)."""
@@ -94,15 +139,9 @@ class DifferentiableSPH:
         gv = None if gv is None else gv.detach().to(**f32).reshape(self.n, 3).contiguous()
         self.ctx.adjoint_seed(gx, gv)
         for seg in reversed(range(self.num_checkpoints)):
-            t0 = seg * self.every
-            length = min(self.every, tape.steps - t0)
-            self._load(tape.checkpoints[seg])
-            for a in range(1, length):
-                self.ctx.step(1)
-                self._save(self._seg[a])
-            for a in reversed(range(length)):
-                self._load(tape.checkpoints[seg] if a == 0 else self._seg[a])
-                self.ctx.adjoint_step()
+            length = min(self.every, tape.steps - seg * self.every)
+            self._seg[0].copy_(tape.checkpoints[seg])
+            self._run(self._seg_graphs, (tape.visc, length), lambda: self._segment_body(length))
         return self.ctx.adjoint_read()
 
 

@@ -41,10 +41,11 @@ class SPHStrategy:
 
     def __init__(self, params: SimulationParameters, constants: Optional[SPHConstants] = None, mode: str = "BOX",
                  device="cuda:0", keep_terms: bool = False, seed: int = RNG_SEED, grid_voxel: Optional[float] = None,
-                 launch: Optional[Tuple[int, int]] = None):
+                 launch: Optional[Tuple[int, int]] = None, lanes: int = 1):
         """grid_voxel: NAIVE formulation only, the edge of the search grid's cells in place
         of params.voxel_size (the NAIVE sums do not depend on it; see autotune.py).
-        launch: (density, force) threads per workgroup."""
+        launch: (density, force) threads per workgroup.  lanes: NAIVE only, lanes of a
+        wavefront per particle in the density and force passes."""
         if mode not in _MODES:
             raise Exception(f"Wrong simulation mode! ({mode})")
         if grid_voxel is not None and self.formulation != N.SPH_FORM_NAIVE:
@@ -67,6 +68,8 @@ class SPHStrategy:
         self.device = self.ctx.device
         if launch:
             self.ctx.set_launch(*launch)
+        if lanes != 1:
+            self.ctx.set_lanes(lanes)
         logger.info("SPH strategy on %s: %s", self.device, self.ctx.grid_info())
 
     # -- reference entry point -----------------------------------------------------------
@@ -84,6 +87,16 @@ class SPHStrategy:
         self.ctx.step(int(nsteps))
         self._finalize_computation()
         return self.new_state
+
+    def capture(self, nsteps: int) -> torch.cuda.CUDAGraph:
+        """A HIP graph of nsteps steps of the resident state (recorded, not run).  The step
+        records no memsets and no host work, so each replay advances the resident state by
+        nsteps steps exactly as ctx.step(nsteps) does."""
+        g = torch.cuda.CUDAGraph()
+        torch.cuda.synchronize(self.device)
+        with torch.cuda.graph(g):
+            self.ctx.step(int(nsteps))
+        return g
 
     # -- stages --------------------------------------------------------------------------
     def _initialize_computation(self):

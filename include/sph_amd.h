@@ -199,6 +199,13 @@ int sph_grid_info(const sph_ctx* ctx, int32_t* dims3, int32_t* stencil3, int32_t
 /* Launch shapes of the neighbour kernels (threads per block), for the autotuner. */
 int sph_set_launch(sph_ctx* ctx, int32_t density_block, int32_t force_block);
 
+/* Lanes of a wavefront that share one particle in the NAIVE density and force passes
+ * (1, 2, 4, 8 or 16; default 1), for the autotuner: small or dense problems (few particles,
+ * many neighbours each) get that many times the parallelism.  The neighbour lists are the
+ * same for every value; sums are added in a different order (FP32 rounding only).  The
+ * reference launches one thread per particle (thread_layout.py:27-33). */
+int sph_set_lanes(sph_ctx* ctx, int32_t lanes);
+
 /* Replaces the physics constants of a plain context between steps (the reference freezes
  * them at JIT time, config.py:18-23).  mass, rho0, visc, k and damp may change; h must be
  * the one the context was created with (it fixes the voxel grid).  Host only. */

@@ -7,11 +7,11 @@ from sph_amd import workloads as W
 
 def test_cache_round_trip(tmp_path):
     cache = T.TuneCache(str(tmp_path / "tune.json"))
-    res = T.TuneResult(0.5, 128, 256, 0.123, ((1.0, 64, 64, 0.2), (0.5, 128, 256, 0.123)), 1.5, False)
+    res = T.TuneResult(0.5, 128, 256, 0.123, ((1.0, 64, 64, 1, 0.2), (0.5, 128, 256, 4, 0.123)), 1.5, False, 4)
     cache.put("k", res)
     got = cache.get("k")
     assert got.cached and got.seconds == 0.0
-    assert (got.cell_scale, got.density_block, got.force_block, got.ms_per_step) == (0.5, 128, 256, 0.123)
+    assert (got.cell_scale, got.density_block, got.force_block, got.ms_per_step, got.lanes) == (0.5, 128, 256, 0.123, 4)
     assert got.trials == res.trials
     assert cache.get("other") is None
 
@@ -31,7 +31,9 @@ def test_signature_tracks_workload_and_order():
 
 
 def test_search_picks_fastest_and_skips_rejected_grids(monkeypatch):
-    cost = {(1.0, 64, 64): 3.0, (1.0, 64, 128): 2.0, (1.0, 128, 64): 4.0, (1.0, 128, 128): 5.0}
+    # phase 1: (grid, lanes) at 128/128; phase 2: the other block pairs at the best of them
+    cost = {(1.0, 128, 128, 1): 5.0, (1.0, 128, 128, 4): 4.5, (2.0, 128, 128, 1): 6.0, (2.0, 128, 128, 4): 7.0,
+            (1.0, 64, 64, 4): 3.0, (1.0, 64, 128, 4): 2.0, (1.0, 128, 64, 4): 4.0}
     state = {}
     closed = []
 
@@ -39,14 +41,16 @@ def test_search_picks_fastest_and_skips_rejected_grids(monkeypatch):
         if scale == 0.25:
             raise RuntimeError("grid too fine")
 
-        def set_launch(db, fb):
-            state["cfg"] = (scale, db, fb)
-        return set_launch, lambda k: None, lambda: None, lambda: closed.append(scale)
+        def configure(db, fb, ln):
+            if ln == 16:
+                raise RuntimeError("too many threads")
+            state["cfg"] = (scale, db, fb, ln)
+        return configure, lambda k: None, lambda: None, lambda: closed.append(scale)
 
     monkeypatch.setattr(T, "_time_steps", lambda step, reset, steps, warmup: cost[state["cfg"]])
-    res = T._search(make, [1.0, 0.25], (64, 128), 5, 1)
-    assert (res.cell_scale, res.density_block, res.force_block, res.ms_per_step) == (1.0, 64, 128, 2.0)
-    assert len(res.trials) == 4 and closed == [1.0]
+    res = T._search(make, [1.0, 0.25, 2.0], (64, 128), 5, 1, lanes=(1, 4, 16))
+    assert (res.cell_scale, res.density_block, res.force_block, res.lanes, res.ms_per_step) == (1.0, 64, 128, 4, 2.0)
+    assert len(res.trials) == 7 and closed == [1.0, 2.0, 1.0]
 
 
 def test_grid_voxel_is_power_of_two():

@@ -67,10 +67,12 @@ def dev(a, gpu):
     return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(gpu)
 
 
+@pytest.mark.parametrize("lanes", [1, 4])
 @pytest.mark.parametrize("name", ["box", "walls"])
-def test_adjoint_step_writes_the_step_bitwise(gpu, name):
+def test_adjoint_step_writes_the_step_bitwise(gpu, name, lanes):
     w, x, v = case(name)
     sim = sim_for(w, gpu=gpu)
+    sim.ctx.set_lanes(lanes)
     rows = sim._rows(dev(x, gpu), dev(v, gpu))
     sim._load(rows)
     sim.ctx.step(1)
@@ -82,13 +84,15 @@ def test_adjoint_step_writes_the_step_bitwise(gpu, name):
     assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("lanes", [1, 4])
 @pytest.mark.parametrize("name", ["box", "walls"])
-def test_adjoint_step_matches_oracle(gpu, name):
+def test_adjoint_step_matches_oracle(gpu, name, lanes):
     w, x, v = case(name)
     rng = np.random.default_rng(5)
     gx1 = f32(rng.normal(size=x.shape))
     gv1 = f32(rng.normal(size=x.shape))
     sim = sim_for(w, gpu=gpu)
+    sim.ctx.set_lanes(lanes)
     sim._load(sim._rows(dev(x, gpu), dev(v, gpu)))
     sim.ctx.adjoint_seed(dev(gx1, gpu), dev(gv1, gpu))
     sim.ctx.adjoint_step()
@@ -169,13 +173,16 @@ def test_16k_finite_differences_smooth_horizon(gpu):
     assert worst <= 5e-4 and emu <= 1e-4
 
 
-def test_full_workload_gradient_along_gpu_trajectory(gpu):
+@pytest.mark.parametrize("lanes", [1, 16])
+def test_full_workload_gradient_along_gpu_trajectory(gpu, lanes):
     """The differentiable workload as specified (16,384 particles, 200 steps, checkpoints
-    every 20): the GPU gradient against the FP64 pullback through the same FP32 states."""
+    every 20): the GPU gradient against the FP64 pullback through the same FP32 states,
+    with one and with sixteen lanes per particle (different FP32 summation orders, so
+    different trajectories and, the map being ill-conditioned, different gradients)."""
     steps = 200
     w = W.adjoint_16k()
     c, n = w.constants, w.params.particle_count
-    sim = sim_for(w, steps=steps, every=20, gpu=gpu)
+    sim = DifferentiableSPH(w.params, c, steps=steps, checkpoint_every=20, device=gpu, lanes=lanes)
     x0 = dev(w.state.position, gpu)
     v0 = dev(w.state.velocity, gpu)
     x5, _, _ = sim.forward(x0, v0, 5.0)
@@ -203,5 +210,22 @@ def test_full_workload_gradient_along_gpu_trajectory(gpu):
     ev = np.linalg.norm(gv0 - gv) / np.linalg.norm(gv)
     ex = np.linalg.norm(gx0 - gx) / np.linalg.norm(gx)
     em = abs(gmu[0] - mu_ref) / abs(mu_ref)
-    print(f"200 steps along the GPU trajectory: dv0 {ev:.2e} dx0 {ex:.2e} dmu {em:.2e}")
+    print(f"200 steps, {lanes} lanes, along the GPU trajectory: dv0 {ev:.2e} dx0 {ex:.2e} dmu {em:.2e} "
+          f"(|dL/dv0| {np.linalg.norm(gv):.3e}, dL/dmu {mu_ref:.3e})")
     assert ev <= 2e-5 and ex <= 2e-5 and em <= 2e-5
+
+
+def test_graph_replay_equals_eager_launches(gpu):
+    """The forward rollout and the backward segments replayed as HIP graphs give bitwise
+    the results of launching every kernel directly; a second call (graph reuse, another
+    viscosity) too."""
+    w, x, v = case("box")
+    g = sim_for(w, steps=9, every=4, gpu=gpu)
+    e = DifferentiableSPH(w.params, w.constants, steps=9, checkpoint_every=4, device=gpu, graphs=False)
+    for mu in (3.5, 5.0, 3.5):
+        xg, vg, tg = g.forward(dev(x, gpu), dev(v, gpu), mu)
+        xe, ve, te = e.forward(dev(x, gpu), dev(v, gpu), mu)
+        assert torch.equal(xg, xe) and torch.equal(vg, ve) and torch.equal(tg.checkpoints, te.checkpoints)
+        seed = torch.ones_like(xg)
+        for s, t in zip(g.backward(tg, seed, None), e.backward(te, seed, None)):
+            assert torch.equal(s, t)

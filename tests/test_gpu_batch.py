@@ -31,14 +31,17 @@ def _instances(seed=0, count=6):
     return out
 
 
-def test_batch_equals_plain_per_instance(gpu):
+@pytest.mark.parametrize("lanes", [1, 8])
+def test_batch_equals_plain_per_instance(gpu, lanes):
     ws = _instances()
     batch = BatchedSPH([(w.params, w.constants) for w in ws], device=gpu)
+    batch.ctx.set_lanes(lanes)
     batch.upload([w.state for w in ws])
     batch.step(10)
     got = batch.download()
     for b, w in enumerate(ws):
         plain = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu)
+        plain.ctx.set_lanes(lanes)
         ref = plain.advance(w.state, 10)
         assert np.array_equal(got[b].position, ref.position), b
         assert np.array_equal(got[b].velocity, ref.velocity), b

@@ -295,3 +295,60 @@ def test_diagnostics_fp64(gpu):
     assert np.allclose(d["momentum"], m * v.sum(axis=0), rtol=1e-12, atol=1e-12)
     assert math.isclose(d["kinetic_energy"], 0.5 * m * np.sum(v * v), rel_tol=1e-12)
     assert d["non_finite"] == 0
+
+
+@pytest.mark.parametrize("mode", ["BOX", "PIPE"])
+def test_graph_replays_match_eager(gpu, mode):
+    """A captured run of steps, replayed several times with eager work and synchronisation
+    in between, advances the state exactly as eager steps do (PIPE: RNG streams included)."""
+    w = box_uniform(3000, seed=23) if mode == "BOX" else pipe_flow(n=3000, seed=24)
+    pos, vel = f32(w.state.position), f32(w.state.velocity)
+    eager = NaiveSPHStrategy(w.params, constants=w.constants, mode=mode, device=gpu)
+    graph = NaiveSPHStrategy(w.params, constants=w.constants, mode=mode, device=gpu)
+    upload(eager, pos, vel)
+    upload(graph, pos, vel)
+    g = graph.capture(5)
+    for r in range(3):
+        eager.ctx.step(5)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        a = [t.cpu().numpy() for t in eager.ctx.download()]
+        b = [t.cpu().numpy() for t in graph.ctx.download()]
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y), r
+        if mode == "PIPE":
+            assert np.array_equal(read(eager, N.BUF_RNG), read(graph, N.BUF_RNG))
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16])
+@pytest.mark.parametrize("dense", [False, True])
+def test_lanes_per_particle_match_oracle(gpu, lanes, dense):
+    """Density and force with several lanes per particle: the step stays within the parity
+    bars; dense = more neighbours than list slots (the rescan path of the force pass)."""
+    if dense:
+        w = box_uniform(3000, seed=25)
+        pos = f32(np.random.default_rng(26).uniform([0.1, 0.1, 0.1], [0.22, 0.22, 0.22], size=(3000, 3)))
+    else:
+        w = box_uniform(4096, seed=27)
+        pos = f32(w.state.position)
+    vel = f32(np.random.default_rng(28).normal(0, 0.3, size=pos.shape))
+    strat = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu, keep_terms=True, launch=(64, 128))
+    strat.ctx.set_lanes(lanes)
+    out = strat.compute_next_state(SimulationState(pos, vel, np.zeros(len(pos))))
+    rp, rv, rr, rf, _ = oracle_step(w, pos, vel)
+    assert np.max(np.abs(out.density - rr) / rr) <= TOL
+    scale = np.max(np.abs(rf), axis=1, keepdims=True) + 1e-30
+    assert np.max(np.abs(strat.result_force - rf) / scale) <= 1e-4
+    assert np.max(np.abs(out.position - rp) / np.maximum(np.abs(rp), 1.0)) <= TOL
+
+
+def test_set_lanes_rejects_bad_values(gpu):
+    w = box_uniform(100, seed=29)
+    s = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu)
+    for bad in (0, 3, 32):
+        with pytest.raises(RuntimeError, match="lanes per particle"):
+            s.ctx.set_lanes(bad)
+    v = VoxelSPHStrategy(w.params, constants=w.constants, device=gpu)
+    with pytest.raises(RuntimeError, match="VOXEL"):
+        v.ctx.set_lanes(4)
