@@ -57,6 +57,53 @@ __device__ __forceinline__ void walk_neighbours(const StepConsts& C, int i, floa
     }
 }
 
+// The same walk with L lanes per particle: lane l visits list slots l, l + L, ... (or every
+// L-th candidate of each row past the list); the caller adds the lanes' sums.
+template <int L, typename F>
+__device__ __forceinline__ void walk_neighbours_lanes(const StepConsts& C, int i, uint32_t l, float4 p,
+                                                      const float4* __restrict__ pos, const uint32_t* __restrict__ cs,
+                                                      const uint32_t* __restrict__ nl, const uint32_t* __restrict__ nn,
+                                                      F&& f) {
+    if constexpr (L == 1) {
+        walk_neighbours(C, i, p, pos, cs, nl, nn, f);
+    } else {
+        const uint32_t cnt = nn[i];
+        if (cnt <= (uint32_t)kListCap) {
+            const size_t stride = (size_t)C.n;
+            for (uint32_t t = l; t < cnt; t += L) f(nl[t * stride + i]);
+        } else {
+            const float hs = C.h_search;
+            const float hs2 = hs * hs;
+            const int ylo = grid_coord(p.y - hs, 1, C), yhi = grid_coord(p.y + hs, 1, C);
+            const int zlo = grid_coord(p.z - hs, 2, C), zhi = grid_coord(p.z + hs, 2, C);
+            for (int cz = zlo; cz <= zhi; ++cz) {
+                const float ez = slab_dist(p.z, cz, 2, C);
+                for (int cy = ylo; cy <= yhi; ++cy) {
+                    const float ey = slab_dist(p.y, cy, 1, C);
+                    const float rem = hs2 - ez * ez - ey * ey;
+                    if (rem < 0.0f) continue;
+                    const float e = sqrtf(rem);
+                    const int xlo = grid_coord(p.x - e, 0, C), xhi = grid_coord(p.x + e, 0, C);
+                    const int row = C.cell_off + C.dims[0] * (cy + C.dims[1] * cz);
+                    const uint32_t k1 = cs[row + xhi + 1];
+                    for (uint32_t k = cs[row + xlo] + l; k < k1; k += L) {
+                        const float4 q = pos[k];
+                        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+                        if (dx * dx + dy * dy + dz * dz <= C.h2) f(k);
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <int L>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+    for (int o = 1; o < L; o <<= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
 __device__ __forceinline__ float comp(const Vec3& a, int d) { return d == 0 ? a.x : (d == 1 ? a.y : a.z); }
 __device__ __forceinline__ float comp(const float4& a, int d) { return d == 0 ? a.x : (d == 1 ? a.y : a.z); }
 
@@ -102,12 +149,15 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_seed_kernel(
     gvs[i] = make_float4(gvp[0], gvp[1], gvp[2], 0.0f);
 }
 
+template <int L>
 __global__ __launch_bounds__(kAdjThreads) void adjoint_pair_kernel(
     StepConsts C, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
     const uint32_t* __restrict__ cs, const uint32_t* __restrict__ nl, const uint32_t* __restrict__ nn,
     const float4* __restrict__ fb, float4* __restrict__ gxs, const float4* __restrict__ gvs, float* __restrict__ rt,
     float4* __restrict__ cotv) {
-    const int k = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int k = (int)(gt / L);
+    const uint32_t lane = gt & (L - 1);
     if (k >= C.n) return;
     const float4 p = pos[k], v = vel[k], fk = fb[k];
     const float ik = v.w, h = C.h;
@@ -116,7 +166,7 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_pair_kernel(
     float rbv = 0.f, rbp = 0.f, pb = 0.f;
     const float ps = 0.5f * C.mass * C.grad_c;
     const float vs = C.mass * C.visc * C.lap_c;
-    walk_neighbours(C, k, p, pos, cs, nl, nn, [&](uint32_t j) {
+    walk_neighbours_lanes<L>(C, k, lane, p, pos, cs, nl, nn, [&](uint32_t j) {
         if (j == (uint32_t)k) return;
         const float4 q = pos[j], w = vel[j], fj = fb[j];
         const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
@@ -150,22 +200,37 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_pair_kernel(
         gxy += sv * dy + ps * (g * by + gp * dy);
         gxz += sv * dz + ps * (g * bz + gp * dz);
     });
+    if constexpr (L > 1) {
+        gvx = group_sum<L>(gvx);
+        gvy = group_sum<L>(gvy);
+        gvz = group_sum<L>(gvz);
+        gxx = group_sum<L>(gxx);
+        gxy = group_sum<L>(gxy);
+        gxz = group_sum<L>(gxz);
+        rbv = group_sum<L>(rbv);
+        rbp = group_sum<L>(rbp);
+        pb = group_sum<L>(pb);
+        if (lane != 0) return;
+    }
     rt[k] = fk.w - vs * ik * ik * rbv - ps * ik * ik * rbp + C.k * ps * pb;
     const float4 gx = gxs[k], gv = gvs[k];
     gxs[k] = make_float4(gx.x + gxx, gx.y + gxy, gx.z + gxz, gx.w);
     cotv[pidA[k]] = make_float4(gv.x + vs * gvx, gv.y + vs * gvy, gv.z + vs * gvz, 0.0f);
 }
 
+template <int L>
 __global__ __launch_bounds__(kAdjThreads) void adjoint_density_kernel(
     StepConsts C, const float4* __restrict__ pos, const uint32_t* __restrict__ pidA, const uint32_t* __restrict__ cs,
     const uint32_t* __restrict__ nl, const uint32_t* __restrict__ nn, const float* __restrict__ rt,
     const float4* __restrict__ gxs, float4* __restrict__ cotx) {
-    const int k = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int k = (int)(gt / L);
+    const uint32_t lane = gt & (L - 1);
     if (k >= C.n) return;
     const float4 p = pos[k];
     const float rk = rt[k];
     float ax = 0.f, ay = 0.f, az = 0.f;
-    walk_neighbours(C, k, p, pos, cs, nl, nn, [&](uint32_t j) {
+    walk_neighbours_lanes<L>(C, k, lane, p, pos, cs, nl, nn, [&](uint32_t j) {
         const float4 q = pos[j];
         const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         const float u = fmaxf(1.0f - (dx * dx + dy * dy + dz * dz) * C.inv_h2, 0.0f);
@@ -174,6 +239,12 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_density_kernel(
         ay += c * dy;
         az += c * dz;
     });
+    if constexpr (L > 1) {
+        ax = group_sum<L>(ax);
+        ay = group_sum<L>(ay);
+        az = group_sum<L>(az);
+        if (lane != 0) return;
+    }
     const float s = -6.0f * C.w_norm * C.inv_h2;
     const float4 gx = gxs[k];
     cotx[pidA[k]] = make_float4(gx.x + s * ax, gx.y + s * ay, gx.z + s * az, 0.0f);
@@ -237,27 +308,32 @@ hipError_t launch_adjoint_get(sph_ctx* c, float* gx, float* gv, float* gmu, hipS
     return hipGetLastError();
 }
 
+// The seed, pair and density pullbacks with L lanes per particle (the step's own count).
+template <int L>
+void launch_pullback_lanes(sph_ctx* c, hipStream_t s) {
+    const Buffers& b = c->b;
+    const int nb = adj_blocks(c->n * L);
+    adjoint_seed_kernel<L><<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.velA, b.pidA, b.rhoA, b.cs, b.nl, b.nn, b.posB,
+                                                       b.velB, b.pidB, b.keyI, b.valI, b.flags, b.cotx, b.cotv,
+                                                       b.adj_fb, b.adj_gx, b.adj_gv);
+    adjoint_pair_kernel<L><<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.velA, b.pidA, b.cs, b.nl, b.nn, b.adj_fb,
+                                                       b.adj_gx, b.adj_gv, b.adj_rt, b.cotv);
+    adjoint_density_kernel<L><<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.pidA, b.cs, b.nl, b.nn, b.adj_rt, b.adj_gx,
+                                                          b.cotx);
+}
+
 // After sort, cell table and density of the step: the seed, pair, density and mu kernels.
 hipError_t launch_adjoint_pullback(sph_ctx* c, hipStream_t s) {
     const int64_t n = c->n;
     if (n == 0) return hipGetLastError();
     const Buffers& b = c->b;
-    const int nb = adj_blocks(n);
-    auto seed = adjoint_seed_kernel<1>;
     switch (c->lanes) {
-        case 2: seed = adjoint_seed_kernel<2>; break;
-        case 4: seed = adjoint_seed_kernel<4>; break;
-        case 8: seed = adjoint_seed_kernel<8>; break;
-        case 16: seed = adjoint_seed_kernel<16>; break;
-        default: break;
+        case 2: launch_pullback_lanes<2>(c, s); break;
+        case 4: launch_pullback_lanes<4>(c, s); break;
+        case 8: launch_pullback_lanes<8>(c, s); break;
+        case 16: launch_pullback_lanes<16>(c, s); break;
+        default: launch_pullback_lanes<1>(c, s); break;
     }
-    seed<<<adj_blocks(n * c->lanes), kAdjThreads, 0, s>>>(c->sc, b.posA, b.velA, b.pidA, b.rhoA, b.cs, b.nl, b.nn,
-                                                            b.posB, b.velB, b.pidB, b.keyI, b.valI, b.flags, b.cotx,
-                                                            b.cotv, b.adj_fb, b.adj_gx, b.adj_gv);
-    adjoint_pair_kernel<<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.velA, b.pidA, b.cs, b.nl, b.nn, b.adj_fb, b.adj_gx,
-                                                    b.adj_gv, b.adj_rt, b.cotv);
-    adjoint_density_kernel<<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.pidA, b.cs, b.nl, b.nn, b.adj_rt, b.adj_gx,
-                                                       b.cotx);
     adjoint_mu_kernel<<<1, kAdjThreads, 0, s>>>((int)n, b.adj_gx, b.mu_acc);
     return hipGetLastError();
 }

@@ -84,7 +84,7 @@ def test_adjoint_step_writes_the_step_bitwise(gpu, name, lanes):
     assert torch.equal(a, b)
 
 
-@pytest.mark.parametrize("lanes", [1, 4])
+@pytest.mark.parametrize("lanes", [1, 4, 16])
 @pytest.mark.parametrize("name", ["box", "walls"])
 def test_adjoint_step_matches_oracle(gpu, name, lanes):
     w, x, v = case(name)
