@@ -377,6 +377,7 @@ def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
     # default grid and blocks and with the batch's tuned ones, the faster counted
     configs = [(None, None, 1)] + ([(scale, launch, lanes)] if tune else [])
     t_seq = 0.0
+    per_instance = []
     for w in bw.instances:
         best = None
         for sc, la, ln in configs:
@@ -395,12 +396,16 @@ def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
             best = t if best is None else min(best, t)
             del strat
         t_seq += best
+        per_instance.append((best * 1e3 / steps, w.params.particle_count, w.name))
     log(f"sequential plain runs: {t_seq * 1e3:.1f} ms")
+    slowest = sorted(per_instance, reverse=True)[:5]
+    log("slowest instances alone (ms/step, particles): " + ", ".join(f"{ms:.3f} ({n})" for ms, n, _ in slowest))
     return {"workload": "batch_64", "instances": len(bw.instances), "particles": total, "steps": steps,
             "value": round(total * steps / t_batch, 1), "unit": "particle-updates/s",
             "ms_per_step": round(t_batch * 1e3 / steps, 4), "graph": True,
             "sequential_plain_value": round(total * steps / t_seq, 1),
-            "speedup_vs_sequential": round(t_seq / t_batch, 3), "tuning": tuning}
+            "speedup_vs_sequential": round(t_seq / t_batch, 3), "tuning": tuning,
+            "slowest_instance_alone_ms_per_step": round(slowest[0][0], 4)}
 
 
 def bench_adjoint(steps: int = 200, every: int = 20, tune: bool = True):

@@ -81,7 +81,7 @@ size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 enum Slice {
     S_POSA, S_VELA, S_POSB, S_VELB, S_PIDA, S_PIDB, S_RHOA, S_KEYI, S_VALI, S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_CS,
     S_HIST, S_DTOT, S_RNG, S_DBGP, S_DBGV, S_DBGF, S_NBR, S_NL, S_NN, S_DIAG, S_FLAGS, S_PIPE, S_INST, S_INSTF,
-    S_COTX, S_COTV, S_AFB, S_AGX, S_AGV, S_ART, S_MU, S_COUNT
+    S_COTX, S_COTV, S_AFB, S_AGX, S_AGV, S_ART, S_MU, S_CCNT, S_CPART, S_COUNT
 };
 
 struct Layout {
@@ -111,6 +111,8 @@ Layout plan_layout(const sph_ctx* c) {
     sz[S_COTX] = sz[S_COTV] = sz[S_AFB] = sz[S_AGX] = sz[S_AGV] = c->adjoint ? 16 * nn : 16;
     sz[S_ART] = c->adjoint ? 4 * nn : 16;
     sz[S_MU] = 64;
+    sz[S_CCNT] = c->cell_sort ? 4 * (size_t)(c->sc.cs_cells > 0 ? c->sc.cs_cells : 1) : 16;
+    sz[S_CPART] = c->cell_sort ? 4 * (size_t)sph::cell_scan_tiles(c->sc.cs_cells > 0 ? c->sc.cs_cells : 1) : 16;
     Layout L;
     size_t o = 0;
     for (int i = 0; i < S_COUNT; ++i) {
@@ -208,6 +210,7 @@ void plan_sort_and_layout(sph_ctx* c) {
         shift += bits;
     }
     c->num_tiles = sph::radix_num_tiles(c->n);
+    c->cell_sort = c->sc.formulation == SPH_FORM_NAIVE ? 1 : 0;
     if (c->density_block <= 0) c->density_block = 256;
     if (c->force_block <= 0) c->force_block = 256;
     if (c->lanes <= 0) c->lanes = 1;
@@ -277,6 +280,11 @@ int plan_batch(sph_ctx* c, const sph_params_t* params, const sph_constants_t* co
 // Sort (with the gather into cell order), cell table and density of a step.
 int run_density_stages(sph_ctx* c, hipStream_t s, bool fused) {
     int rc;
+    if (c->cell_sort) {  // the counting sort builds the cell table itself
+        if ((rc = hip_check(sph::launch_cell_sort(c, fused, s), "cell sort"))) return rc;
+        if (!fused && (rc = hip_check(sph::launch_reorder(c, s), "reorder"))) return rc;
+        return hip_check(sph::launch_density(c, s), "density");
+    }
     if (fused && c->passes > 0) {
         if ((rc = hip_check(sph::launch_radix_sort_impl(c, true, s), "sort"))) return rc;
     } else {
@@ -414,11 +422,15 @@ int sph_bind_workspace(sph_ctx* c, void* dev, size_t bytes, void* stream) {
     b.adj_gv = (float4*)(base + L.off[S_AGV]);
     b.adj_rt = (float*)(base + L.off[S_ART]);
     b.mu_acc = (float*)(base + L.off[S_MU]);
+    b.cell_cnt = (uint32_t*)(base + L.off[S_CCNT]);
+    b.cell_part = (uint32_t*)(base + L.off[S_CPART]);
     c->ws = dev;
     c->sorted_keys = b.keyI;
     c->sorted_vals = b.valI;
     int rc;
     if ((rc = hip_check(sph::launch_fill_u32(b.flags, 0u, 16, s), "clear flags"))) return rc;
+    if (c->cell_sort && (rc = hip_check(sph::launch_fill_u32(b.cell_cnt, 0u, c->sc.cs_cells, s), "clear cell counts")))
+        return rc;
     if ((rc = hip_check(hipMemcpyAsync(b.pipe, &c->pipe_host, sizeof(sph::PipeTable), hipMemcpyHostToDevice, s),
                         "upload pipe table")))
         return rc;
@@ -486,8 +498,13 @@ int sph_run_stage(sph_ctx* c, int32_t stage, void* stream) {
     const int expected = c->last_stage == SPH_STAGE_FORCE ? SPH_STAGE_SORT : c->last_stage + 1;
     if (stage != expected) return fail(SPH_E_STATE, "stage %d out of order (expected %d)", stage, expected);
     switch (stage) {
-        case SPH_STAGE_SORT: rc = hip_check(sph::launch_radix_sort_impl(c, false, s), "sort"); break;
-        case SPH_STAGE_TABLE: rc = hip_check(sph::launch_cell_table(c, s), "cell table"); break;
+        case SPH_STAGE_SORT:
+            rc = c->cell_sort ? hip_check(sph::launch_cell_sort(c, false, s), "cell sort")
+                              : hip_check(sph::launch_radix_sort_impl(c, false, s), "sort");
+            break;
+        case SPH_STAGE_TABLE:  // the counting sort has built the table already
+            rc = c->cell_sort ? SPH_OK : hip_check(sph::launch_cell_table(c, s), "cell table");
+            break;
         case SPH_STAGE_REORDER: rc = hip_check(sph::launch_reorder(c, s), "reorder"); break;
         case SPH_STAGE_DENSITY: rc = hip_check(sph::launch_density(c, s), "density"); break;
         case SPH_STAGE_FORCE: rc = hip_check(sph::launch_force(c, s), "force"); break;

@@ -1,0 +1,220 @@
+// sph_cellsort.hip -- counting sort of the NAIVE formulation's cell keys for gfx950.
+//
+// The NAIVE path's keys are cell indices in [0, C) (clamped to the grid), so the order the
+// step needs -- by (cell, particle id), the order of the reference's stable sort of the voxel
+// map (voxel_sph_strategy.py:81-89) -- is a counting sort by cell followed by id order within
+// each cell.  Per step:
+//   count    one thread per particle id: rank = atomicAdd(&cnt[key], 1) (an arbitrary order
+//            among the particles of a cell)
+//   scan     exclusive prefix sum of cnt over all cells in three launches (tile sums, a
+//            one-workgroup scan of the tile sums, tile scans); the result is the cell start
+//            table cs[0..C] itself, and the tile scan resets cnt to zero for the next step
+//   scatter  one thread per particle id: tmp[cs[key] + rank] = id
+//   order    one thread per slot: the id there moves to cs[key] + (number of ids of its cell
+//            that are smaller), which is id order within the cell; the sorted key and slot
+//            are written there and, in the fused step, the particle's state is gathered
+// The within-cell pass costs one comparison per particle of the cell, less than the density
+// pass spends on the same cell.  The final order does not depend on the atomic order.
+#include "sph_internal.h"
+
+namespace sph {
+namespace {
+
+constexpr int CT = 256;          // threads per workgroup
+constexpr int CI = 16;           // cells per thread in the scan
+constexpr int CTILE = CT * CI;   // cells per scan tile
+constexpr int CWAVES = CT / 64;
+
+// Consecutive particle ids are often in the same cell, so a wavefront's lanes come in runs of
+// equal keys: the first lane of each run adds the run's length with one atomic and the run's
+// lanes take consecutive ranks from the returned base.
+__global__ __launch_bounds__(CT) void cell_count_kernel(int n, const uint32_t* __restrict__ keyI,
+                                                         uint32_t* __restrict__ cnt, uint32_t* __restrict__ rank) {
+    const int i = blockIdx.x * CT + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool valid = i < n;
+    const uint32_t key = valid ? keyI[i] : 0xffffffffu;
+    const uint32_t prev = __shfl_up(key, 1);
+    const bool head = valid && (lane == 0 || prev != key);
+    const uint64_t heads = __ballot(head);
+    const uint64_t valids = __ballot(valid);
+    // first lane of this lane's run, and the first lane of the next run (or the end)
+    const uint64_t below = heads & (lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull));
+    const int first = 63 - __clzll((long long)below);
+    const uint64_t above = heads & ~(lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull));
+    const int next = above ? __ffsll((unsigned long long)above) - 1 : 64 - __clzll((long long)valids);
+    uint32_t base = 0;
+    if (head) base = atomicAdd(&cnt[key], (uint32_t)(next - lane));
+    base = __shfl(base, first);
+    if (valid) rank[i] = base + (uint32_t)(lane - first);
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+// Exclusive scan of one value per thread across the workgroup; *total receives the sum.
+__device__ __forceinline__ uint32_t block_excl(uint32_t v, uint32_t* lw, uint32_t* total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t x = wave_incl_scan(v);
+    if (lane == 63) lw[w] = x;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < CWAVES; ++k) {
+        const uint32_t t = lw[k];
+        off += k < w ? t : 0u;
+        tot += t;
+    }
+    *total = tot;
+    __syncthreads();
+    return off + x - v;
+}
+
+__global__ __launch_bounds__(CT) void cell_scan_reduce(const uint32_t* __restrict__ cnt, int m,
+                                                        uint32_t* __restrict__ part) {
+    __shared__ uint32_t lw[CWAVES];
+    const int64_t base = (int64_t)blockIdx.x * CTILE;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < CI; ++k) {
+        const int64_t c = base + k * CT + threadIdx.x;
+        if (c < m) s += cnt[c];
+    }
+    uint32_t tot;
+    block_excl(s, lw, &tot);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// One workgroup: exclusive scan of the tile sums, in chunks of CT with a carry.
+__global__ __launch_bounds__(CT) void cell_scan_partials(uint32_t* __restrict__ part, int nparts) {
+    __shared__ uint32_t lw[CWAVES];
+    uint32_t carry = 0;
+    for (int base = 0; base < nparts; base += CT) {
+        const int i = base + threadIdx.x;
+        const uint32_t v = i < nparts ? part[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl(v, lw, &tot);
+        if (i < nparts) part[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+// The tile is read (and cnt cleared) with coalesced accesses into LDS, each thread scans CI
+// consecutive cells there, and cs is written back coalesced.  LDS index i is padded by
+// i / 32 so that the thread-contiguous pass does not hit one bank.  The last tile also
+// writes cs[m] = n.
+__device__ __forceinline__ int cpad(int i) { return i + (i >> 5); }
+
+__global__ __launch_bounds__(CT) void cell_scan_down(uint32_t* __restrict__ cnt, int m,
+                                                      const uint32_t* __restrict__ part, uint32_t* __restrict__ cs,
+                                                      uint32_t n) {
+    __shared__ uint32_t lw[CWAVES];
+    __shared__ uint32_t tile[CTILE + CTILE / 32];
+    const int64_t base = (int64_t)blockIdx.x * CTILE;
+#pragma unroll
+    for (int k = 0; k < CI; ++k) {
+        const int j = k * CT + threadIdx.x;
+        const int64_t c = base + j;
+        uint32_t v = 0u;
+        if (c < m) {
+            v = cnt[c];
+            cnt[c] = 0u;
+        }
+        tile[cpad(j)] = v;
+    }
+    __syncthreads();
+    const int j0 = threadIdx.x * CI;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < CI; ++k) s += tile[cpad(j0 + k)];
+    uint32_t tot;
+    uint32_t run = part[blockIdx.x] + block_excl(s, lw, &tot);  // synchronises
+#pragma unroll
+    for (int k = 0; k < CI; ++k) {
+        const uint32_t v = tile[cpad(j0 + k)];
+        tile[cpad(j0 + k)] = run;
+        run += v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < CI; ++k) {
+        const int j = k * CT + threadIdx.x;
+        if (base + j < m) cs[base + j] = tile[cpad(j)];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) cs[m] = n;
+}
+
+__global__ __launch_bounds__(CT) void cell_scatter_kernel(int n, const uint32_t* __restrict__ keyI,
+                                                           const uint32_t* __restrict__ rank,
+                                                           const uint32_t* __restrict__ cs, uint32_t* __restrict__ tmp) {
+    const int i = blockIdx.x * CT + threadIdx.x;
+    if (i < n) tmp[cs[keyI[i]] + rank[i]] = (uint32_t)i;
+}
+
+template <bool GATHER>
+__global__ __launch_bounds__(CT) void cell_order_kernel(int n, const uint32_t* __restrict__ tmp,
+                                                         const uint32_t* __restrict__ keyI,
+                                                         const uint32_t* __restrict__ valI,
+                                                         const uint32_t* __restrict__ cs, uint32_t* __restrict__ skey,
+                                                         uint32_t* __restrict__ sval, const float4* __restrict__ posB,
+                                                         const float4* __restrict__ velB,
+                                                         const uint32_t* __restrict__ pidB, float4* __restrict__ posA,
+                                                         float4* __restrict__ velA, uint32_t* __restrict__ pidA) {
+    const int s = blockIdx.x * CT + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t id = tmp[s];
+    const uint32_t k = keyI[id];
+    const uint32_t a = cs[k], b = cs[k + 1];
+    uint32_t r = 0;
+    for (uint32_t t = a; t < b; ++t) r += tmp[t] < id ? 1u : 0u;
+    const uint32_t f = a + r;
+    const uint32_t v = valI[id];
+    skey[f] = k;
+    sval[f] = v;
+    if (GATHER) {
+        posA[f] = posB[v];
+        velA[f] = velB[v];
+        pidA[f] = pidB[v];
+    }
+}
+
+inline int cblocks(int64_t n) { return (int)((n + CT - 1) / CT); }
+
+}  // namespace
+
+int64_t cell_scan_tiles(int64_t cells) { return (cells + CTILE - 1) / CTILE; }
+
+// Sorted keys and slots go to keyS[1] / valS[1]; the cell table to cs.
+hipError_t launch_cell_sort(sph_ctx* c, bool gather, hipStream_t s) {
+    const int n = (int)c->n;
+    const int m = c->sc.cs_cells;
+    const Buffers& b = c->b;
+    uint32_t* rank = b.keyS[0];
+    uint32_t* tmp = b.valS[0];
+    const int tiles = (int)cell_scan_tiles(m);
+    if (n > 0) cell_count_kernel<<<cblocks(n), CT, 0, s>>>(n, b.keyI, b.cell_cnt, rank);
+    cell_scan_reduce<<<tiles, CT, 0, s>>>(b.cell_cnt, m, b.cell_part);
+    cell_scan_partials<<<1, CT, 0, s>>>(b.cell_part, tiles);
+    cell_scan_down<<<tiles, CT, 0, s>>>(b.cell_cnt, m, b.cell_part, b.cs, (uint32_t)n);
+    if (n > 0) {
+        cell_scatter_kernel<<<cblocks(n), CT, 0, s>>>(n, b.keyI, rank, b.cs, tmp);
+        if (gather)
+            cell_order_kernel<true><<<cblocks(n), CT, 0, s>>>(n, tmp, b.keyI, b.valI, b.cs, b.keyS[1], b.valS[1],
+                                                               b.posB, b.velB, b.pidB, b.posA, b.velA, b.pidA);
+        else
+            cell_order_kernel<false><<<cblocks(n), CT, 0, s>>>(n, tmp, b.keyI, b.valI, b.cs, b.keyS[1], b.valS[1],
+                                                                nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    }
+    c->sorted_keys = b.keyS[1];
+    c->sorted_vals = b.valS[1];
+    return hipGetLastError();
+}
+
+}  // namespace sph

@@ -10,6 +10,7 @@
 //   keyI/valI  u32[N]     sort input in particle-id order: voxel key, slot in posB
 //   keyS/valS  u32[2][N]  radix ping-pong buffers
 //   cs         u32[C+1]   cell start table: cs[c] = first cell-sorted index with key >= c
+//   cell_cnt   u32[C]     NAIVE: per-cell counts of the counting sort (kept zero between steps)
 //   rng        u64[2N]    xoroshiro128+ stream per particle id (PIPE mode)
 //   cotx/cotv  float4[N]  reverse mode only: cotangent of the state in particle order
 //   nl/nn      u32[96][N], u32[N]  NAIVE neighbour lists of the density pass, read by the
@@ -98,6 +99,8 @@ struct Buffers {
     // cell-ordered particle the force cotangent (.w: rho cotangent of the integration),
     // position cotangent so far (.w: the particle's dL/dmu term), velocity cotangent of the
     // integration and the total rho cotangent
+    uint32_t* cell_cnt;    // NAIVE counting sort: particles per cell (zero between steps)
+    uint32_t* cell_part;   // NAIVE counting sort: per-tile sums of the cell-count scan
     float4 *cotx, *cotv;
     float4 *adj_fb, *adj_gx, *adj_gv;
     float* adj_rt;
@@ -123,6 +126,7 @@ struct sph_ctx {
     int lanes;               // NAIVE formulation: lanes of a wavefront per particle in density / force (1..16)
     int keep_terms;
     int adjoint;             // SPH_FLAG_ADJOINT: reverse-mode buffers are part of the workspace
+    int cell_sort;           // NAIVE formulation: counting sort by cell instead of the radix sort
     bool adj_seeded;
     size_t ws_bytes;
     void* ws;
@@ -146,6 +150,8 @@ hipError_t launch_upload_f64(sph_ctx* c, const double* pos, const double* vel, h
 hipError_t launch_upload_f32(sph_ctx* c, const float* x, const float* y, const float* z, const float* vx,
                              const float* vy, const float* vz, hipStream_t s);
 hipError_t launch_radix_sort_impl(sph_ctx* c, bool gather, hipStream_t s);
+hipError_t launch_cell_sort(sph_ctx* c, bool gather, hipStream_t s);  // sph_cellsort.hip; also builds cs
+int64_t cell_scan_tiles(int64_t cells);
 hipError_t launch_cell_table(sph_ctx* c, hipStream_t s);
 hipError_t launch_reorder(sph_ctx* c, hipStream_t s);
 hipError_t launch_density(sph_ctx* c, hipStream_t s);

@@ -14,7 +14,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(PKG_DIR), "csrc")
 LIB_NAME = "libsph_amd.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
-SOURCES = ["sph_capi.hip", "sph_kernels.hip", "sph_radix.hip", "sph_adjoint.hip"]
+SOURCES = ["sph_capi.hip", "sph_kernels.hip", "sph_radix.hip", "sph_cellsort.hip", "sph_adjoint.hip"]
 
 # Packed FP32 instructions are switched off for every HIP file (v_pk_*_f32 forms have
 # produced silently wrong results on these machines).  The host pass warns that it does

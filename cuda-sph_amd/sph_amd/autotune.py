@@ -141,20 +141,24 @@ def _time_steps(step, reset, steps: int, warmup: int) -> float:
 
 
 def _search(make, scales: Sequence[Optional[float]], blocks: Sequence[int], steps: int, warmup: int,
-            log=None, lanes: Sequence[int] = (1,)) -> TuneResult:
+            log=None, lanes: Sequence[int] = (1,), finalists: int = 3) -> TuneResult:
     """make(scale) -> (configure(db, fb, lanes), step(k), reset(), close()) for one grid.
-    First every (grid, lanes) pair is timed with 256-thread workgroups, then every block
-    pair on the fastest of them."""
+    First every (grid, lanes) pair is timed with the largest workgroups, then every block
+    pair on the fastest of them; the best `finalists` are timed once more and the fastest of
+    that second timing wins (the first timing of a run carries warm-up effects).  The very
+    first timing is repeated and discarded for the same reason."""
     t0 = time.perf_counter()
     trials: List[Tuple[Optional[float], int, int, int, float]] = []
+    warmed = [False]
 
-    def run(scale, pairs):
+    def run(scale, pairs, record):
         try:
             configure, step, reset, close = make(scale)
         except RuntimeError as e:  # e.g. a grid too fine for the planner's limits
             if log:
                 log(f"autotune: cell scale {scale} rejected: {e}")
-            return
+            return []
+        out = []
         try:
             for db, fb, ln in pairs:
                 try:
@@ -163,21 +167,31 @@ def _search(make, scales: Sequence[Optional[float]], blocks: Sequence[int], step
                     if log:
                         log(f"autotune: lanes {ln} rejected: {e}")
                     continue
+                if not warmed[0]:
+                    _time_steps(step, reset, steps, warmup)
+                    warmed[0] = True
                 ms = _time_steps(step, reset, steps, warmup)
-                trials.append((scale, db, fb, ln, round(ms, 5)))
+                out.append((scale, db, fb, ln, round(ms, 5)))
                 if log:
                     log(f"autotune: scale {scale} blocks {db}/{fb} lanes {ln}: {ms:.4f} ms/step")
         finally:
             close()
+        if record:
+            trials.extend(out)
+        return out
 
     top = max(blocks)
     for scale in scales:
-        run(scale, [(top, top, ln) for ln in lanes])
+        run(scale, [(top, top, ln) for ln in lanes], True)
     if not trials:
         raise RuntimeError("autotune: no candidate could be planned")
     s_best, _, _, l_best, _ = min(trials, key=lambda t: t[4])
-    run(s_best, [(db, fb, l_best) for db in blocks for fb in blocks if (db, fb) != (top, top)])
-    best = min(trials, key=lambda t: t[4])
+    run(s_best, [(db, fb, l_best) for db in blocks for fb in blocks if (db, fb) != (top, top)], True)
+    ranked = sorted(trials, key=lambda t: t[4])[:max(1, finalists)]
+    final = []
+    for scale in dict.fromkeys(t[0] for t in ranked):
+        final += run(scale, [t[1:4] for t in ranked if t[0] == scale], False)
+    best = min(final or ranked, key=lambda t: t[4])
     return TuneResult(best[0], best[1], best[2], best[4], tuple(trials), time.perf_counter() - t0, False, best[3])
 
 

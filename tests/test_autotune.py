@@ -50,7 +50,27 @@ def test_search_picks_fastest_and_skips_rejected_grids(monkeypatch):
     monkeypatch.setattr(T, "_time_steps", lambda step, reset, steps, warmup: cost[state["cfg"]])
     res = T._search(make, [1.0, 0.25, 2.0], (64, 128), 5, 1, lanes=(1, 4, 16))
     assert (res.cell_scale, res.density_block, res.force_block, res.lanes, res.ms_per_step) == (1.0, 64, 128, 4, 2.0)
-    assert len(res.trials) == 7 and closed == [1.0, 2.0, 1.0]
+    # two grid passes, the block pass, one re-timing pass for the three finalists (all on 1.0)
+    assert len(res.trials) == 7 and closed == [1.0, 2.0, 1.0, 1.0]
+
+
+def test_finalists_are_retimed(monkeypatch):
+    # the first timing of the default is slow (warm-up); its second timing wins
+    timings = {(1.0, 128, 128, 1): [9.0, 1.0, 1.0], (1.0, 128, 128, 2): [2.0, 2.0], (1.0, 64, 64, 1): [3.0, 3.0],
+               (1.0, 64, 128, 1): [4.0], (1.0, 128, 64, 1): [5.0]}
+    state = {}
+
+    def make(scale):
+        def configure(db, fb, ln):
+            state["cfg"] = (scale, db, fb, ln)
+        return configure, lambda k: None, lambda: None, lambda: None
+
+    def fake_time(step, reset, steps, warmup):
+        return timings[state["cfg"]].pop(0)
+
+    monkeypatch.setattr(T, "_time_steps", fake_time)
+    res = T._search(make, [1.0], (64, 128), 5, 1, lanes=(1, 2))
+    assert (res.density_block, res.force_block, res.lanes, res.ms_per_step) == (128, 128, 1, 1.0)
 
 
 def test_grid_voxel_is_power_of_two():
