@@ -336,8 +336,10 @@ def bench_pipe(steps: int = 1000, warmup: int = 5, tune: bool = True):
 def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
     """Aggregate particle-updates/s of the batch_64 workload: one launch per stage for all 64
     instances, the timed steps replayed as one HIP graph; against the same 64 simulations
-    advanced one after another with the plain step (same kernels, grid and block sizes, one
-    context each).  With tune, the grid cell scale and block sizes come from the autotuner."""
+    advanced one after another with the plain step (one context each, its steps replayed as
+    one HIP graph, the best of the default, the batch's and the instance's own autotuned
+    configuration).  With tune, each instance's grid scale and lanes per particle and the
+    shared block sizes come from the autotuner."""
     import numpy as np
     import torch
 
@@ -349,15 +351,20 @@ def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
     total = bw.total_particles
     states = [w.state for w in bw.instances]
     inst = [(w.params, w.constants) for w in bw.instances]
-    scale, launch, lanes, tuning = None, None, 1, None
+    count = len(bw.instances)
+    scale, launch, lanes, tuning = [None] * count, None, [1] * count, None
     if tune:
+        from collections import Counter
+
         from sph_amd.autotune import tune_batch
 
         log("autotuning the batch")
         res = tune_batch(inst, states, log=log)
-        scale, launch, lanes = res.cell_scale, res.launch, res.lanes
-        tuning = {"cell_scale": scale, "density_block": res.density_block, "force_block": res.force_block,
-                  "lanes": lanes, "trials": len(res.trials), "seconds": round(res.seconds, 2),
+        scale, launch, lanes = list(res.cell_scale), res.launch, list(res.lanes)
+        tuning = {"rule": list(res.rule), "density_block": res.density_block, "force_block": res.force_block,
+                  "instances_per_lanes": {str(k): v for k, v in sorted(Counter(lanes).items())},
+                  "instances_per_cell_scale": {str(k): v for k, v in sorted(Counter(scale).items())},
+                  "trials": len(res.trials), "seconds": round(res.seconds, 2),
                   "candidate_ms_per_step": [min(t[4] for t in res.trials), max(t[4] for t in res.trials)]}
         log(f"batch tuned: {tuning}")
     batch = BatchedSPH(inst, cell_scale=scale, launch=launch, lanes=lanes)
@@ -373,28 +380,38 @@ def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
     torch.cuda.synchronize()
     t_batch = e0.elapsed_time(e1) / 1e3
     log(f"batched: {steps} steps of {total} particles in {t_batch * 1e3:.1f} ms")
-    # the baseline gets its best case: each instance alone, timed with the plain step's
-    # default grid and blocks and with the batch's tuned ones, the faster counted
-    configs = [(None, None, 1)] + ([(scale, launch, lanes)] if tune else [])
+    # the baseline gets its best case: each instance alone, its steps replayed as one HIP
+    # graph of the plain step, with the plain step's default grid and blocks, with the batch's
+    # choices for that instance and (with tune) with the instance's own autotuned plain
+    # configuration; the fastest counted
     t_seq = 0.0
     per_instance = []
-    for w in bw.instances:
+    for b, w in enumerate(bw.instances):
+        configs = [(None, None, 1)]
+        if tune:
+            from sph_amd.autotune import TuneCache, tune_strategy
+
+            own = tune_strategy(w.params, w.constants, w.state, cache=TuneCache())
+            configs += [(scale[b], launch, lanes[b]), (own.cell_scale, own.launch, own.lanes)]
+        pos = torch.from_numpy(np.ascontiguousarray(w.state.position)).cuda()
+        vel = torch.from_numpy(np.ascontiguousarray(w.state.velocity)).cuda()
         best = None
-        for sc, la, ln in configs:
+        for sc, la, ln in dict.fromkeys(configs):
             gv = None if sc is None else workloads.voxel_for(w.constants.h) * sc
             strat = NaiveSPHStrategy(w.params, constants=w.constants, grid_voxel=gv, launch=la, lanes=ln)
-            strat.ctx.upload(torch.from_numpy(np.ascontiguousarray(w.state.position)).cuda(),
-                             torch.from_numpy(np.ascontiguousarray(w.state.velocity)).cuda())
+            strat.ctx.upload(pos, vel)
+            g = strat.capture(steps)  # recorded, not run
+            strat.ctx.upload(pos, vel)
             strat.ctx.step(warmup)
             torch.cuda.synchronize()
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record()
-            strat.ctx.step(steps)
-            b.record()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            g.replay()
+            e1.record()
             torch.cuda.synchronize()
-            t = a.elapsed_time(b) / 1e3
+            t = e0.elapsed_time(e1) / 1e3
             best = t if best is None else min(best, t)
-            del strat
+            del g, strat
         t_seq += best
         per_instance.append((best * 1e3 / steps, w.params.particle_count, w.name))
     log(f"sequential plain runs: {t_seq * 1e3:.1f} ms")

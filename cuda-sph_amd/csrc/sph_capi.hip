@@ -141,6 +141,7 @@ int plan_instance(const sph_params_t& P, const sph_constants_t& K, sph::StepCons
     S.ref_keys = P.formulation == SPH_FORM_VOXEL ? 1 : 0;
     S.pow2 = 1;
     S.n_inst = 1;
+    S.lanes = 1;
     int64_t cells = 1;
     for (int d = 0; d < 3; ++d) {
         const double sp = P.space_size[d], vx = P.voxel_size[d];
@@ -229,6 +230,45 @@ int plan(sph_ctx* c) {
     return SPH_OK;
 }
 
+// Thread ranges of the batched density and force passes: instance b takes n_b * lanes_b
+// threads from tfirst_b on, rounded up to whole wavefronts; the totals entry gets the count.
+int batch_thread_layout(sph_ctx* c, const std::vector<int>& lanes) {
+    int64_t t = 0;
+    for (size_t b = 0; b < c->inst.size(); ++b) {
+        const int L = lanes[b];
+        if (L != 1 && L != 2 && L != 4 && L != 8 && L != 16)
+            return fail(SPH_E_INVALID, "lanes per particle must be 1, 2, 4, 8 or 16 (instance %zu: %d)", b, L);
+        t += (((int64_t)c->inst[b].n * L) + 63) & ~int64_t(63);
+        if (t >= (int64_t(1) << 31))
+            return fail(SPH_E_INVALID, "the batch's particles x lanes exceed 2^31 threads");
+    }
+    t = 0;
+    for (size_t b = 0; b < c->inst.size(); ++b) {
+        sph::StepConsts& S = c->inst[b];
+        S.lanes = lanes[b];
+        S.lane_shift = __builtin_ctz((unsigned)lanes[b]);
+        S.tfirst = (int)t;
+        t += (((int64_t)S.n * S.lanes) + 63) & ~int64_t(63);
+    }
+    c->sc.tfirst = (int)t;
+    return SPH_OK;
+}
+
+// Instance table and particle offsets to the workspace (at bind, and when lanes change).
+int upload_instances(sph_ctx* c, hipStream_t s) {
+    sph::Buffers& b = c->b;
+    for (auto& st : c->inst) {
+        st.inst_table = b.inst_table;
+        st.inst_first = b.inst_first;
+    }
+    int rc;
+    if ((rc = hip_check(hipMemcpyAsync(b.inst_table, c->inst.data(), sizeof(sph::StepConsts) * c->inst.size(),
+                                       hipMemcpyHostToDevice, s), "upload instance table")))
+        return rc;
+    return hip_check(hipMemcpyAsync(b.inst_first, c->inst_first.data(), sizeof(int) * c->inst_first.size(),
+                                    hipMemcpyHostToDevice, s), "upload instance offsets");
+}
+
 // Batched form: instances are concatenated in the given order; instance b owns particle ids
 // [first_b, first_b + N_b) and cell keys [cell_off_b, cell_off_b + C_b).  All instances use
 // the NAIVE formulation and box walls.
@@ -271,6 +311,7 @@ int plan_batch(sph_ctx* c, const sph_params_t* params, const sph_constants_t* co
     c->sc.first = 0;
     c->sc.n_inst = count;
     c->n = n;
+    if (int rc = batch_thread_layout(c, std::vector<int>(count, 1))) return rc;
     c->keep_terms = (params[0].flags & SPH_FLAG_KEEP_TERMS) ? 1 : 0;
     memset(&c->pipe_host, 0, sizeof(c->pipe_host));
     plan_sort_and_layout(c);
@@ -441,16 +482,7 @@ int sph_bind_workspace(sph_ctx* c, void* dev, size_t bytes, void* stream) {
     if (!c->inst.empty()) {
         c->sc.inst_table = b.inst_table;
         c->sc.inst_first = b.inst_first;
-        for (auto& st : c->inst) {
-            st.inst_table = b.inst_table;
-            st.inst_first = b.inst_first;
-        }
-        if ((rc = hip_check(hipMemcpyAsync(b.inst_table, c->inst.data(), sizeof(sph::StepConsts) * c->inst.size(),
-                                           hipMemcpyHostToDevice, s), "upload instance table")))
-            return rc;
-        if ((rc = hip_check(hipMemcpyAsync(b.inst_first, c->inst_first.data(), sizeof(int) * c->inst_first.size(),
-                                           hipMemcpyHostToDevice, s), "upload instance offsets")))
-            return rc;
+        if ((rc = upload_instances(c, s))) return rc;
     }
     if ((rc = hip_check(hipStreamSynchronize(s), "bind"))) return rc;
     c->bound = true;
@@ -619,7 +651,33 @@ int sph_set_lanes(sph_ctx* c, int32_t lanes) {
         return fail(SPH_E_INVALID, "the VOXEL formulation walks its capped list with one lane per particle");
     if (c->n * (int64_t)lanes >= (int64_t(1) << 31))
         return fail(SPH_E_INVALID, "%lld particles x %d lanes exceed 2^31 threads", (long long)c->n, lanes);
+    if (!c->inst.empty()) {
+        std::vector<int32_t> all(c->inst.size(), lanes);
+        return sph_set_batch_lanes(c, all.data(), (int32_t)all.size(), nullptr);
+    }
     c->lanes = lanes;
+    return SPH_OK;
+}
+
+int sph_set_batch_lanes(sph_ctx* c, const int32_t* lanes, int32_t count, void* stream) {
+    if (!c || !lanes) return fail(SPH_E_INVALID, "null argument");
+    if (c->inst.empty()) return fail(SPH_E_STATE, "not a batched context (use sph_set_lanes)");
+    if (count != (int32_t)c->inst.size())
+        return fail(SPH_E_INVALID, "%d lane counts for %zu instances", count, c->inst.size());
+    const std::vector<sph::StepConsts> keep = c->inst;
+    const int keep_total = c->sc.tfirst;
+    int rc = batch_thread_layout(c, std::vector<int>(lanes, lanes + count));
+    if (rc == SPH_OK && c->bound) {
+        hipStream_t s = (hipStream_t)stream;
+        rc = upload_instances(c, s);
+        if (rc == SPH_OK) rc = hip_check(hipStreamSynchronize(s), "set lanes");
+    }
+    if (rc != SPH_OK) {
+        c->inst = keep;
+        c->sc.tfirst = keep_total;
+        return rc;
+    }
+    c->lanes = lanes[0];
     return SPH_OK;
 }
 

@@ -69,6 +69,12 @@ struct StepConsts {
     int n_inst;                            // 1 for a plain step
     const struct StepConsts* inst_table;   // device, n_inst entries (batched form)
     const int* inst_first;                 // device, n_inst + 1 particle offsets (batched form)
+    // batched form: lanes per particle of this instance in the NAIVE density and force passes
+    // (lanes = 1 << lane_shift) and the first thread of its range in those passes, a multiple
+    // of 64 so that no wavefront spans two instances; the totals entry holds the thread count
+    int lanes;
+    int lane_shift;
+    int tfirst;
 };
 
 struct PipeTable {

@@ -115,45 +115,51 @@ __device__ __forceinline__ void store_density(const StepConsts& C, int i, float 
 // Density (naive_kernels.py:7-21) and the particle's neighbour list: every k with
 // r^2 <= h^2 in candidate order, the particle itself included (the force pass gives the
 // self pair a zero contribution).  (1 - r^2/h^2)^3 is accumulated without a branch and is
-// zero outside h.  Candidates are read two at a time through 32-bit buffer offsets.
+// zero outside h.
 constexpr int kDensityThreads = 256;
 #ifndef SPH_DENSITY_UNROLL
 #define SPH_DENSITY_UNROLL 2
 #endif
 constexpr int kDensityUnroll = SPH_DENSITY_UNROLL;
 
-template <bool BATCH>
-__global__ __launch_bounds__(kDensityThreads) void naive_density_kernel(StepConsts G, float4* pos, float4* vel,
-                                                                         const uint32_t* __restrict__ cs,
-                                                                         float* __restrict__ rhoA,
-                                                                         uint32_t* __restrict__ nl,
-                                                                         uint32_t* __restrict__ nn) {
-    __shared__ uint32_t s_list[kLdsListCap][kDensityThreads];
-    const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= G.n) return;
-    const StepConsts& C = inst_consts<BATCH>(G, i);
+typedef uint32_t DensityList[kDensityThreads];
+
+// Density of cell-ordered particle i with L lanes per particle.  L = 1: the lane walks every
+// candidate row, two candidates in flight through 32-bit buffer offsets.  L = 2..16: a group
+// of L consecutive lanes of one wavefront shares the particle, walks the same rows and takes
+// each row's candidates L at a time (lane l tests candidate k0 + l + L m), so a particle with
+// many candidates is spread over L lanes; hits are appended in candidate order (a group
+// ballot ranks them), so the neighbour list is the one a single lane builds, and the density
+// is summed per lane, then across the group.  G holds the totals (particle count, cell
+// table length), C the particle's own instance; col is the particle's LDS list column
+// (unique in the workgroup), gbase its group's first lane in the wavefront.
+template <int L>
+__device__ __forceinline__ void density_particle(const StepConsts& G, const StepConsts& C, int i, uint32_t l,
+                                                 uint32_t col, uint32_t gbase, float4* pos, float4* vel,
+                                                 const uint32_t* __restrict__ cs, float* __restrict__ rhoA,
+                                                 uint32_t* __restrict__ nl, uint32_t* __restrict__ nn,
+                                                 DensityList* s_list) {
     const __amdgpu_buffer_rsrc_t rpos = make_rsrc(pos, (uint32_t)G.n * 16u);
     const __amdgpu_buffer_rsrc_t rcs = make_rsrc(cs, ((uint32_t)G.cs_cells + 1u) * 4u);
     const float4 p = pos[i];
     const size_t stride = (size_t)G.n;
     float acc = 0.0f;
     uint32_t cnt = 0;
-    auto visit = [&](float4 q, uint32_t off) {
+    const float hs = C.h_search;
+    const float hs2 = hs * hs;
+    const int ylo = grid_coord(p.y - hs, 1, C), yhi = grid_coord(p.y + hs, 1, C);
+    const int zlo = grid_coord(p.z - hs, 2, C), zhi = grid_coord(p.z + hs, 2, C);
+    auto visit = [&](float4 q, uint32_t k) {
         const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
         const float t = fmaxf(1.0f - r2 * C.inv_h2, 0.0f);
         acc += t * t * t;
         if (r2 <= C.h2) {
-            const uint32_t k = off >> 4;
-            if (cnt < (uint32_t)kLdsListCap) s_list[cnt][threadIdx.x] = k;
+            if (cnt < (uint32_t)kLdsListCap) s_list[cnt][col] = k;
             else if (cnt < (uint32_t)kListCap) nl[cnt * stride + i] = k;
             ++cnt;
         }
     };
-    const float hs = C.h_search;
-    const float hs2 = hs * hs;
-    const int ylo = grid_coord(p.y - hs, 1, C), yhi = grid_coord(p.y + hs, 1, C);
-    const int zlo = grid_coord(p.z - hs, 2, C), zhi = grid_coord(p.z + hs, 2, C);
     for (int cz = zlo; cz <= zhi; ++cz) {
         const float ez = slab_dist(p.z, cz, 2, C);
         for (int cy = ylo; cy <= yhi; ++cy) {
@@ -163,82 +169,37 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_kernel(StepCons
             const float e = __builtin_amdgcn_sqrtf(rem);  // within 1 ulp; hs carries the margin
             const int xlo = grid_coord(p.x - e, 0, C), xhi = grid_coord(p.x + e, 0, C);
             const uint32_t row = (uint32_t)(C.cell_off + C.dims[0] * (cy + C.dims[1] * cz));
-            uint32_t off = ld_u32(rcs, (row + (uint32_t)xlo) * 4u) * 16u;
-            const uint32_t end = ld_u32(rcs, (row + (uint32_t)xhi + 1u) * 4u) * 16u;
-            for (; off + 16u * (kDensityUnroll - 1) < end; off += 16u * kDensityUnroll) {
-                float4 q[kDensityUnroll];
-#pragma unroll
-                for (int u = 0; u < kDensityUnroll; ++u) q[u] = ld_f4(rpos, off + 16u * u);
-#pragma unroll
-                for (int u = 0; u < kDensityUnroll; ++u) visit(q[u], off + 16u * u);
-            }
-            for (; off < end; off += 16u) visit(ld_f4(rpos, off), off);
-        }
-    }
-    nn[i] = cnt;
-    const uint32_t staged = cnt < (uint32_t)kLdsListCap ? cnt : (uint32_t)kLdsListCap;
-    for (uint32_t t = 0; t < staged; ++t) nl[t * stride + i] = s_list[t][threadIdx.x];
-    store_density(C, i, acc, pos, vel, rhoA);
-}
-
-// Density with L lanes per particle (L = 2, 4, 8 or 16; a group of L consecutive lanes of
-// one wavefront shares a particle).  The group walks the same rows and takes each row's
-// candidates L at a time (lane l tests candidate k0 + l + L m), so a particle with many
-// candidates is spread over L lanes: small or dense problems get L times the parallelism.
-// Hits are appended in candidate order (a group ballot ranks them), so the neighbour list is
-// the one the one-lane kernel builds; the density is summed per lane, then across the group.
-template <bool BATCH, int L>
-__global__ __launch_bounds__(kDensityThreads) void naive_density_group_kernel(StepConsts G, float4* pos, float4* vel,
-                                                                               const uint32_t* __restrict__ cs,
-                                                                               float* __restrict__ rhoA,
-                                                                               uint32_t* __restrict__ nl,
-                                                                               uint32_t* __restrict__ nn) {
-    static_assert(L >= 2 && L <= 16 && (L & (L - 1)) == 0, "lanes per particle: 2, 4, 8 or 16");
-    __shared__ uint32_t s_list[kLdsListCap][kDensityThreads];
-    const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const int i = (int)(gt / L);
-    if (i >= G.n) return;  // groups are aligned, so a whole group leaves together
-    const uint32_t l = gt & (L - 1);
-    const uint32_t col = threadIdx.x / L;
-    const uint32_t gbase = (threadIdx.x & 63u) & ~(uint32_t)(L - 1);
-    const StepConsts& C = inst_consts<BATCH>(G, i);
-    const __amdgpu_buffer_rsrc_t rpos = make_rsrc(pos, (uint32_t)G.n * 16u);
-    const __amdgpu_buffer_rsrc_t rcs = make_rsrc(cs, ((uint32_t)G.cs_cells + 1u) * 4u);
-    const float4 p = pos[i];
-    const size_t stride = (size_t)G.n;
-    float acc = 0.0f;
-    uint32_t cnt = 0;
-    const float hs = C.h_search;
-    const float hs2 = hs * hs;
-    const int ylo = grid_coord(p.y - hs, 1, C), yhi = grid_coord(p.y + hs, 1, C);
-    const int zlo = grid_coord(p.z - hs, 2, C), zhi = grid_coord(p.z + hs, 2, C);
-    for (int cz = zlo; cz <= zhi; ++cz) {
-        const float ez = slab_dist(p.z, cz, 2, C);
-        for (int cy = ylo; cy <= yhi; ++cy) {
-            const float ey = slab_dist(p.y, cy, 1, C);
-            const float rem = hs2 - ez * ez - ey * ey;
-            if (rem < 0.0f) continue;
-            const float e = __builtin_amdgcn_sqrtf(rem);
-            const int xlo = grid_coord(p.x - e, 0, C), xhi = grid_coord(p.x + e, 0, C);
-            const uint32_t row = (uint32_t)(C.cell_off + C.dims[0] * (cy + C.dims[1] * cz));
             const uint32_t k0 = ld_u32(rcs, (row + (uint32_t)xlo) * 4u);
             const uint32_t k1 = ld_u32(rcs, (row + (uint32_t)xhi + 1u) * 4u);
-            for (uint32_t base = k0; base < k1; base += L) {
-                const uint32_t k = base + l;
-                const bool valid = k < k1;
-                const float4 q = ld_f4(rpos, (valid ? k : k0) * 16u);
-                const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-                const float r2 = valid ? dx * dx + dy * dy + dz * dz : INFINITY;
-                const float t = fmaxf(1.0f - r2 * C.inv_h2, 0.0f);
-                acc += t * t * t;
-                const bool hit = r2 <= C.h2;
-                const uint32_t gm = (uint32_t)(__ballot(hit) >> gbase) & ((1u << L) - 1u);
-                if (hit) {
-                    const uint32_t slot = cnt + (uint32_t)__popc(gm & ((1u << l) - 1u));
-                    if (slot < (uint32_t)kLdsListCap) s_list[slot][col] = k;
-                    else if (slot < (uint32_t)kListCap) nl[slot * stride + i] = k;
+            if constexpr (L == 1) {
+                uint32_t off = k0 * 16u;
+                const uint32_t end = k1 * 16u;
+                for (; off + 16u * (kDensityUnroll - 1) < end; off += 16u * kDensityUnroll) {
+                    float4 q[kDensityUnroll];
+#pragma unroll
+                    for (int u = 0; u < kDensityUnroll; ++u) q[u] = ld_f4(rpos, off + 16u * u);
+#pragma unroll
+                    for (int u = 0; u < kDensityUnroll; ++u) visit(q[u], (off >> 4) + (uint32_t)u);
                 }
-                cnt += (uint32_t)__popc(gm);
+                for (; off < end; off += 16u) visit(ld_f4(rpos, off), off >> 4);
+            } else {
+                for (uint32_t base = k0; base < k1; base += L) {
+                    const uint32_t k = base + l;
+                    const bool valid = k < k1;
+                    const float4 q = ld_f4(rpos, (valid ? k : k0) * 16u);
+                    const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+                    const float r2 = valid ? dx * dx + dy * dy + dz * dz : INFINITY;
+                    const float t = fmaxf(1.0f - r2 * C.inv_h2, 0.0f);
+                    acc += t * t * t;
+                    const bool hit = r2 <= C.h2;
+                    const uint32_t gm = (uint32_t)(__ballot(hit) >> gbase) & ((1u << L) - 1u);
+                    if (hit) {
+                        const uint32_t slot = cnt + (uint32_t)__popc(gm & ((1u << l) - 1u));
+                        if (slot < (uint32_t)kLdsListCap) s_list[slot][col] = k;
+                        else if (slot < (uint32_t)kListCap) nl[slot * stride + i] = k;
+                    }
+                    cnt += (uint32_t)__popc(gm);
+                }
             }
         }
     }
@@ -252,11 +213,82 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_group_kernel(St
     }
 }
 
+// Plain step: L lanes per particle (1, 2, 4, 8 or 16), particle gt / L.  Groups are aligned,
+// so a whole group leaves together at the end of the range.
+template <int L>
+__global__ __launch_bounds__(kDensityThreads) void naive_density_kernel(StepConsts G, float4* pos, float4* vel,
+                                                                         const uint32_t* __restrict__ cs,
+                                                                         float* __restrict__ rhoA,
+                                                                         uint32_t* __restrict__ nl,
+                                                                         uint32_t* __restrict__ nn) {
+    static_assert(L >= 1 && L <= 16 && (L & (L - 1)) == 0, "lanes per particle: 1, 2, 4, 8 or 16");
+    __shared__ DensityList s_list[kLdsListCap];
+    const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int i = (int)(gt / L);
+    if (i >= G.n) return;
+    density_particle<L>(G, G, i, gt & (L - 1), threadIdx.x / L, (threadIdx.x & 63u) & ~(uint32_t)(L - 1), pos, vel,
+                        cs, rhoA, nl, nn, s_list);
+}
+
+// Batched form: each instance has its own lanes per particle and its own thread range, which
+// starts at a multiple of 64, so a wavefront belongs to one instance and its lane count is
+// uniform.  The instance is found once per wavefront, by a binary search over the ranges'
+// first threads with the wavefront's first thread made scalar.
+struct BatchLane {
+    int b;       // instance
+    int i;       // cell-ordered particle (total index), or -1 past the instance's particles
+    uint32_t l;  // lane within the particle's group
+    int lanes;
+};
+
+__device__ __forceinline__ BatchLane batch_lane(const StepConsts& G, uint32_t gt) {
+    const int w0 = __builtin_amdgcn_readfirstlane((int)(gt & ~63u));
+    int lo = 0, hi = G.n_inst;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (G.inst_table[mid].tfirst <= w0) lo = mid;
+        else hi = mid;
+    }
+    const StepConsts& C = G.inst_table[lo];
+    const uint32_t local = gt - (uint32_t)C.tfirst;
+    const uint32_t p = local >> C.lane_shift;
+    BatchLane r;
+    r.b = lo;
+    r.i = p < (uint32_t)C.n ? C.first + (int)p : -1;
+    r.l = local & (uint32_t)(C.lanes - 1);
+    r.lanes = C.lanes;
+    return r;
+}
+
+__global__ __launch_bounds__(kDensityThreads) void naive_density_batch_kernel(StepConsts G, float4* pos, float4* vel,
+                                                                               const uint32_t* __restrict__ cs,
+                                                                               float* __restrict__ rhoA,
+                                                                               uint32_t* __restrict__ nl,
+                                                                               uint32_t* __restrict__ nn) {
+    __shared__ DensityList s_list[kLdsListCap];
+    const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const BatchLane t = batch_lane(G, gt);
+    if (t.i < 0) return;
+    const StepConsts& C = G.inst_table[t.b];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = threadIdx.x & ~63u;
+    const uint32_t gbase = lane & ~(uint32_t)(t.lanes - 1);
+    const uint32_t col = wave + (lane >> C.lane_shift);
+    switch (t.lanes) {
+        case 1: density_particle<1>(G, C, t.i, 0u, col, gbase, pos, vel, cs, rhoA, nl, nn, s_list); break;
+        case 2: density_particle<2>(G, C, t.i, t.l, col, gbase, pos, vel, cs, rhoA, nl, nn, s_list); break;
+        case 4: density_particle<4>(G, C, t.i, t.l, col, gbase, pos, vel, cs, rhoA, nl, nn, s_list); break;
+        case 8: density_particle<8>(G, C, t.i, t.l, col, gbase, pos, vel, cs, rhoA, nl, nn, s_list); break;
+        default: density_particle<16>(G, C, t.i, t.l, col, gbase, pos, vel, cs, rhoA, nl, nn, s_list); break;
+    }
+}
+
 // Force pass (naive_kernels.py:24-80) fused with integration and collision: each lane
 // walks its particle's neighbour list (one coalesced load per slot, kForceUnroll slots and
 // their gathers in flight); a particle with more neighbours than list slots rescans its
-// candidates instead.  MODE selects the wall collision at compile time.
-template <int MODE, bool BATCH, int L = 1>
+// candidates instead.  With L lanes per particle the group splits the list (or the rescan)
+// and lane 0 finishes the particle.  MODE selects the wall collision at compile time.
+template <int MODE, int L = 1>
 __global__ __launch_bounds__(256) void naive_force_kernel(
     StepConsts G, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
     const float* __restrict__ rhoA, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ nl,
@@ -267,15 +299,44 @@ __global__ __launch_bounds__(256) void naive_force_kernel(
     const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int i = (int)(gt / L);
     if (i >= G.n) return;
-    const StepConsts& C = inst_consts<BATCH>(G, i);
+    const StepConsts& C = G;
     const float4 p = pos[i], v = vel[i];
-    const PairSums S = naive_pair_sums_lanes<BATCH, L>(C, i, gt & (L - 1), G.n, p, v, pos, vel, cs, nl, nn);
+    const PairSums S = naive_pair_sums_lanes<false, L>(C, i, gt & (L - 1), G.n, p, v, pos, vel, cs, nl, nn);
     if (L > 1 && (gt & (L - 1)) != 0) return;
     const float ps = 0.5f * C.mass * C.grad_c;   // naive_kernels.py:40-52: m * (pi+pj)/(2 rho_j) * grad W
     const float vs = C.mass * C.visc * C.lap_c;  // naive_kernels.py:71-77: m * mu * (vj-vi)/rho_j * lap W
     finish_particle<MODE>(C, i, p, v, pidA[i], rhoA[i], Vec3{ps * S.ax, ps * S.ay, ps * S.az},
                           Vec3{vs * S.bx, vs * S.by, vs * S.bz}, posOut, velOut, pidOut, keyI, valI, rng, pipe, dbgP,
                           dbgV, dbgF, flags);
+}
+
+// Batched form of the force pass (box walls): the thread map of naive_density_batch_kernel.
+__global__ __launch_bounds__(256) void naive_force_batch_kernel(
+    StepConsts G, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
+    const float* __restrict__ rhoA, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ nl,
+    const uint32_t* __restrict__ nn, float4* __restrict__ posOut, float4* __restrict__ velOut,
+    uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
+    float4* __restrict__ dbgP, float4* __restrict__ dbgV, float4* __restrict__ dbgF, uint32_t* __restrict__ flags) {
+    const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const BatchLane t = batch_lane(G, gt);
+    if (t.i < 0) return;
+    const StepConsts& C = G.inst_table[t.b];
+    const int i = t.i;
+    const float4 p = pos[i], v = vel[i];
+    PairSums S;
+    switch (t.lanes) {
+        case 1: S = naive_pair_sums_lanes<false, 1>(C, i, 0u, G.n, p, v, pos, vel, cs, nl, nn); break;
+        case 2: S = naive_pair_sums_lanes<false, 2>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
+        case 4: S = naive_pair_sums_lanes<false, 4>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
+        case 8: S = naive_pair_sums_lanes<false, 8>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
+        default: S = naive_pair_sums_lanes<false, 16>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
+    }
+    if (t.l != 0) return;
+    const float ps = 0.5f * C.mass * C.grad_c;
+    const float vs = C.mass * C.visc * C.lap_c;
+    finish_particle<SPH_MODE_BOX>(C, i, p, v, pidA[i], rhoA[i], Vec3{ps * S.ax, ps * S.ay, ps * S.az},
+                                  Vec3{vs * S.bx, vs * S.by, vs * S.bz}, posOut, velOut, pidOut, keyI, valI, nullptr,
+                                  nullptr, dbgP, dbgV, dbgF, flags);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -562,12 +623,7 @@ template <int L>
 void launch_density_lanes(sph_ctx* c, hipStream_t s) {
     const int bs = c->density_block;
     const Buffers& b = c->b;
-    if constexpr (L == 1)
-        (c->sc.n_inst > 1 ? naive_density_kernel<true> : naive_density_kernel<false>)<<<blocks_for(c->n, bs), bs, 0, s>>>(
-            c->sc, b.posA, b.velA, b.cs, b.rhoA, b.nl, b.nn);
-    else
-        (c->sc.n_inst > 1 ? naive_density_group_kernel<true, L> : naive_density_group_kernel<false, L>)<<<blocks_for(c->n * L, bs), bs, 0, s>>>(
-            c->sc, b.posA, b.velA, b.cs, b.rhoA, b.nl, b.nn);
+    naive_density_kernel<L><<<blocks_for(c->n * L, bs), bs, 0, s>>>(c->sc, b.posA, b.velA, b.cs, b.rhoA, b.nl, b.nn);
 }
 
 hipError_t launch_density(sph_ctx* c, hipStream_t s) {
@@ -576,6 +632,12 @@ hipError_t launch_density(sph_ctx* c, hipStream_t s) {
     if (c->sc.formulation == SPH_FORM_VOXEL) {
         voxel_density_kernel<<<blocks_for(c->n, bs), bs, 0, s>>>(c->sc, c->b.posA, c->b.velA, c->b.cs, c->b.rhoA,
                                                                    c->b.nbr);
+        return hipGetLastError();
+    }
+    if (!c->inst.empty()) {
+        const Buffers& b = c->b;
+        naive_density_batch_kernel<<<blocks_for(c->sc.tfirst, bs), bs, 0, s>>>(c->sc, b.posA, b.velA, b.cs, b.rhoA,
+                                                                              b.nl, b.nn);
         return hipGetLastError();
     }
     switch (c->lanes) {
@@ -591,7 +653,7 @@ hipError_t launch_density(sph_ctx* c, hipStream_t s) {
 template <int MODE, int L>
 void launch_force_lanes(sph_ctx* c, hipStream_t s, float4* dP, float4* dV, float4* dF) {
     const int bs = c->force_block;
-    (c->sc.n_inst > 1 ? naive_force_kernel<MODE, true, L> : naive_force_kernel<MODE, false, L>)<<<blocks_for(c->n * L, bs), bs, 0, s>>>(
+    naive_force_kernel<MODE, L><<<blocks_for(c->n * L, bs), bs, 0, s>>>(
         c->sc, c->b.posA, c->b.velA, c->b.pidA, c->b.rhoA, c->b.cs, c->b.nl, c->b.nn, c->b.posB, c->b.velB,
         c->b.pidB, c->b.keyI, c->b.valI, c->b.rng, c->b.pipe, dP, dV, dF, c->b.flags);
 }
@@ -606,6 +668,12 @@ void launch_force_mode(sph_ctx* c, hipStream_t s) {
         voxel_force_kernel<MODE><<<blocks_for(c->n, bs), bs, 0, s>>>(
             c->sc, c->b.posA, c->b.velA, c->b.pidA, c->b.rhoA, c->b.nbr, c->b.posB, c->b.velB, c->b.pidB, c->b.keyI,
             c->b.valI, c->b.rng, c->b.pipe, dP, dV, dF, c->b.flags);
+        return;
+    }
+    if (!c->inst.empty()) {  // batched instances have box walls
+        naive_force_batch_kernel<<<blocks_for(c->sc.tfirst, bs), bs, 0, s>>>(
+            c->sc, c->b.posA, c->b.velA, c->b.pidA, c->b.rhoA, c->b.cs, c->b.nl, c->b.nn, c->b.posB, c->b.velB,
+            c->b.pidB, c->b.keyI, c->b.valI, dP, dV, dF, c->b.flags);
         return;
     }
     switch (c->lanes) {

@@ -59,7 +59,7 @@ EXPORTED = [
     "sph_upload_state", "sph_upload_state_f32", "sph_step", "sph_download_state", "sph_download_state_f32",
     "sph_run_stage", "sph_buffer_elems", "sph_read_buffer", "sph_diagnostics", "sph_grid_info", "sph_set_launch",
     "sph_create_batch", "sph_batch_info", "sph_set_constants", "sph_set_lanes", "sph_adjoint_seed", "sph_adjoint_step",
-    "sph_adjoint_read",
+    "sph_adjoint_read", "sph_set_batch_lanes",
 ]
 
 _lib = None
@@ -97,6 +97,7 @@ def load_library() -> ctypes.CDLL:
         "sph_batch_info": ([vp, POINTER(i32), POINTER(i32)], i32),
         "sph_set_constants": ([vp, POINTER(SphConstantsC)], i32),
         "sph_set_lanes": ([vp, i32], i32),
+        "sph_set_batch_lanes": ([vp, vp, i32, vp], i32),
         "sph_adjoint_seed": ([vp, vp, vp, vp], i32),
         "sph_adjoint_step": ([vp, vp], i32),
         "sph_adjoint_read": ([vp, vp, vp, vp, vp], i32),
@@ -248,6 +249,12 @@ class NativeContext:
 
     def set_lanes(self, lanes: int):
         check(self.lib.sph_set_lanes(self.handle, int(lanes)), "sph_set_lanes")
+
+    def set_batch_lanes(self, lanes):
+        """Lanes per particle of each instance of a batched context."""
+        arr = (ctypes.c_int32 * len(lanes))(*[int(x) for x in lanes])
+        check(self.lib.sph_set_batch_lanes(self.handle, arr, len(lanes), _stream(self.device)),
+              "sph_set_batch_lanes")
 
     def set_constants(self, consts: SphConstantsC):
         check(self.lib.sph_set_constants(self.handle, ctypes.byref(consts)), "sph_set_constants")

@@ -12,6 +12,14 @@ Three knobs decide how the step maps onto the GPU:
     to FP32 summation order), while in the VOXEL formulation the voxel edge is part of the
     reference's semantics (voxel_sph_strategy.py:109-116), so only block sizes are tuned.
 
+The batched form tunes the grid and the lanes per instance: instances differ by orders of
+magnitude in neighbours per particle (a dense one wants 16 lanes and finer cells, a sparse
+one a single lane and coarse cells, whose table the counting sort scans every step).  Each
+instance's neighbour load is estimated from one short run (the density over the particle
+mass is the number density, times the volume of the h-sphere), and the candidates are
+rules over that load: lanes = the load over a divisor (a power of two in 1..16), finer
+cells above a load threshold.
+
 Every candidate is timed on the GPU from the same initial state over the same steps
 (HIP events on the stream the step runs on), the fastest wins, and the result is cached
 in a JSON file keyed by the workload's signature and the device name, so a later run
@@ -25,7 +33,7 @@ import json
 import os
 import time
 from dataclasses import asdict, dataclass, field
-from typing import Dict, Iterable, List, Optional, Sequence, Tuple
+from typing import Dict, Iterable, List, Optional, Sequence, Tuple, Union
 
 import numpy as np
 import torch
@@ -41,14 +49,20 @@ LANES = (1, 2, 4, 8, 16)
 
 @dataclass(frozen=True)
 class TuneResult:
-    cell_scale: Optional[float]   # grid edge = voxel_for(h) * cell_scale; None = the params' voxel_size
+    # grid edge = voxel_for(h) * cell_scale; None = the params' voxel_size.  Batched form: one
+    # value per instance.
+    cell_scale: Union[None, float, Tuple[float, ...]]
     density_block: int
     force_block: int
     ms_per_step: float
-    trials: Tuple[Tuple[Optional[float], int, int, int, float], ...] = field(default=())  # scale, blocks, lanes, ms
+    # (scale, density block, force block, lanes, ms); the batched form records its rules
+    # ("load>=T" for the grid, "load/D" for the lanes) in place of scale and lanes
+    trials: Tuple[tuple, ...] = field(default=())
     seconds: float = 0.0          # time the search took (0 when read from the cache)
     cached: bool = False
-    lanes: int = 1                # lanes per particle of the density and force passes
+    # lanes per particle of the density and force passes (batched form: one per instance)
+    lanes: Union[int, Tuple[int, ...]] = 1
+    rule: Tuple[str, ...] = ()    # batched form: the winning (grid, lanes) rules
 
     @property
     def launch(self) -> Tuple[int, int]:
@@ -64,8 +78,12 @@ class TuneResult:
 
     @staticmethod
     def from_json(d: dict) -> "TuneResult":
-        return TuneResult(d["cell_scale"], int(d["density_block"]), int(d["force_block"]), float(d["ms_per_step"]),
-                          tuple(tuple(t) for t in d.get("trials", ())), 0.0, True, int(d.get("lanes", 1)))
+        scale, lanes = d["cell_scale"], d.get("lanes", 1)
+        scale = tuple(float(x) for x in scale) if isinstance(scale, list) else scale
+        lanes = tuple(int(x) for x in lanes) if isinstance(lanes, list) else int(lanes)
+        return TuneResult(scale, int(d["density_block"]), int(d["force_block"]), float(d["ms_per_step"]),
+                          tuple(tuple(t) for t in d.get("trials", ())), 0.0, True, lanes,
+                          tuple(d.get("rule", ())))
 
 
 def default_cache_path() -> str:
@@ -140,15 +158,17 @@ def _time_steps(step, reset, steps: int, warmup: int) -> float:
     return e0.elapsed_time(e1) / steps
 
 
-def _search(make, scales: Sequence[Optional[float]], blocks: Sequence[int], steps: int, warmup: int,
-            log=None, lanes: Sequence[int] = (1,), finalists: int = 3) -> TuneResult:
-    """make(scale) -> (configure(db, fb, lanes), step(k), reset(), close()) for one grid.
-    First every (grid, lanes) pair is timed with the largest workgroups, then every block
-    pair on the fastest of them; the best `finalists` are timed once more and the fastest of
-    that second timing wins (the first timing of a run carries warm-up effects).  The very
-    first timing is repeated and discarded for the same reason."""
+def _search(make, scales: Sequence, blocks: Sequence[int], steps: int, warmup: int,
+            log=None, lanes: Sequence = (1,), finalists: int = 3, cross: bool = True) -> TuneResult:
+    """make(scale) -> (configure(db, fb, lanes), step(k), reset(), close()) for one grid;
+    scale and lanes are opaque keys.  With cross, every (grid, lanes) pair is timed with the
+    largest workgroups first; otherwise every grid with the middle lanes key, then every
+    lanes key on the fastest grid.  Then every block pair is timed on the fastest of those;
+    the best `finalists` are timed once more and the fastest of that second timing wins (the
+    first timing of a run carries warm-up effects).  The very first timing is repeated and
+    discarded for the same reason."""
     t0 = time.perf_counter()
-    trials: List[Tuple[Optional[float], int, int, int, float]] = []
+    trials: List[tuple] = []
     warmed = [False]
 
     def run(scale, pairs, record):
@@ -181,8 +201,16 @@ def _search(make, scales: Sequence[Optional[float]], blocks: Sequence[int], step
         return out
 
     top = max(blocks)
-    for scale in scales:
-        run(scale, [(top, top, ln) for ln in lanes], True)
+    if cross:
+        for scale in scales:
+            run(scale, [(top, top, ln) for ln in lanes], True)
+    else:
+        mid = lanes[len(lanes) // 2]
+        for scale in scales:
+            run(scale, [(top, top, mid)], True)
+        if trials:
+            s_best = min(trials, key=lambda t: t[4])[0]
+            run(s_best, [(top, top, ln) for ln in lanes if ln != mid], True)
     if not trials:
         raise RuntimeError("autotune: no candidate could be planned")
     s_best, _, _, l_best, _ = min(trials, key=lambda t: t[4])
@@ -228,12 +256,56 @@ def tune_strategy(params: SimulationParameters, constants: SPHConstants, state: 
     return res
 
 
+# Batched-form rules over the neighbour load (mean neighbours within h per particle).
+LOAD_THRESHOLDS = (0.0, 128.0, 512.0, 2048.0, float("inf"))  # finer cells from this load on
+LOAD_DIVISORS = (4.0, 8.0, 16.0, 32.0, 64.0, 128.0)        # lanes = load / divisor
+BATCH_SCALES = (1.0, 0.5)                                   # (coarse, fine) grid scales
+
+
+def neighbour_load(instances: Sequence[Tuple[SimulationParameters, SPHConstants]],
+                   states: Sequence[SimulationState], steps: int = 3, device="cuda:0") -> np.ndarray:
+    """Mean neighbours within h per particle of each instance after `steps` batched steps,
+    from the density: rho / m is the number density of the poly6 estimate, and a particle's
+    neighbours are that times the volume of the sphere of radius h (itself included)."""
+    from .batched import BatchedSPH
+
+    b = BatchedSPH(instances, device=device)
+    try:
+        b.upload(states)
+        b.step(steps)
+        out = b.download()
+    finally:
+        b.ctx.close()
+    load = []
+    for st, (_, c) in zip(out, instances):
+        rho = np.asarray(st.density, dtype=np.float64)
+        rho = rho[np.isfinite(rho)]
+        load.append(float(rho.mean()) / c.mass * (4.0 / 3.0) * np.pi * c.h ** 3 if rho.size else 0.0)
+    return np.asarray(load)
+
+
+def lanes_for_load(load: np.ndarray, divisor: float) -> Tuple[int, ...]:
+    """Largest power of two in 1..16 not above load / divisor, per instance."""
+    out = []
+    for x in np.asarray(load, dtype=np.float64):
+        q = x / divisor if np.isfinite(x) else 1.0
+        out.append(int(min(16, 2 ** int(np.floor(np.log2(q))))) if q >= 1.0 else 1)
+    return tuple(out)
+
+
+def scales_for_load(load: np.ndarray, threshold: float, coarse: float = BATCH_SCALES[0],
+                    fine: float = BATCH_SCALES[1]) -> Tuple[float, ...]:
+    return tuple(fine if x >= threshold else coarse for x in np.asarray(load, dtype=np.float64))
+
+
 def tune_batch(instances: Sequence[Tuple[SimulationParameters, SPHConstants]], states: Sequence[SimulationState],
-               steps: int = 20, warmup: int = 3, scales: Sequence[float] = CELL_SCALES[:3],
-               blocks: Sequence[int] = BLOCKS, lanes: Sequence[int] = LANES, cache: Optional[TuneCache] = None,
-               device="cuda:0", log=None) -> TuneResult:
-    """Tunes the batched form: one cell scale for all instances (each instance's grid edge
-    is voxel_for(h_b) times it) and one block pair for the shared launches."""
+               steps: int = 20, warmup: int = 3, thresholds: Sequence[float] = LOAD_THRESHOLDS,
+               divisors: Sequence[float] = LOAD_DIVISORS, blocks: Sequence[int] = BLOCKS,
+               cache: Optional[TuneCache] = None, device="cuda:0", log=None) -> TuneResult:
+    """Tunes the batched form: per instance a grid scale and a lane count, both set by rules
+    over the instance's neighbour load (finer cells from a load threshold on, lanes = load /
+    divisor), and one block pair for the shared launches.  Thresholds are searched first (at
+    the middle divisor), then divisors, then block pairs."""
     from .batched import BatchedSPH
 
     key = signature("batch", [(p, c, s) for (p, c), s in zip(instances, states)], "BOX", "NAIVE", steps, device)
@@ -241,18 +313,27 @@ def tune_batch(instances: Sequence[Tuple[SimulationParameters, SPHConstants]], s
         hit = cache.get(key)
         if hit is not None:
             return hit
+    t0 = time.perf_counter()
+    load = neighbour_load(instances, states, device=device)
+    if log:
+        log(f"autotune: neighbour load per instance: min {load.min():.1f}, median {np.median(load):.1f}, "
+            f"max {load.max():.1f}")
+    scale_rule = {f"load>={t:g}": scales_for_load(load, t) for t in thresholds}
+    lane_rule = {f"load/{d:g}": lanes_for_load(load, d) for d in divisors}
 
-    def make(scale):
-        b = BatchedSPH(instances, device=device, cell_scale=scale)
+    def make(rule):
+        b = BatchedSPH(instances, device=device, cell_scale=scale_rule[rule])
         b.upload(states)
         pos, vel = b.ctx_state_tensors()
 
         def configure(db, fb, ln):
             b.ctx.set_launch(db, fb)
-            b.ctx.set_lanes(ln)
+            b.ctx.set_batch_lanes(lane_rule[ln])
         return configure, b.step, lambda: b.ctx.upload(pos, vel), b.ctx.close
 
-    res = _search(make, list(scales), blocks, steps, warmup, log, lanes)
+    res = _search(make, list(scale_rule), blocks, steps, warmup, log, list(lane_rule), cross=False)
+    res = TuneResult(scale_rule[res.cell_scale], res.density_block, res.force_block, res.ms_per_step, res.trials,
+                     time.perf_counter() - t0, False, lane_rule[res.lanes], (res.cell_scale, res.lanes))
     if cache is not None:
         cache.put(key, res)
     return res

@@ -9,7 +9,7 @@ same neighbour lists, same arithmetic).
 """
 from __future__ import annotations
 
-from typing import List, Optional, Sequence, Tuple
+from typing import List, Optional, Sequence, Tuple, Union
 
 import numpy as np
 import torch
@@ -20,20 +20,37 @@ from .data_classes import SimulationParameters, SimulationState
 from .workloads import voxel_for
 
 
+def _per_instance(value, count: int, what: str) -> list:
+    """One value per instance from a scalar or a sequence of length count."""
+    if value is None or np.isscalar(value):
+        return [value] * count
+    out = list(value)
+    if len(out) != count:
+        raise ValueError(f"{len(out)} values of {what} for {count} instances")
+    return out
+
+
 class BatchedSPH:
     def __init__(self, instances: Sequence[Tuple[SimulationParameters, SPHConstants]], device="cuda:0",
-                 launch: Optional[Tuple[int, int]] = None, cell_scale: Optional[float] = None, lanes: int = 1):
-        """launch: (density, force) threads per workgroup.  cell_scale: when given, instance
-        b searches a grid of edge voxel_for(h_b) * cell_scale instead of its voxel_size
-        (the NAIVE sums do not depend on the grid; see autotune.py)."""
+                 launch: Optional[Tuple[int, int]] = None,
+                 cell_scale: Optional[Union[float, Sequence[float]]] = None,
+                 lanes: Union[int, Sequence[int]] = 1):
+        """launch: (density, force) threads per workgroup.  cell_scale: when given (one value,
+        or one per instance), instance b searches a grid of edge voxel_for(h_b) * cell_scale_b
+        instead of its voxel_size (the NAIVE sums do not depend on the grid; see autotune.py).
+        lanes: lanes of a wavefront per particle in the density and force passes, one value or
+        one per instance (1, 2, 4, 8 or 16)."""
         self.instances = list(instances)
-        self.cell_scale = cell_scale
+        count = len(self.instances)
+        self.cell_scale = _per_instance(cell_scale, count, "cell_scale")
+        self.lanes = _per_instance(lanes, count, "lanes")
 
-        def grid(p, c):
-            return p.voxel_size if cell_scale is None else [voxel_for(c.h) * cell_scale] * 3
+        def grid(b, p, c):
+            sc = self.cell_scale[b]
+            return p.voxel_size if sc is None else [voxel_for(c.h) * sc] * 3
 
-        params = [N.make_params(p.particle_count, p.external_force, 1.0 / p.fps, p.space_size, grid(p, c),
-                                N.SPH_MODE_BOX, N.SPH_FORM_NAIVE) for p, c in self.instances]
+        params = [N.make_params(p.particle_count, p.external_force, 1.0 / p.fps, p.space_size, grid(b, p, c),
+                                N.SPH_MODE_BOX, N.SPH_FORM_NAIVE) for b, (p, c) in enumerate(self.instances)]
         consts = [N.make_consts(c) for _, c in self.instances]
         self.ctx = N.NativeContext(params, consts, device)
         self.device = self.ctx.device
@@ -41,8 +58,8 @@ class BatchedSPH:
         self.n = self.ctx.n
         if launch:
             self.ctx.set_launch(*launch)
-        if lanes != 1:
-            self.ctx.set_lanes(lanes)
+        if any(L != 1 for L in self.lanes):
+            self.ctx.set_batch_lanes(self.lanes)
         self._graphs = {}
 
     # -- state ---------------------------------------------------------------------------

@@ -203,8 +203,16 @@ int sph_set_launch(sph_ctx* ctx, int32_t density_block, int32_t force_block);
  * (1, 2, 4, 8 or 16; default 1), for the autotuner: small or dense problems (few particles,
  * many neighbours each) get that many times the parallelism.  The neighbour lists are the
  * same for every value; sums are added in a different order (FP32 rounding only).  The
- * reference launches one thread per particle (thread_layout.py:27-33). */
+ * reference launches one thread per particle (thread_layout.py:27-33).  On a batched
+ * context every instance gets `lanes`. */
 int sph_set_lanes(sph_ctx* ctx, int32_t lanes);
+
+/* Batched context: lanes per particle of each instance (count = the number of instances, each
+ * 1, 2, 4, 8 or 16), so that instances with many neighbours per particle get more lanes and
+ * sparse ones fewer.  Each instance computes what a plain context with its lane count
+ * computes.  The instance table is re-uploaded on `stream` (synchronised) when the workspace
+ * is bound.  No counterpart in the reference, which advances one simulation per process. */
+int sph_set_batch_lanes(sph_ctx* ctx, const int32_t* lanes, int32_t count, void* stream);
 
 /* Replaces the physics constants of a plain context between steps (the reference freezes
  * them at JIT time, config.py:18-23).  mass, rho0, visc, k and damp may change; h must be

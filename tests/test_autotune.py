@@ -78,3 +78,50 @@ def test_grid_voxel_is_power_of_two():
     g = res.grid_voxel(0.0457)
     assert g == 2.0 ** -6 and np.log2(g) == int(np.log2(g))
     assert T.TuneResult(None, 64, 64, 1.0).grid_voxel(0.0457) is None
+
+
+def test_batch_result_round_trips_per_instance_values(tmp_path):
+    cache = T.TuneCache(str(tmp_path / "tune.json"))
+    res = T.TuneResult((1.0, 0.5, 1.0), 256, 64, 0.5, (("load>=512", 256, 256, "load/16", 0.6),), 2.0, False,
+                       (1, 16, 2), ("load>=512", "load/16"))
+    cache.put("b", res)
+    got = cache.get("b")
+    assert got.cell_scale == (1.0, 0.5, 1.0) and got.lanes == (1, 16, 2)
+    assert got.rule == ("load>=512", "load/16") and got.trials == res.trials
+
+
+def test_load_rules():
+    load = np.array([0.5, 3.9, 4.0, 31.9, 64.0, 2627.0, np.nan])
+    # largest power of two in 1..16 not above load / divisor; below 1 (or unknown) one lane
+    assert T.lanes_for_load(load, 4.0) == (1, 1, 1, 4, 16, 16, 1)
+    assert T.lanes_for_load(load, 16.0) == (1, 1, 1, 1, 4, 16, 1)
+    assert T.lanes_for_load(load, 128.0) == (1, 1, 1, 1, 1, 16, 1)
+    assert T.scales_for_load(load, 64.0) == (1.0, 1.0, 1.0, 1.0, 0.5, 0.5, 1.0)
+    assert T.scales_for_load(load, 0.0)[:6] == (0.5,) * 6
+    assert T.scales_for_load(load, float("inf")) == (1.0,) * 7
+
+
+def test_staged_search_times_grids_then_lanes(monkeypatch):
+    """cross=False: every grid with the middle lanes key, then the other lanes keys on the
+    fastest grid, then block pairs, then finalists."""
+    cost = {("g0", "l0"): 3.0, ("g0", "l1"): 2.0, ("g0", "l2"): 2.5,
+            ("g1", "l0"): 1.5, ("g1", "l1"): 1.2, ("g1", "l2"): 1.0}
+    seen = []
+    # drive _search through a make() whose timing is looked up from the configured key
+    cfg = {}
+
+    def make2(g):
+        def configure(db, fb, ln):
+            cfg["k"] = (g, ln, db, fb)
+        return configure, lambda k: None, lambda: None, lambda: None
+
+    def time_cfg(step, reset, steps, warmup):
+        g, ln, db, fb = cfg["k"]
+        seen.append((g, ln, db, fb))
+        return cost[(g, ln)] + (0.0 if (db, fb) == (64, 64) else 0.1)
+
+    monkeypatch.setattr(T, "_time_steps", time_cfg)
+    res = T._search(make2, ["g0", "g1"], (64, 256), 5, 1, None, ["l0", "l1", "l2"], cross=False)
+    first = seen[1:4]  # seen[0] is the discarded warm-up timing
+    assert [(g, ln) for g, ln, _, _ in first] == [("g0", "l1"), ("g1", "l1"), ("g1", "l0")]
+    assert (res.cell_scale, res.lanes, res.launch) == ("g1", "l2", (64, 64))

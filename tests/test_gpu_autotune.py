@@ -21,7 +21,9 @@ def test_tune_strategy_caches_and_keeps_parity(gpu, tmp_path):
     res = T.tune_strategy(w.params, w.constants, w.state, steps=4, warmup=1, scales=(1.0, 0.5, 0.25),
                           blocks=(64, 256), lanes=(1, 4), cache=cache, device=gpu)
     assert len(res.trials) == 3 * 2 + 3 and not res.cached
-    assert res.ms_per_step == min(t[4] for t in res.trials)
+    # the winner is one of the three fastest trials, re-timed
+    finalists = sorted(res.trials, key=lambda t: t[4])[:3]
+    assert (res.cell_scale, res.density_block, res.force_block, res.lanes) in [t[:4] for t in finalists]
     again = T.tune_strategy(w.params, w.constants, w.state, steps=4, warmup=1, scales=(1.0, 0.5, 0.25),
                             blocks=(64, 256), lanes=(1, 4), cache=cache, device=gpu)
     assert again.cached and again.launch == res.launch and again.cell_scale == res.cell_scale
@@ -43,18 +45,24 @@ def test_tune_batch_and_bitwise_plain_equivalence(gpu, tmp_path):
     rng = np.random.default_rng(53)
     ws = [box_uniform(int(rng.integers(800, 3000)), seed=60 + b, h=float(rng.uniform(0.03, 0.06)),
                       edge=float(rng.uniform(0.5, 2.0))) for b in range(4)]
+    ws.append(box_uniform(5000, seed=64, h=0.058, edge=0.6))  # hundreds of neighbours each
     inst = [(w.params, w.constants) for w in ws]
-    res = T.tune_batch(inst, [w.state for w in ws], steps=3, warmup=1, blocks=(64, 256), lanes=(1, 8),
-                       cache=T.TuneCache(str(tmp_path / "t.json")), device=gpu)
-    assert len(res.trials) == 3 * 2 + 3
+    load = T.neighbour_load(inst, [w.state for w in ws], device=gpu)
+    assert load.shape == (5,) and load[4] > 100 and np.all(load > 0)
+    res = T.tune_batch(inst, [w.state for w in ws], steps=3, warmup=1, thresholds=(0.0, 64.0, float("inf")),
+                       divisors=(4.0, 16.0), blocks=(64, 256), cache=T.TuneCache(str(tmp_path / "t.json")),
+                       device=gpu)
+    # 3 grid rules, 1 more lanes rule, 3 more block pairs
+    assert len(res.trials) == 3 + 1 + 3
+    assert len(res.cell_scale) == 5 and len(res.lanes) == 5 and len(res.rule) == 2
     batch = BatchedSPH(inst, device=gpu, cell_scale=res.cell_scale, launch=res.launch, lanes=res.lanes)
     batch.upload([w.state for w in ws])
     batch.step(5)
     got = batch.download()
     for b, w in enumerate(ws):
         plain = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu,
-                                 grid_voxel=voxel_for(w.constants.h) * res.cell_scale, launch=res.launch,
-                                 lanes=res.lanes)
+                                 grid_voxel=voxel_for(w.constants.h) * res.cell_scale[b], launch=res.launch,
+                                 lanes=res.lanes[b])
         ref = plain.advance(w.state, 5)
         assert np.array_equal(got[b].position, ref.position), b
         assert np.array_equal(got[b].velocity, ref.velocity), b

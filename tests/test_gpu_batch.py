@@ -12,7 +12,7 @@ import torch
 from sph_amd.batched import BatchedSPH
 from sph_amd.data_classes import SimulationState
 from sph_amd.strategy import NaiveSPHStrategy
-from sph_amd.workloads import box_uniform
+from sph_amd.workloads import box_uniform, voxel_for
 
 pytestmark = pytest.mark.gpu
 
@@ -46,6 +46,60 @@ def test_batch_equals_plain_per_instance(gpu, lanes):
         assert np.array_equal(got[b].position, ref.position), b
         assert np.array_equal(got[b].velocity, ref.velocity), b
         assert np.array_equal(got[b].density, ref.density), b
+
+
+def test_batch_mixed_lanes_and_grids_equal_plain(gpu):
+    """Each instance with its own lane count and grid: the batch's thread ranges start on
+    wavefront boundaries, so an instance computes what the plain step with its lanes and grid
+    computes.  One dense instance (hundreds of neighbours, past the 96-slot list) takes the
+    rescan path with 16 lanes."""
+    ws = _instances(seed=4, count=7)
+    dense = box_uniform(6000, seed=77, h=0.058, edge=0.6, visc=3.0)
+    ws.append(dense)
+    lanes = [1, 2, 4, 8, 16, 1, 2, 16]
+    scales = [1.0, 0.5, 1.0, 0.5, 1.0, 0.5, 1.0, 0.5]
+    batch = BatchedSPH([(w.params, w.constants) for w in ws], device=gpu, cell_scale=scales, lanes=lanes)
+    batch.upload([w.state for w in ws])
+    batch.step(6)
+    got = batch.download()
+    for b, w in enumerate(ws):
+        plain = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu,
+                                 grid_voxel=voxel_for(w.constants.h) * scales[b], lanes=lanes[b])
+        ref = plain.advance(w.state, 6)
+        assert np.array_equal(got[b].position, ref.position), b
+        assert np.array_equal(got[b].velocity, ref.velocity), b
+        assert np.array_equal(got[b].density, ref.density), b
+    # lanes changed on a bound batch take effect from the next step, and the step is then the
+    # plain step with the new lanes
+    new = [16, 1, 8, 1, 2, 4, 1, 4]
+    batch.ctx.set_batch_lanes(new)
+    batch.upload([w.state for w in ws])
+    batch.step(3)
+    got = batch.download()
+    for b, w in enumerate(ws):
+        plain = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu,
+                                 grid_voxel=voxel_for(w.constants.h) * scales[b], lanes=new[b])
+        ref = plain.advance(w.state, 3)
+        assert np.array_equal(got[b].position, ref.position), b
+
+
+def test_set_batch_lanes_rejects(gpu):
+    ws = _instances(seed=5, count=3)
+    batch = BatchedSPH([(w.params, w.constants) for w in ws], device=gpu)
+    with pytest.raises(RuntimeError, match="2 lane counts for 3"):
+        batch.ctx.set_batch_lanes([1, 2])
+    with pytest.raises(RuntimeError, match="1, 2, 4, 8 or 16"):
+        batch.ctx.set_batch_lanes([1, 3, 1])
+    plain = NaiveSPHStrategy(ws[0].params, constants=ws[0].constants, device=gpu)
+    with pytest.raises(RuntimeError, match="not a batched context"):
+        plain.ctx.set_batch_lanes([1])
+    # a rejected call leaves the previous lanes in place
+    batch.upload([w.state for w in ws])
+    batch.step(2)
+    got = batch.download()
+    for b, w in enumerate(ws):
+        ref = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu).advance(w.state, 2)
+        assert np.array_equal(got[b].position, ref.position), b
 
 
 def test_batch_graph_replays_match_eager(gpu):
