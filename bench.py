@@ -3,9 +3,11 @@
 
 Workload: the 1,048,576-particle dam break (64 x 128 x 128 jittered lattice in a
 [0,2]x[0,1]x[0,1] box, h = 0.01298, NAIVE formulation, box walls), state resident in HBM.
-A step is one full advance: voxel hash, radix sort, cell table, density, force,
-integration, collision.  W warm-up steps, then K timed steps bracketed by device
-synchronisation and HIP events on the stream the kernels run on.
+A step is one full advance: voxel hash, counting sort by cell (which builds the cell table
+and gathers the state into cell order), density, force, integration, collision.  W
+warm-up steps, then K timed steps (launched one by one, or replayed as one HIP graph with
+--graph on) bracketed by device synchronisation and HIP events on the stream the kernels
+run on.
 
 Also measured in the same run:
   tuning        the autotuner's choice of grid cell edge and block sizes (before timing)
@@ -86,6 +88,9 @@ def parse() -> argparse.Namespace:
                     help="also measure frame output to disk in the reference's format")
     ap.add_argument("--pipe", choices=["on", "off"], default="on",
                     help="also measure the pipe workload (524,288 particles, 1000 steps)")
+    ap.add_argument("--graph", choices=["on", "off"], default="off",
+                    help="replay the timed steps of the plain and pipe legs as one HIP graph (default: launch "
+                         "them one by one, which measures the same on these workloads)")
     ap.add_argument("--adjoint", choices=["on", "off"], default="on",
                     help="also measure the differentiable form (16,384 particles, 200 steps, forward + backward)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -303,9 +308,10 @@ def bench_small(warmup: int = 3, tune: bool = True):
             "unit": "particle-updates/s", "ms_per_step": round(ms / w.steps, 5), "graph": True, "tuning": tuning}
 
 
-def bench_pipe(steps: int = 1000, warmup: int = 5, tune: bool = True):
+def bench_pipe(steps: int = 1000, warmup: int = 5, tune: bool = True, graph: bool = True):
     """The pipe workload (pipe_512k: 8 segments, inflow, re-injection through the seeded
-    xoroshiro128+ streams): particle-updates/s over its 1000 steps."""
+    xoroshiro128+ streams): particle-updates/s over its 1000 steps (one HIP graph with
+    graph)."""
     import numpy as np
     import torch
 
@@ -316,13 +322,15 @@ def bench_pipe(steps: int = 1000, warmup: int = 5, tune: bool = True):
     n = w.params.particle_count
     params, launch, lanes, tuning = tune_workload(w, "PIPE", enabled=tune)
     strat = NaiveSPHStrategy(params, constants=w.constants, mode="PIPE", launch=launch, lanes=lanes)
-    strat.ctx.upload(torch.from_numpy(np.ascontiguousarray(w.state.position)).cuda(),
-                     torch.from_numpy(np.ascontiguousarray(w.state.velocity)).cuda())
+    pos = torch.from_numpy(np.ascontiguousarray(w.state.position)).cuda()
+    vel = torch.from_numpy(np.ascontiguousarray(w.state.velocity)).cuda()
+    run = timed_steps(strat, steps, graph, pos, vel)
+    strat.ctx.upload(pos, vel)
     strat.ctx.step(warmup)
     torch.cuda.synchronize()
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     a.record()
-    strat.ctx.step(steps)
+    run()
     b.record()
     torch.cuda.synchronize()
     ms = a.elapsed_time(b)
@@ -330,16 +338,26 @@ def bench_pipe(steps: int = 1000, warmup: int = 5, tune: bool = True):
     log(f"pipe: {steps} steps of {n} particles in {ms:.1f} ms")
     return {"workload": "pipe_512k", "particles": n, "steps": steps, "value": round(n * steps / (ms / 1e3), 1),
             "unit": "particle-updates/s", "ms_per_step": round(ms / steps, 5), "h": w.constants.h,
-            "non_finite": d["non_finite"], "tuning": tuning}
+            "non_finite": d["non_finite"], "graph": graph, "tuning": tuning}
+
+
+def timed_steps(strat, steps: int, graph: bool, pos, vel):
+    """A callable that runs `steps` steps of the resident state: one HIP graph replay (the
+    graph is recorded here from the uploaded state, not run) or direct launches."""
+    if not graph:
+        return lambda: strat.ctx.step(steps)
+    strat.ctx.upload(pos, vel)
+    g = strat.capture(steps)
+    return g.replay
 
 
 def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
     """Aggregate particle-updates/s of the batch_64 workload: one launch per stage for all 64
     instances, the timed steps replayed as one HIP graph; against the same 64 simulations
-    advanced one after another with the plain step (one context each, its steps replayed as
-    one HIP graph, the best of the default, the batch's and the instance's own autotuned
-    configuration).  With tune, each instance's grid scale and lanes per particle and the
-    shared block sizes come from the autotuner."""
+    advanced one after another with the plain step (one context each, the best of the
+    default, the batch's and the instance's own autotuned configuration, each launched step
+    by step and as one HIP graph).  With tune, each instance's grid scale and lanes per
+    particle and the shared block sizes come from the autotuner."""
     import numpy as np
     import torch
 
@@ -359,7 +377,9 @@ def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
         from sph_amd.autotune import tune_batch
 
         log("autotuning the batch")
-        res = tune_batch(inst, states, log=log)
+        # candidates are timed over the whole run: its cost falls several-fold as the
+        # instances' dense starting blocks spread, so a short window misjudges the rules
+        res = tune_batch(inst, states, steps=steps, log=log)
         scale, launch, lanes = list(res.cell_scale), res.launch, list(res.lanes)
         tuning = {"rule": list(res.rule), "density_block": res.density_block, "force_block": res.force_block,
                   "instances_per_lanes": {str(k): v for k, v in sorted(Counter(lanes).items())},
@@ -380,10 +400,10 @@ def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
     torch.cuda.synchronize()
     t_batch = e0.elapsed_time(e1) / 1e3
     log(f"batched: {steps} steps of {total} particles in {t_batch * 1e3:.1f} ms")
-    # the baseline gets its best case: each instance alone, its steps replayed as one HIP
-    # graph of the plain step, with the plain step's default grid and blocks, with the batch's
-    # choices for that instance and (with tune) with the instance's own autotuned plain
-    # configuration; the fastest counted
+    # the baseline gets its best case: each instance alone, with the plain step's default grid
+    # and blocks, with the batch's choices for that instance and (with tune) with the
+    # instance's own autotuned plain configuration, its steps launched one by one and replayed
+    # as one HIP graph; the fastest counted
     t_seq = 0.0
     per_instance = []
     for b, w in enumerate(bw.instances):
@@ -399,19 +419,20 @@ def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
         for sc, la, ln in dict.fromkeys(configs):
             gv = None if sc is None else workloads.voxel_for(w.constants.h) * sc
             strat = NaiveSPHStrategy(w.params, constants=w.constants, grid_voxel=gv, launch=la, lanes=ln)
-            strat.ctx.upload(pos, vel)
-            g = strat.capture(steps)  # recorded, not run
-            strat.ctx.upload(pos, vel)
-            strat.ctx.step(warmup)
-            torch.cuda.synchronize()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            g.replay()
-            e1.record()
-            torch.cuda.synchronize()
-            t = e0.elapsed_time(e1) / 1e3
-            best = t if best is None else min(best, t)
-            del g, strat
+            for graph in (False, True):
+                run = timed_steps(strat, steps, graph, pos, vel)
+                strat.ctx.upload(pos, vel)
+                strat.ctx.step(warmup)
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                run()
+                e1.record()
+                torch.cuda.synchronize()
+                t = e0.elapsed_time(e1) / 1e3
+                best = t if best is None else min(best, t)
+                del run
+            del strat
         t_seq += best
         per_instance.append((best * 1e3 / steps, w.params.particle_count, w.name))
     log(f"sequential plain runs: {t_seq * 1e3:.1f} ms")
@@ -500,6 +521,8 @@ def main() -> int:
     strat = NaiveSPHStrategy(w.params, constants=w.constants, mode=w.mode, device=dev, launch=launch, lanes=lanes)
     pos = torch.from_numpy(np.ascontiguousarray(w.state.position)).to(dev)
     vel = torch.from_numpy(np.ascontiguousarray(w.state.velocity)).to(dev)
+    graph = args.graph == "on"
+    run = timed_steps(strat, args.steps, graph, pos, vel)
     strat.ctx.upload(pos, vel)
     log(f"warmup {args.warmup} steps")
     strat.ctx.step(args.warmup)
@@ -510,7 +533,7 @@ def main() -> int:
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e0.record(stream)
-    strat.ctx.step(args.steps)
+    run()
     e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -573,7 +596,7 @@ def main() -> int:
         strat.ctx.step(args.warmup)
         torch.cuda.synchronize()
         e0.record(stream)
-        strat.ctx.step(args.steps)
+        run()
         e1.record(stream)
         torch.cuda.synchronize()
         repeats.append(round(e0.elapsed_time(e1) / max(args.steps, 1), 5))
@@ -587,7 +610,7 @@ def main() -> int:
         torch.cuda.synchronize()
         log("copy calibration: 5 copies of 256 MiB")
 
-    pipe = bench_pipe(tune=args.tune == "on") if args.pipe == "on" else None
+    pipe = bench_pipe(tune=args.tune == "on", graph=graph) if args.pipe == "on" else None
     frames = bench_frames() if args.frames == "on" else None
     small = bench_small(tune=args.tune == "on")
     batched = bench_batched(args.batch_steps, tune=args.tune == "on") if args.batched == "on" else None
@@ -604,7 +627,7 @@ def main() -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": args.workload, "particles": n, "form": "plain", "formulation": "NAIVE",
                    "mode": w.mode, "h": w.constants.h, "voxel": [float(v) for v in np.asarray(w.params.voxel_size)],
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "graph": graph},
         "roofline": roofline,
         "roofline_force": roofline_force,
         "kernels": kernels,

@@ -6,9 +6,11 @@
 // each cell.  Per step:
 //   count    one thread per particle id: rank = atomicAdd(&cnt[key], 1) (an arbitrary order
 //            among the particles of a cell)
-//   scan     exclusive prefix sum of cnt over all cells in three launches (tile sums, a
-//            one-workgroup scan of the tile sums, tile scans); the result is the cell start
-//            table cs[0..C] itself, and the tile scan resets cnt to zero for the next step
+//   scan     exclusive prefix sum of cnt over all cells: tile sums (16-byte loads), then each
+//            tile's offset is the sum of the tile sums before it (added up by the tile's own
+//            workgroup when there are few tiles, else by a one-workgroup scan first), then
+//            the tile is scanned; the result is the cell start table cs[0..C] itself, and the
+//            tile scan resets cnt to zero for the next step
 //   scatter  one thread per particle id: tmp[cs[key] + rank] = id
 //   order    one thread per slot: the id there moves to cs[key] + (number of ids of its cell
 //            that are smaller), which is id order within the cell; the sorted key and slot
@@ -24,29 +26,44 @@ constexpr int CT = 256;          // threads per workgroup
 constexpr int CI = 16;           // cells per thread in the scan
 constexpr int CTILE = CT * CI;   // cells per scan tile
 constexpr int CWAVES = CT / 64;
+constexpr int kMergedTiles = 2048;  // up to this many tiles, each tile's workgroup sums the counts before it
 
 // Consecutive particle ids are often in the same cell, so a wavefront's lanes come in runs of
 // equal keys: the first lane of each run adds the run's length with one atomic and the run's
 // lanes take consecutive ranks from the returned base.
+struct Run {
+    bool head;  // first lane of its run
+    int first;  // first lane of this lane's run
+    int next;   // first lane of the next run, or one past the last valid lane
+};
+
+// Runs of equal values among the valid lanes (a prefix of the wavefront).
+__device__ __forceinline__ Run lane_run(uint32_t x, bool valid) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t prev = __shfl_up(x, 1);
+    Run r;
+    r.head = valid && (lane == 0 || prev != x);
+    const uint64_t heads = __ballot(r.head);
+    const uint64_t valids = __ballot(valid);
+    const uint64_t upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    const uint64_t below = heads & upto;
+    r.first = 63 - __clzll((long long)below);
+    const uint64_t above = heads & ~upto;
+    r.next = above ? __ffsll((unsigned long long)above) - 1 : 64 - __clzll((long long)valids);
+    return r;
+}
+
 __global__ __launch_bounds__(CT) void cell_count_kernel(int n, const uint32_t* __restrict__ keyI,
                                                          uint32_t* __restrict__ cnt, uint32_t* __restrict__ rank) {
     const int i = blockIdx.x * CT + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool valid = i < n;
     const uint32_t key = valid ? keyI[i] : 0xffffffffu;
-    const uint32_t prev = __shfl_up(key, 1);
-    const bool head = valid && (lane == 0 || prev != key);
-    const uint64_t heads = __ballot(head);
-    const uint64_t valids = __ballot(valid);
-    // first lane of this lane's run, and the first lane of the next run (or the end)
-    const uint64_t below = heads & (lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull));
-    const int first = 63 - __clzll((long long)below);
-    const uint64_t above = heads & ~(lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull));
-    const int next = above ? __ffsll((unsigned long long)above) - 1 : 64 - __clzll((long long)valids);
+    const Run r = lane_run(key, valid);
     uint32_t base = 0;
-    if (head) base = atomicAdd(&cnt[key], (uint32_t)(next - lane));
-    base = __shfl(base, first);
-    if (valid) rank[i] = base + (uint32_t)(lane - first);
+    if (r.head) base = atomicAdd(&cnt[key], (uint32_t)(r.next - lane));
+    base = __shfl(base, r.first);
+    if (valid) rank[i] = base + (uint32_t)(lane - r.first);
 }
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
@@ -77,15 +94,25 @@ __device__ __forceinline__ uint32_t block_excl(uint32_t v, uint32_t* lw, uint32_
     return off + x - v;
 }
 
+constexpr int CV = CI / 4;  // 16-byte vectors per thread and tile
+
 __global__ __launch_bounds__(CT) void cell_scan_reduce(const uint32_t* __restrict__ cnt, int m,
                                                         uint32_t* __restrict__ part) {
     __shared__ uint32_t lw[CWAVES];
     const int64_t base = (int64_t)blockIdx.x * CTILE;
     uint32_t s = 0;
+    if (base + CTILE <= m) {  // base is a multiple of 4 cells: 16-byte aligned
+        const uint4* c4 = reinterpret_cast<const uint4*>(cnt + base);
 #pragma unroll
-    for (int k = 0; k < CI; ++k) {
-        const int64_t c = base + k * CT + threadIdx.x;
-        if (c < m) s += cnt[c];
+        for (int k = 0; k < CV; ++k) {
+            const uint4 x = c4[k * CT + threadIdx.x];
+            s += x.x + x.y + x.z + x.w;
+        }
+    } else {
+        for (int k = 0; k < CI; ++k) {
+            const int64_t c = base + k * CT + threadIdx.x;
+            if (c < m) s += cnt[c];
+        }
     }
     uint32_t tot;
     block_excl(s, lw, &tot);
@@ -106,28 +133,57 @@ __global__ __launch_bounds__(CT) void cell_scan_partials(uint32_t* __restrict__ 
     }
 }
 
-// The tile is read (and cnt cleared) with coalesced accesses into LDS, each thread scans CI
-// consecutive cells there, and cs is written back coalesced.  LDS index i is padded by
-// i / 32 so that the thread-contiguous pass does not hit one bank.  The last tile also
-// writes cs[m] = n.
+// Sum of part[0 .. b) over the workgroup (the tile's offset when the tile counts were not
+// scanned first).
+__device__ __forceinline__ uint32_t tiles_before(const uint32_t* __restrict__ part, int b, uint32_t* lw) {
+    uint32_t s = 0;
+    for (int j = threadIdx.x; j < b; j += CT) s += part[j];
+    uint32_t tot;
+    block_excl(s, lw, &tot);
+    return tot;
+}
+
+// The tile is read (and cnt cleared) with coalesced 16-byte accesses into LDS, each thread
+// scans CI consecutive cells there, and cs is written back coalesced.  LDS index i is padded
+// by i / 32 so that the thread-contiguous pass does not hit one bank.  The last tile also
+// writes cs[m] = n.  MERGED: part holds tile sums (the offset is summed here); otherwise
+// part holds the tiles' exclusive prefix sums.
+
 __device__ __forceinline__ int cpad(int i) { return i + (i >> 5); }
 
+template <bool MERGED>
 __global__ __launch_bounds__(CT) void cell_scan_down(uint32_t* __restrict__ cnt, int m,
                                                       const uint32_t* __restrict__ part, uint32_t* __restrict__ cs,
                                                       uint32_t n) {
     __shared__ uint32_t lw[CWAVES];
     __shared__ uint32_t tile[CTILE + CTILE / 32];
     const int64_t base = (int64_t)blockIdx.x * CTILE;
+    const uint32_t offset = MERGED ? tiles_before(part, blockIdx.x, lw) : part[blockIdx.x];
+    const bool full = base + CTILE <= m;  // base is a multiple of 4 cells: 16-byte aligned
+    if (full) {
+        uint4* c4 = reinterpret_cast<uint4*>(cnt + base);
 #pragma unroll
-    for (int k = 0; k < CI; ++k) {
-        const int j = k * CT + threadIdx.x;
-        const int64_t c = base + j;
-        uint32_t v = 0u;
-        if (c < m) {
-            v = cnt[c];
-            cnt[c] = 0u;
+        for (int k = 0; k < CV; ++k) {
+            const int v = k * CT + threadIdx.x;
+            const uint4 x = c4[v];
+            c4[v] = make_uint4(0u, 0u, 0u, 0u);
+            tile[cpad(4 * v)] = x.x;
+            tile[cpad(4 * v + 1)] = x.y;
+            tile[cpad(4 * v + 2)] = x.z;
+            tile[cpad(4 * v + 3)] = x.w;
         }
-        tile[cpad(j)] = v;
+    } else {
+#pragma unroll
+        for (int k = 0; k < CI; ++k) {
+            const int j = k * CT + threadIdx.x;
+            const int64_t c = base + j;
+            uint32_t v = 0u;
+            if (c < m) {
+                v = cnt[c];
+                cnt[c] = 0u;
+            }
+            tile[cpad(j)] = v;
+        }
     }
     __syncthreads();
     const int j0 = threadIdx.x * CI;
@@ -135,7 +191,7 @@ __global__ __launch_bounds__(CT) void cell_scan_down(uint32_t* __restrict__ cnt,
 #pragma unroll
     for (int k = 0; k < CI; ++k) s += tile[cpad(j0 + k)];
     uint32_t tot;
-    uint32_t run = part[blockIdx.x] + block_excl(s, lw, &tot);  // synchronises
+    uint32_t run = offset + block_excl(s, lw, &tot);  // synchronises
 #pragma unroll
     for (int k = 0; k < CI; ++k) {
         const uint32_t v = tile[cpad(j0 + k)];
@@ -143,10 +199,19 @@ __global__ __launch_bounds__(CT) void cell_scan_down(uint32_t* __restrict__ cnt,
         run += v;
     }
     __syncthreads();
+    if (full) {
+        uint4* o4 = reinterpret_cast<uint4*>(cs + base);
 #pragma unroll
-    for (int k = 0; k < CI; ++k) {
-        const int j = k * CT + threadIdx.x;
-        if (base + j < m) cs[base + j] = tile[cpad(j)];
+        for (int k = 0; k < CV; ++k) {
+            const int v = k * CT + threadIdx.x;
+            o4[v] = make_uint4(tile[cpad(4 * v)], tile[cpad(4 * v + 1)], tile[cpad(4 * v + 2)], tile[cpad(4 * v + 3)]);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < CI; ++k) {
+            const int j = k * CT + threadIdx.x;
+            if (base + j < m) cs[base + j] = tile[cpad(j)];
+        }
     }
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) cs[m] = n;
 }
@@ -201,8 +266,12 @@ hipError_t launch_cell_sort(sph_ctx* c, bool gather, hipStream_t s) {
     const int tiles = (int)cell_scan_tiles(m);
     if (n > 0) cell_count_kernel<<<cblocks(n), CT, 0, s>>>(n, b.keyI, b.cell_cnt, rank);
     cell_scan_reduce<<<tiles, CT, 0, s>>>(b.cell_cnt, m, b.cell_part);
-    cell_scan_partials<<<1, CT, 0, s>>>(b.cell_part, tiles);
-    cell_scan_down<<<tiles, CT, 0, s>>>(b.cell_cnt, m, b.cell_part, b.cs, (uint32_t)n);
+    if (tiles <= kMergedTiles) {
+        cell_scan_down<true><<<tiles, CT, 0, s>>>(b.cell_cnt, m, b.cell_part, b.cs, (uint32_t)n);
+    } else {
+        cell_scan_partials<<<1, CT, 0, s>>>(b.cell_part, tiles);
+        cell_scan_down<false><<<tiles, CT, 0, s>>>(b.cell_cnt, m, b.cell_part, b.cs, (uint32_t)n);
+    }
     if (n > 0) {
         cell_scatter_kernel<<<cblocks(n), CT, 0, s>>>(n, b.keyI, rank, b.cs, tmp);
         if (gather)

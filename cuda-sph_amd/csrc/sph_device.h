@@ -343,6 +343,9 @@ struct PairSums {
 #define SPH_FORCE_UNROLL 4
 #endif
 constexpr int kForceUnroll = SPH_FORCE_UNROLL;
+#ifndef SPH_FORCE_PREFETCH
+#define SPH_FORCE_PREFETCH 1
+#endif
 
 // Pressure and viscosity sums of particle i over its neighbour list (or, for a particle
 // with more neighbours than list slots, a rescan of its candidates).  The list holds the
@@ -360,12 +363,31 @@ __device__ __forceinline__ PairSums naive_pair_sums(const StepConsts& C, int i, 
         const uint32_t* slot = nl + i;
         const size_t stride = (size_t)n;
         uint32_t t = 0;
+#if SPH_FORCE_PREFETCH
+        // the next group's list slots are loaded while this group's neighbours are gathered,
+        // so each group waits on one memory latency instead of two
+        uint32_t kn[kForceUnroll];
+        if (cnt >= (uint32_t)kForceUnroll) {
+#pragma unroll
+            for (int u = 0; u < kForceUnroll; ++u) kn[u] = slot[u * stride];
+        }
+#endif
         for (; t + kForceUnroll <= cnt; t += kForceUnroll) {
             uint32_t k[kForceUnroll];
             float4 q[kForceUnroll], w[kForceUnroll];
+#if SPH_FORCE_PREFETCH
+#pragma unroll
+            for (int u = 0; u < kForceUnroll; ++u) k[u] = kn[u];
+            slot += kForceUnroll * stride;
+            if (t + 2 * kForceUnroll <= cnt) {
+#pragma unroll
+                for (int u = 0; u < kForceUnroll; ++u) kn[u] = slot[u * stride];
+            }
+#else
 #pragma unroll
             for (int u = 0; u < kForceUnroll; ++u) k[u] = slot[u * stride];
             slot += kForceUnroll * stride;
+#endif
 #pragma unroll
             for (int u = 0; u < kForceUnroll; ++u) {
                 q[u] = ld_f4(rpos, k[u] * 16u);
@@ -408,10 +430,20 @@ __device__ __forceinline__ PairSums naive_pair_sums_lanes(const StepConsts& C, i
         const uint32_t cnt = nn[i];
         if (cnt <= (uint32_t)kListCap) {
             const size_t stride = (size_t)n;
+#if SPH_FORCE_PREFETCH
+            // the lane's next list slot is loaded while this slot's neighbour is gathered
+            uint32_t kn = l < cnt ? nl[l * stride + i] : 0u;
+            for (uint32_t t = l; t < cnt; t += L) {
+                const uint32_t k = kn;
+                if (t + L < cnt) kn = nl[(t + L) * stride + i];
+                S.add(C, p, v, ld_f4(rpos, k * 16u), ld_f4(rvel, k * 16u), k == (uint32_t)i);
+            }
+#else
             for (uint32_t t = l; t < cnt; t += L) {
                 const uint32_t k = nl[t * stride + i];
                 S.add(C, p, v, ld_f4(rpos, k * 16u), ld_f4(rvel, k * 16u), k == (uint32_t)i);
             }
+#endif
         } else {
             const float hs = C.h_search;
             const float hs2 = hs * hs;

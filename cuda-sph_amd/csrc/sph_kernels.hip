@@ -234,6 +234,11 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_kernel(StepCons
 // starts at a multiple of 64, so a wavefront belongs to one instance and its lane count is
 // uniform.  The instance is found once per wavefront, by a binary search over the ranges'
 // first threads with the wavefront's first thread made scalar.
+#ifndef SPH_BATCH_XCD
+#define SPH_BATCH_XCD 1
+#endif
+__device__ __forceinline__ int batch_tile(int bid, int nb) { return SPH_BATCH_XCD ? xcd_tile(bid, nb) : bid; }
+
 struct BatchLane {
     int b;       // instance
     int i;       // cell-ordered particle (total index), or -1 past the instance's particles
@@ -266,7 +271,7 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_batch_kernel(St
                                                                                uint32_t* __restrict__ nl,
                                                                                uint32_t* __restrict__ nn) {
     __shared__ DensityList s_list[kLdsListCap];
-    const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const uint32_t gt = (uint32_t)batch_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const BatchLane t = batch_lane(G, gt);
     if (t.i < 0) return;
     const StepConsts& C = G.inst_table[t.b];
@@ -317,7 +322,7 @@ __global__ __launch_bounds__(256) void naive_force_batch_kernel(
     const uint32_t* __restrict__ nn, float4* __restrict__ posOut, float4* __restrict__ velOut,
     uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
     float4* __restrict__ dbgP, float4* __restrict__ dbgV, float4* __restrict__ dbgF, uint32_t* __restrict__ flags) {
-    const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const uint32_t gt = (uint32_t)batch_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const BatchLane t = batch_lane(G, gt);
     if (t.i < 0) return;
     const StepConsts& C = G.inst_table[t.b];
