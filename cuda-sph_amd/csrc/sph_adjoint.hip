@@ -332,6 +332,8 @@ hipError_t launch_adjoint_pullback(sph_ctx* c, hipStream_t s) {
         case 4: launch_pullback_lanes<4>(c, s); break;
         case 8: launch_pullback_lanes<8>(c, s); break;
         case 16: launch_pullback_lanes<16>(c, s); break;
+        case 32: launch_pullback_lanes<32>(c, s); break;
+        case 64: launch_pullback_lanes<64>(c, s); break;
         default: launch_pullback_lanes<1>(c, s); break;
     }
     adjoint_mu_kernel<<<1, kAdjThreads, 0, s>>>((int)n, b.adj_gx, b.mu_acc);

@@ -230,14 +230,17 @@ int plan(sph_ctx* c) {
     return SPH_OK;
 }
 
+// Lanes of a wavefront per particle: a power of two that divides the wavefront.
+bool valid_lanes(int L) { return L >= 1 && L <= 64 && (L & (L - 1)) == 0; }
+
 // Thread ranges of the batched density and force passes: instance b takes n_b * lanes_b
 // threads from tfirst_b on, rounded up to whole wavefronts; the totals entry gets the count.
 int batch_thread_layout(sph_ctx* c, const std::vector<int>& lanes) {
     int64_t t = 0;
     for (size_t b = 0; b < c->inst.size(); ++b) {
         const int L = lanes[b];
-        if (L != 1 && L != 2 && L != 4 && L != 8 && L != 16)
-            return fail(SPH_E_INVALID, "lanes per particle must be 1, 2, 4, 8 or 16 (instance %zu: %d)", b, L);
+        if (!valid_lanes(L))
+            return fail(SPH_E_INVALID, "lanes per particle must be 1, 2, 4, 8, 16, 32 or 64 (instance %zu: %d)", b, L);
         t += (((int64_t)c->inst[b].n * L) + 63) & ~int64_t(63);
         if (t >= (int64_t(1) << 31))
             return fail(SPH_E_INVALID, "the batch's particles x lanes exceed 2^31 threads");
@@ -645,8 +648,8 @@ int sph_set_launch(sph_ctx* c, int32_t density_block, int32_t force_block) {
 
 int sph_set_lanes(sph_ctx* c, int32_t lanes) {
     if (!c) return fail(SPH_E_INVALID, "null context");
-    if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16)
-        return fail(SPH_E_INVALID, "lanes per particle must be 1, 2, 4, 8 or 16");
+    if (!valid_lanes(lanes))
+        return fail(SPH_E_INVALID, "lanes per particle must be 1, 2, 4, 8, 16, 32 or 64");
     if (lanes > 1 && c->prm.formulation != SPH_FORM_NAIVE)
         return fail(SPH_E_INVALID, "the VOXEL formulation walks its capped list with one lane per particle");
     if (c->n * (int64_t)lanes >= (int64_t(1) << 31))

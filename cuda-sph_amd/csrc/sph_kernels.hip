@@ -192,13 +192,14 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
                     const float t = fmaxf(1.0f - r2 * C.inv_h2, 0.0f);
                     acc += t * t * t;
                     const bool hit = r2 <= C.h2;
-                    const uint32_t gm = (uint32_t)(__ballot(hit) >> gbase) & ((1u << L) - 1u);
+                    constexpr uint64_t group = L == 64 ? ~0ull : (1ull << L) - 1ull;
+                    const uint64_t gm = (__ballot(hit) >> gbase) & group;
                     if (hit) {
-                        const uint32_t slot = cnt + (uint32_t)__popc(gm & ((1u << l) - 1u));
+                        const uint32_t slot = cnt + (uint32_t)__popcll(gm & ((1ull << l) - 1ull));
                         if (slot < (uint32_t)kLdsListCap) s_list[slot][col] = k;
                         else if (slot < (uint32_t)kListCap) nl[slot * stride + i] = k;
                     }
-                    cnt += (uint32_t)__popc(gm);
+                    cnt += (uint32_t)__popcll(gm);
                 }
             }
         }
@@ -213,7 +214,7 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
     }
 }
 
-// Plain step: L lanes per particle (1, 2, 4, 8 or 16), particle gt / L.  Groups are aligned,
+// Plain step: L lanes per particle (1, 2, 4, 8, 16, 32 or 64), particle gt / L.  Groups are aligned,
 // so a whole group leaves together at the end of the range.
 template <int L>
 __global__ __launch_bounds__(kDensityThreads) void naive_density_kernel(StepConsts G, float4* pos, float4* vel,
@@ -221,7 +222,7 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_kernel(StepCons
                                                                          float* __restrict__ rhoA,
                                                                          uint32_t* __restrict__ nl,
                                                                          uint32_t* __restrict__ nn) {
-    static_assert(L >= 1 && L <= 16 && (L & (L - 1)) == 0, "lanes per particle: 1, 2, 4, 8 or 16");
+    static_assert(L >= 1 && L <= 64 && (L & (L - 1)) == 0, "lanes per particle: 1, 2, 4, ..., 64");
     __shared__ DensityList s_list[kLdsListCap];
     const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int i = (int)(gt / L);
@@ -284,7 +285,9 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_batch_kernel(St
         case 2: density_particle<2>(G, C, t.i, t.l, col, gbase, pos, vel, cs, rhoA, nl, nn, s_list); break;
         case 4: density_particle<4>(G, C, t.i, t.l, col, gbase, pos, vel, cs, rhoA, nl, nn, s_list); break;
         case 8: density_particle<8>(G, C, t.i, t.l, col, gbase, pos, vel, cs, rhoA, nl, nn, s_list); break;
-        default: density_particle<16>(G, C, t.i, t.l, col, gbase, pos, vel, cs, rhoA, nl, nn, s_list); break;
+        case 16: density_particle<16>(G, C, t.i, t.l, col, gbase, pos, vel, cs, rhoA, nl, nn, s_list); break;
+        case 32: density_particle<32>(G, C, t.i, t.l, col, gbase, pos, vel, cs, rhoA, nl, nn, s_list); break;
+        default: density_particle<64>(G, C, t.i, t.l, col, gbase, pos, vel, cs, rhoA, nl, nn, s_list); break;
     }
 }
 
@@ -334,7 +337,9 @@ __global__ __launch_bounds__(256) void naive_force_batch_kernel(
         case 2: S = naive_pair_sums_lanes<false, 2>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
         case 4: S = naive_pair_sums_lanes<false, 4>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
         case 8: S = naive_pair_sums_lanes<false, 8>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
-        default: S = naive_pair_sums_lanes<false, 16>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
+        case 16: S = naive_pair_sums_lanes<false, 16>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
+        case 32: S = naive_pair_sums_lanes<false, 32>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
+        default: S = naive_pair_sums_lanes<false, 64>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
     }
     if (t.l != 0) return;
     const float ps = 0.5f * C.mass * C.grad_c;
@@ -650,6 +655,8 @@ hipError_t launch_density(sph_ctx* c, hipStream_t s) {
         case 4: launch_density_lanes<4>(c, s); break;
         case 8: launch_density_lanes<8>(c, s); break;
         case 16: launch_density_lanes<16>(c, s); break;
+        case 32: launch_density_lanes<32>(c, s); break;
+        case 64: launch_density_lanes<64>(c, s); break;
         default: launch_density_lanes<1>(c, s); break;
     }
     return hipGetLastError();
@@ -686,6 +693,8 @@ void launch_force_mode(sph_ctx* c, hipStream_t s) {
         case 4: launch_force_lanes<MODE, 4>(c, s, dP, dV, dF); break;
         case 8: launch_force_lanes<MODE, 8>(c, s, dP, dV, dF); break;
         case 16: launch_force_lanes<MODE, 16>(c, s, dP, dV, dF); break;
+        case 32: launch_force_lanes<MODE, 32>(c, s, dP, dV, dF); break;
+        case 64: launch_force_lanes<MODE, 64>(c, s, dP, dV, dF); break;
         default: launch_force_lanes<MODE, 1>(c, s, dP, dV, dF); break;
     }
 }

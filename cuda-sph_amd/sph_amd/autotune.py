@@ -3,7 +3,7 @@
 Three knobs decide how the step maps onto the GPU:
   - the workgroup sizes of the density and force passes (64, 128 or 256 threads: one to
     four wavefronts over consecutive cell-ordered particles);
-  - the lanes per particle of those passes (1 to 16 lanes of a wavefront share one
+  - the lanes per particle of those passes (1 to 64 lanes of a wavefront share one
     particle's candidates: small or dense problems gain parallelism); and
   - the grid cell edge, which sets how many cells a workgroup's particles cover ("cells
     per tile") and how many candidates each particle tests.  Smaller cells test fewer
@@ -13,7 +13,7 @@ Three knobs decide how the step maps onto the GPU:
     reference's semantics (voxel_sph_strategy.py:109-116), so only block sizes are tuned.
 
 The batched form tunes the grid and the lanes per instance: instances differ by orders of
-magnitude in neighbours per particle (a dense one wants 16 lanes and finer cells, a sparse
+magnitude in neighbours per particle (a dense one wants many lanes and finer cells, a sparse
 one a single lane and coarse cells, whose table the counting sort scans every step).  Each
 instance's neighbour load is estimated from one short run (the density over the particle
 mass is the number density, times the volume of the h-sphere), and the candidates are
@@ -44,7 +44,7 @@ from .workloads import voxel_for
 
 BLOCKS = (64, 128, 256)
 CELL_SCALES = (2.0, 1.0, 0.5, 0.25)
-LANES = (1, 2, 4, 8, 16)
+LANES = (1, 2, 4, 8, 16, 32, 64)
 
 
 @dataclass(frozen=True)
@@ -284,12 +284,15 @@ def neighbour_load(instances: Sequence[Tuple[SimulationParameters, SPHConstants]
     return np.asarray(load)
 
 
-def lanes_for_load(load: np.ndarray, divisor: float) -> Tuple[int, ...]:
-    """Largest power of two in 1..16 not above load / divisor, per instance."""
+def lanes_for_load(load: np.ndarray, divisor: float, cap: int = 16) -> Tuple[int, ...]:
+    """Largest power of two in 1..cap not above load / divisor, per instance.  Every lane of a
+    group walks all of the particle's cell rows, so beyond 16 lanes the row walk outgrows the
+    candidates each lane saves (measured on batch_64: 64 lanes for its densest instance made
+    the batch slower)."""
     out = []
     for x in np.asarray(load, dtype=np.float64):
         q = x / divisor if np.isfinite(x) else 1.0
-        out.append(int(min(16, 2 ** int(np.floor(np.log2(q))))) if q >= 1.0 else 1)
+        out.append(int(min(cap, 2 ** int(np.floor(np.log2(q))))) if q >= 1.0 else 1)
     return tuple(out)
 
 

@@ -39,7 +39,7 @@ class BatchedSPH:
         or one per instance), instance b searches a grid of edge voxel_for(h_b) * cell_scale_b
         instead of its voxel_size (the NAIVE sums do not depend on the grid; see autotune.py).
         lanes: lanes of a wavefront per particle in the density and force passes, one value or
-        one per instance (1, 2, 4, 8 or 16)."""
+        one per instance (1, 2, 4, 8, 16, 32 or 64)."""
         self.instances = list(instances)
         count = len(self.instances)
         self.cell_scale = _per_instance(cell_scale, count, "cell_scale")

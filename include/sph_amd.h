@@ -200,7 +200,7 @@ int sph_grid_info(const sph_ctx* ctx, int32_t* dims3, int32_t* stencil3, int32_t
 int sph_set_launch(sph_ctx* ctx, int32_t density_block, int32_t force_block);
 
 /* Lanes of a wavefront that share one particle in the NAIVE density and force passes
- * (1, 2, 4, 8 or 16; default 1), for the autotuner: small or dense problems (few particles,
+ * (1, 2, 4, 8, 16, 32 or 64; default 1), for the autotuner: small or dense problems (few particles,
  * many neighbours each) get that many times the parallelism.  The neighbour lists are the
  * same for every value; sums are added in a different order (FP32 rounding only).  The
  * reference launches one thread per particle (thread_layout.py:27-33).  On a batched
@@ -208,7 +208,7 @@ int sph_set_launch(sph_ctx* ctx, int32_t density_block, int32_t force_block);
 int sph_set_lanes(sph_ctx* ctx, int32_t lanes);
 
 /* Batched context: lanes per particle of each instance (count = the number of instances, each
- * 1, 2, 4, 8 or 16), so that instances with many neighbours per particle get more lanes and
+ * 1, 2, 4, 8, 16, 32 or 64), so that instances with many neighbours per particle get more lanes and
  * sparse ones fewer.  Each instance computes what a plain context with its lane count
  * computes.  The instance table is re-uploaded on `stream` (synchronised) when the workspace
  * is bound.  No counterpart in the reference, which advances one simulation per process. */

@@ -92,10 +92,12 @@ def test_batch_result_round_trips_per_instance_values(tmp_path):
 
 def test_load_rules():
     load = np.array([0.5, 3.9, 4.0, 31.9, 64.0, 2627.0, np.nan])
-    # largest power of two in 1..16 not above load / divisor; below 1 (or unknown) one lane
+    # largest power of two in 1..16 (or 1..cap) not above load / divisor; below 1 (or
+    # unknown) one lane
     assert T.lanes_for_load(load, 4.0) == (1, 1, 1, 4, 16, 16, 1)
     assert T.lanes_for_load(load, 16.0) == (1, 1, 1, 1, 4, 16, 1)
     assert T.lanes_for_load(load, 128.0) == (1, 1, 1, 1, 1, 16, 1)
+    assert T.lanes_for_load(load, 4.0, cap=64) == (1, 1, 1, 4, 16, 64, 1)
     assert T.scales_for_load(load, 64.0) == (1.0, 1.0, 1.0, 1.0, 0.5, 0.5, 1.0)
     assert T.scales_for_load(load, 0.0)[:6] == (0.5,) * 6
     assert T.scales_for_load(load, float("inf")) == (1.0,) * 7

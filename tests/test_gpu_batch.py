@@ -56,7 +56,7 @@ def test_batch_mixed_lanes_and_grids_equal_plain(gpu):
     ws = _instances(seed=4, count=7)
     dense = box_uniform(6000, seed=77, h=0.058, edge=0.6, visc=3.0)
     ws.append(dense)
-    lanes = [1, 2, 4, 8, 16, 1, 2, 16]
+    lanes = [1, 2, 4, 8, 16, 32, 2, 64]
     scales = [1.0, 0.5, 1.0, 0.5, 1.0, 0.5, 1.0, 0.5]
     batch = BatchedSPH([(w.params, w.constants) for w in ws], device=gpu, cell_scale=scales, lanes=lanes)
     batch.upload([w.state for w in ws])
@@ -88,7 +88,7 @@ def test_set_batch_lanes_rejects(gpu):
     batch = BatchedSPH([(w.params, w.constants) for w in ws], device=gpu)
     with pytest.raises(RuntimeError, match="2 lane counts for 3"):
         batch.ctx.set_batch_lanes([1, 2])
-    with pytest.raises(RuntimeError, match="1, 2, 4, 8 or 16"):
+    with pytest.raises(RuntimeError, match="1, 2, 4, 8, 16, 32 or 64"):
         batch.ctx.set_batch_lanes([1, 3, 1])
     plain = NaiveSPHStrategy(ws[0].params, constants=ws[0].constants, device=gpu)
     with pytest.raises(RuntimeError, match="not a batched context"):

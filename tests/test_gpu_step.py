@@ -321,7 +321,7 @@ def test_graph_replays_match_eager(gpu, mode):
             assert np.array_equal(read(eager, N.BUF_RNG), read(graph, N.BUF_RNG))
 
 
-@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16])
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16, 32, 64])
 @pytest.mark.parametrize("dense", [False, True])
 def test_lanes_per_particle_match_oracle(gpu, lanes, dense):
     """Density and force with several lanes per particle: the step stays within the parity
@@ -346,7 +346,7 @@ def test_lanes_per_particle_match_oracle(gpu, lanes, dense):
 def test_set_lanes_rejects_bad_values(gpu):
     w = box_uniform(100, seed=29)
     s = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu)
-    for bad in (0, 3, 32):
+    for bad in (0, 3, 48, 128):
         with pytest.raises(RuntimeError, match="lanes per particle"):
             s.ctx.set_lanes(bad)
     v = VoxelSPHStrategy(w.params, constants=w.constants, device=gpu)
