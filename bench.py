@@ -88,6 +88,8 @@ def parse() -> argparse.Namespace:
                     help="also measure frame output to disk in the reference's format")
     ap.add_argument("--pipe", choices=["on", "off"], default="on",
                     help="also measure the pipe workload (524,288 particles, 1000 steps)")
+    ap.add_argument("--small", choices=["on", "off"], default="on",
+                    help="also measure box_4k (4,096 particles, 100 steps) on the GPU")
     ap.add_argument("--graph", choices=["on", "off"], default="off",
                     help="replay the timed steps of the plain and pipe legs as one HIP graph (default: launch "
                          "them one by one, which measures the same on these workloads)")
@@ -612,7 +614,7 @@ def main() -> int:
 
     pipe = bench_pipe(tune=args.tune == "on", graph=graph) if args.pipe == "on" else None
     frames = bench_frames() if args.frames == "on" else None
-    small = bench_small(tune=args.tune == "on")
+    small = bench_small(tune=args.tune == "on") if args.small == "on" else None
     batched = bench_batched(args.batch_steps, tune=args.tune == "on") if args.batched == "on" else None
     adjoint = bench_adjoint(tune=args.tune == "on") if args.adjoint == "on" else None
 

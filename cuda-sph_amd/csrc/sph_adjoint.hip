@@ -127,7 +127,7 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_seed_kernel(
     const Vec3 P{ps * S.ax, ps * S.ay, ps * S.az}, V{vs * S.bx, vs * S.by, vs * S.bz};
     const float rho = rhoA[i];
     const uint32_t pid = pidA[i];
-    finish_particle<SPH_MODE_BOX>(C, i, p, v, pid, rho, P, V, posOut, velOut, pidOut, keyI, valI, nullptr, nullptr,
+    finish_particle<SPH_MODE_BOX, true>(C, i, p, v, pid, rho, P, V, posOut, velOut, pidOut, keyI, valI, nullptr, nullptr,
                                   nullptr, nullptr, nullptr, flags);
     const Integrated I = integrate_particle(C, p, v, rho, P, V);
     const float4 gx1 = cotx[pid], gv1 = cotv[pid];

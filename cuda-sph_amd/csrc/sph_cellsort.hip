@@ -3,18 +3,21 @@
 // The NAIVE path's keys are cell indices in [0, C) (clamped to the grid), so the order the
 // step needs -- by (cell, particle id), the order of the reference's stable sort of the voxel
 // map (voxel_sph_strategy.py:81-89) -- is a counting sort by cell followed by id order within
-// each cell.  Per step:
-//   count    one thread per particle id: rank = atomicAdd(&cnt[key], 1) (an arbitrary order
-//            among the particles of a cell)
+// each cell.  The keys arrive in the state's current slot order (the previous step's cell
+// order, see hash_particle), so neighbouring threads mostly hit the same or neighbouring
+// cells.  Per step:
+//   count    one thread per slot: rank = atomicAdd(&cnt[key], 1) (an arbitrary order among
+//            the particles of a cell)
 //   scan     exclusive prefix sum of cnt over all cells: tile sums (16-byte loads), then each
 //            tile's offset is the sum of the tile sums before it (added up by the tile's own
 //            workgroup when there are few tiles, else by a one-workgroup scan first), then
 //            the tile is scanned; the result is the cell start table cs[0..C] itself, and the
 //            tile scan resets cnt to zero for the next step
-//   scatter  one thread per particle id: tmp[cs[key] + rank] = id
-//   order    one thread per slot: the id there moves to cs[key] + (number of ids of its cell
-//            that are smaller), which is id order within the cell; the sorted key and slot
-//            are written there and, in the fused step, the particle's state is gathered
+//   scatter  one thread per slot: tmp[cs[key] + rank] = slot
+//   order    one thread per position: the particle there moves to cs[key] + (number of
+//            particles of its cell with a smaller id), which is id order within the cell; the
+//            sorted key and slot are written there and, in the fused step, the particle's
+//            state is gathered
 // The within-cell pass costs one comparison per particle of the cell, less than the density
 // pass spends on the same cell.  The final order does not depend on the atomic order.
 #include "sph_internal.h"
@@ -226,7 +229,6 @@ __global__ __launch_bounds__(CT) void cell_scatter_kernel(int n, const uint32_t*
 template <bool GATHER>
 __global__ __launch_bounds__(CT) void cell_order_kernel(int n, const uint32_t* __restrict__ tmp,
                                                          const uint32_t* __restrict__ keyI,
-                                                         const uint32_t* __restrict__ valI,
                                                          const uint32_t* __restrict__ cs, uint32_t* __restrict__ skey,
                                                          uint32_t* __restrict__ sval, const float4* __restrict__ posB,
                                                          const float4* __restrict__ velB,
@@ -234,19 +236,19 @@ __global__ __launch_bounds__(CT) void cell_order_kernel(int n, const uint32_t* _
                                                          float4* __restrict__ velA, uint32_t* __restrict__ pidA) {
     const int s = blockIdx.x * CT + threadIdx.x;
     if (s >= n) return;
-    const uint32_t id = tmp[s];
-    const uint32_t k = keyI[id];
+    const uint32_t slot = tmp[s];
+    const uint32_t id = pidB[slot];
+    const uint32_t k = keyI[slot];
     const uint32_t a = cs[k], b = cs[k + 1];
     uint32_t r = 0;
-    for (uint32_t t = a; t < b; ++t) r += tmp[t] < id ? 1u : 0u;
+    for (uint32_t t = a; t < b; ++t) r += pidB[tmp[t]] < id ? 1u : 0u;
     const uint32_t f = a + r;
-    const uint32_t v = valI[id];
     skey[f] = k;
-    sval[f] = v;
+    sval[f] = slot;
     if (GATHER) {
-        posA[f] = posB[v];
-        velA[f] = velB[v];
-        pidA[f] = pidB[v];
+        posA[f] = posB[slot];
+        velA[f] = velB[slot];
+        pidA[f] = id;
     }
 }
 
@@ -275,11 +277,11 @@ hipError_t launch_cell_sort(sph_ctx* c, bool gather, hipStream_t s) {
     if (n > 0) {
         cell_scatter_kernel<<<cblocks(n), CT, 0, s>>>(n, b.keyI, rank, b.cs, tmp);
         if (gather)
-            cell_order_kernel<true><<<cblocks(n), CT, 0, s>>>(n, tmp, b.keyI, b.valI, b.cs, b.keyS[1], b.valS[1],
-                                                               b.posB, b.velB, b.pidB, b.posA, b.velA, b.pidA);
+            cell_order_kernel<true><<<cblocks(n), CT, 0, s>>>(n, tmp, b.keyI, b.cs, b.keyS[1], b.valS[1], b.posB,
+                                                               b.velB, b.pidB, b.posA, b.velA, b.pidA);
         else
-            cell_order_kernel<false><<<cblocks(n), CT, 0, s>>>(n, tmp, b.keyI, b.valI, b.cs, b.keyS[1], b.valS[1],
-                                                                nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+            cell_order_kernel<false><<<cblocks(n), CT, 0, s>>>(n, tmp, b.keyI, b.cs, b.keyS[1], b.valS[1], nullptr,
+                                                                nullptr, b.pidB, nullptr, nullptr, nullptr);
     }
     c->sorted_keys = b.keyS[1];
     c->sorted_vals = b.valS[1];

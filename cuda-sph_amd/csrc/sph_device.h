@@ -96,11 +96,21 @@ __device__ __forceinline__ bool finite3(float x, float y, float z) {
     return isfinite(x) && isfinite(y) && isfinite(z);
 }
 
+// Sort input of the next step.  SLOT_KEYS (the NAIVE formulation's counting sort): the key
+// of the particle in posB slot `slot`, indexed by the slot, so the keys stay in the current
+// cell order (coalesced stores; the sort's atomics and scatters touch neighbouring cells).
+// Otherwise (the radix sort, which must see particle-id order to be stable by id): key and
+// slot indexed by particle id.
+template <bool SLOT_KEYS>
 __device__ __forceinline__ void hash_particle(const StepConsts& C, float x, float y, float z, uint32_t pid,
                                               uint32_t slot, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
                                               uint32_t* __restrict__ flags) {
-    keyI[pid] = sort_key(x, y, z, C);
-    valI[pid] = slot;
+    if constexpr (SLOT_KEYS) {
+        keyI[slot] = sort_key(x, y, z, C);
+    } else {
+        keyI[pid] = sort_key(x, y, z, C);
+        valI[pid] = slot;
+    }
     if (!finite3(x, y, z)) atomicAdd(flags, 1u);
 }
 
@@ -285,8 +295,8 @@ __device__ __forceinline__ bool box_bounced(const StepConsts& C, float x, int d)
 }
 
 // Symplectic Euler (base_kernels.py:30-53), collision, output in the same sorted slot of
-// posB/velB, hash of the new position for the next step.
-template <int MODE>
+// posB/velB, hash of the new position for the next step (SLOT_KEYS: see hash_particle).
+template <int MODE, bool SLOT_KEYS>
 __device__ __forceinline__ void finish_particle(const StepConsts& C, int i, float4 p, float4 v, uint32_t pid,
                                                 float rho, Vec3 P, Vec3 V, float4* __restrict__ posOut,
                                                 float4* __restrict__ velOut, uint32_t* __restrict__ pidOut,
@@ -312,7 +322,7 @@ __device__ __forceinline__ void finish_particle(const StepConsts& C, int i, floa
         dbgV[i] = make_float4(V.x, V.y, V.z, 0.0f);
         dbgF[i] = make_float4(fx, fy, fz, 0.0f);
     }
-    hash_particle(C, pp.x, pp.y, pp.z, pid, (uint32_t)i, keyI, valI, flags);
+    hash_particle<SLOT_KEYS>(C, pp.x, pp.y, pp.z, pid, (uint32_t)i, keyI, valI, flags);
 }
 
 // Pressure and viscosity sums of one pair (naive_kernels.py:38-52, :70-77) without the

@@ -33,7 +33,7 @@ __global__ __launch_bounds__(256) void upload_f64_kernel(StepConsts G, const dou
     velB[i] = make_float4((float)vel[3 * (int64_t)i], (float)vel[3 * (int64_t)i + 1], (float)vel[3 * (int64_t)i + 2],
                           0.0f);
     pidB[i] = (uint32_t)i;
-    hash_particle(C, x, y, z, (uint32_t)i, (uint32_t)i, keyI, valI, flags);
+    hash_particle<false>(C, x, y, z, (uint32_t)i, (uint32_t)i, keyI, valI, flags);  // slot = id: both orders
 }
 
 template <bool BATCH>
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(256) void upload_f32_kernel(StepConsts G, const flo
     posB[i] = make_float4(x, y, z, 0.0f);
     velB[i] = make_float4(vx[i], vy[i], vz[i], 0.0f);
     pidB[i] = (uint32_t)i;
-    hash_particle(C, x, y, z, (uint32_t)i, (uint32_t)i, keyI, valI, flags);
+    hash_particle<false>(C, x, y, z, (uint32_t)i, (uint32_t)i, keyI, valI, flags);  // slot = id: both orders
 }
 
 __global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, int64_t n) {
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(256) void naive_force_kernel(
     if (L > 1 && (gt & (L - 1)) != 0) return;
     const float ps = 0.5f * C.mass * C.grad_c;   // naive_kernels.py:40-52: m * (pi+pj)/(2 rho_j) * grad W
     const float vs = C.mass * C.visc * C.lap_c;  // naive_kernels.py:71-77: m * mu * (vj-vi)/rho_j * lap W
-    finish_particle<MODE>(C, i, p, v, pidA[i], rhoA[i], Vec3{ps * S.ax, ps * S.ay, ps * S.az},
+    finish_particle<MODE, true>(C, i, p, v, pidA[i], rhoA[i], Vec3{ps * S.ax, ps * S.ay, ps * S.az},
                           Vec3{vs * S.bx, vs * S.by, vs * S.bz}, posOut, velOut, pidOut, keyI, valI, rng, pipe, dbgP,
                           dbgV, dbgF, flags);
 }
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(256) void naive_force_batch_kernel(
     if (t.l != 0) return;
     const float ps = 0.5f * C.mass * C.grad_c;
     const float vs = C.mass * C.visc * C.lap_c;
-    finish_particle<SPH_MODE_BOX>(C, i, p, v, pidA[i], rhoA[i], Vec3{ps * S.ax, ps * S.ay, ps * S.az},
+    finish_particle<SPH_MODE_BOX, true>(C, i, p, v, pidA[i], rhoA[i], Vec3{ps * S.ax, ps * S.ay, ps * S.az},
                                   Vec3{vs * S.bx, vs * S.by, vs * S.bz}, posOut, velOut, pidOut, keyI, valI, nullptr,
                                   nullptr, dbgP, dbgV, dbgF, flags);
 }
@@ -433,7 +433,7 @@ __global__ __launch_bounds__(256) void voxel_force_kernel(
     }
     // voxel_kernels.py:211 min(1, .)
     const Vec3 V{fminf(1.0f, bx), fminf(1.0f, by), fminf(1.0f, bz)};
-    finish_particle<MODE>(C, i, p, v, pidA[i], rho_i, Vec3{ax, ay, az}, V, posOut, velOut, pidOut, keyI, valI, rng, pipe,
+    finish_particle<MODE, false>(C, i, p, v, pidA[i], rho_i, Vec3{ax, ay, az}, V, posOut, velOut, pidOut, keyI, valI, rng, pipe,
                     dbgP, dbgV, dbgF, flags);
 }
 
@@ -492,9 +492,10 @@ __global__ __launch_bounds__(256) void read_buffer_kernel(StepConsts C, int whic
     }
     if (i >= C.n) return;
     switch (which) {
-        case SPH_BUF_KEYS: {
+        case SPH_BUF_KEYS: {  // NAIVE: keys in posB slot order (see hash_particle)
             const uint32_t u = keyI[i];
-            ((int32_t*)out)[i] = C.ref_keys ? (int32_t)(u ^ 0x80000000u) : (int32_t)u;
+            const int64_t o = C.formulation == SPH_FORM_NAIVE ? (int64_t)pidB[i] : i;
+            ((int32_t*)out)[o] = C.ref_keys ? (int32_t)(u ^ 0x80000000u) : (int32_t)u;
             break;
         }
         case SPH_BUF_PERM: {
