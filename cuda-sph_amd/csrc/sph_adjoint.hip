@@ -120,15 +120,15 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_seed_kernel(
     const int i = (int)(gt / L);
     if (i >= C.n) return;
     const float4 p = pos[i], v = vel[i];
-    const PairSums S = naive_pair_sums_lanes<false, L>(C, i, gt & (L - 1), C.n, p, v, pos, vel, cs, nl, nn);
+    const PairSums S = naive_pair_sums_lanes<L>(C, i, gt & (L - 1), C.n, p, v, pos, vel, cs, nl, nn);
     if (L > 1 && (gt & (L - 1)) != 0) return;
     const float ps = 0.5f * C.mass * C.grad_c;
     const float vs = C.mass * C.visc * C.lap_c;
     const Vec3 P{ps * S.ax, ps * S.ay, ps * S.az}, V{vs * S.bx, vs * S.by, vs * S.bz};
     const float rho = rhoA[i];
     const uint32_t pid = pidA[i];
-    finish_particle<SPH_MODE_BOX, true>(C, i, p, v, pid, rho, P, V, posOut, velOut, pidOut, keyI, valI, nullptr, nullptr,
-                                  nullptr, nullptr, nullptr, flags);
+    finish_particle<SPH_MODE_BOX, true>(C, i, p, v, pid, rho, P, V, posOut, velOut, pidOut, keyI, valI, nullptr,
+                                        nullptr, nullptr, nullptr, nullptr, flags);
     const Integrated I = integrate_particle(C, p, v, rho, P, V);
     const float4 gx1 = cotx[pid], gv1 = cotv[pid];
     float gxp[3], gvp[3];

@@ -360,7 +360,6 @@ constexpr int kForceUnroll = SPH_FORCE_UNROLL;
 // Pressure and viscosity sums of particle i over its neighbour list (or, for a particle
 // with more neighbours than list slots, a rescan of its candidates).  The list holds the
 // particle itself; PairSums gives that pair a zero contribution.
-template <bool BATCH = false>
 __device__ __forceinline__ PairSums naive_pair_sums(const StepConsts& C, int i, int n, float4 p, float4 v,
                                                     const float4* __restrict__ pos, const float4* __restrict__ vel,
                                                     const uint32_t* __restrict__ cs, const uint32_t* __restrict__ nl,
@@ -424,7 +423,7 @@ __device__ __forceinline__ PairSums naive_pair_sums(const StepConsts& C, int i, 
 // The same sums with L lanes per particle (L = 1 is naive_pair_sums): lane l of the group
 // walks list slots l, l + L, ... (or, past the list, every L-th candidate of each row), and
 // the partial sums are added across the group; every lane of the group returns the total.
-template <bool BATCH, int L>
+template <int L>
 __device__ __forceinline__ PairSums naive_pair_sums_lanes(const StepConsts& C, int i, uint32_t l, int n, float4 p,
                                                           float4 v, const float4* __restrict__ pos,
                                                           const float4* __restrict__ vel,
@@ -432,7 +431,7 @@ __device__ __forceinline__ PairSums naive_pair_sums_lanes(const StepConsts& C, i
                                                           const uint32_t* __restrict__ nl,
                                                           const uint32_t* __restrict__ nn) {
     if constexpr (L == 1) {
-        return naive_pair_sums<BATCH>(C, i, n, p, v, pos, vel, cs, nl, nn);
+        return naive_pair_sums(C, i, n, p, v, pos, vel, cs, nl, nn);
     } else {
         const __amdgpu_buffer_rsrc_t rpos = make_rsrc(pos, (uint32_t)n * 16u);
         const __amdgpu_buffer_rsrc_t rvel = make_rsrc(vel, (uint32_t)n * 16u);

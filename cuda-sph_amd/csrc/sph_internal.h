@@ -7,7 +7,8 @@
 //   pidA, rhoA u32[N], f32[N]  particle id and density of each cell-ordered slot
 //   posB/velB  float4[N]  state after integration, still in the previous cell order, with
 //   pidB       u32[N]     rho of the step in posB.w; this is the resident state between steps
-//   keyI/valI  u32[N]     sort input in particle-id order: voxel key, slot in posB
+//   keyI/valI  u32[N]     sort input written with the new positions.  NAIVE: key per posB slot
+//                         (valI unused); VOXEL: key and posB slot per particle id
 //   keyS/valS  u32[2][N]  radix ping-pong buffers
 //   cs         u32[C+1]   cell start table: cs[c] = first cell-sorted index with key >= c
 //   cell_cnt   u32[C]     NAIVE: per-cell counts of the counting sort (kept zero between steps)

@@ -235,11 +235,6 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_kernel(StepCons
 // starts at a multiple of 64, so a wavefront belongs to one instance and its lane count is
 // uniform.  The instance is found once per wavefront, by a binary search over the ranges'
 // first threads with the wavefront's first thread made scalar.
-#ifndef SPH_BATCH_XCD
-#define SPH_BATCH_XCD 1
-#endif
-__device__ __forceinline__ int batch_tile(int bid, int nb) { return SPH_BATCH_XCD ? xcd_tile(bid, nb) : bid; }
-
 struct BatchLane {
     int b;       // instance
     int i;       // cell-ordered particle (total index), or -1 past the instance's particles
@@ -272,7 +267,7 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_batch_kernel(St
                                                                                uint32_t* __restrict__ nl,
                                                                                uint32_t* __restrict__ nn) {
     __shared__ DensityList s_list[kLdsListCap];
-    const uint32_t gt = (uint32_t)batch_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const BatchLane t = batch_lane(G, gt);
     if (t.i < 0) return;
     const StepConsts& C = G.inst_table[t.b];
@@ -309,7 +304,7 @@ __global__ __launch_bounds__(256) void naive_force_kernel(
     if (i >= G.n) return;
     const StepConsts& C = G;
     const float4 p = pos[i], v = vel[i];
-    const PairSums S = naive_pair_sums_lanes<false, L>(C, i, gt & (L - 1), G.n, p, v, pos, vel, cs, nl, nn);
+    const PairSums S = naive_pair_sums_lanes<L>(C, i, gt & (L - 1), G.n, p, v, pos, vel, cs, nl, nn);
     if (L > 1 && (gt & (L - 1)) != 0) return;
     const float ps = 0.5f * C.mass * C.grad_c;   // naive_kernels.py:40-52: m * (pi+pj)/(2 rho_j) * grad W
     const float vs = C.mass * C.visc * C.lap_c;  // naive_kernels.py:71-77: m * mu * (vj-vi)/rho_j * lap W
@@ -325,7 +320,7 @@ __global__ __launch_bounds__(256) void naive_force_batch_kernel(
     const uint32_t* __restrict__ nn, float4* __restrict__ posOut, float4* __restrict__ velOut,
     uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
     float4* __restrict__ dbgP, float4* __restrict__ dbgV, float4* __restrict__ dbgF, uint32_t* __restrict__ flags) {
-    const uint32_t gt = (uint32_t)batch_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const BatchLane t = batch_lane(G, gt);
     if (t.i < 0) return;
     const StepConsts& C = G.inst_table[t.b];
@@ -333,13 +328,13 @@ __global__ __launch_bounds__(256) void naive_force_batch_kernel(
     const float4 p = pos[i], v = vel[i];
     PairSums S;
     switch (t.lanes) {
-        case 1: S = naive_pair_sums_lanes<false, 1>(C, i, 0u, G.n, p, v, pos, vel, cs, nl, nn); break;
-        case 2: S = naive_pair_sums_lanes<false, 2>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
-        case 4: S = naive_pair_sums_lanes<false, 4>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
-        case 8: S = naive_pair_sums_lanes<false, 8>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
-        case 16: S = naive_pair_sums_lanes<false, 16>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
-        case 32: S = naive_pair_sums_lanes<false, 32>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
-        default: S = naive_pair_sums_lanes<false, 64>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
+        case 1: S = naive_pair_sums_lanes<1>(C, i, 0u, G.n, p, v, pos, vel, cs, nl, nn); break;
+        case 2: S = naive_pair_sums_lanes<2>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
+        case 4: S = naive_pair_sums_lanes<4>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
+        case 8: S = naive_pair_sums_lanes<8>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
+        case 16: S = naive_pair_sums_lanes<16>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
+        case 32: S = naive_pair_sums_lanes<32>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
+        default: S = naive_pair_sums_lanes<64>(C, i, t.l, G.n, p, v, pos, vel, cs, nl, nn); break;
     }
     if (t.l != 0) return;
     const float ps = 0.5f * C.mass * C.grad_c;
