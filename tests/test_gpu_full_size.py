@@ -1,0 +1,160 @@
+"""The workloads at their full sizes, through the C ABI, against the FP64 oracle.
+
+The oracle's grid search (identical neighbour sets to the O(N^2) sums, OpenMP) takes well
+under a second per step at these sizes, so the full-size cases are compared element by
+element with the same bars as the small cases: keys, permutation and cell table bit-exact;
+density and positions within 1e-5 relative; the pipe's xoroshiro128+ streams bit-exact.
+The batch is compared with the plain step per instance (bitwise), and a full-size step is
+checked to be deterministic.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from sph_amd import _native as N
+from sph_amd import workloads as W
+from sph_amd.batched import BatchedSPH
+from sph_amd.data_classes import SimulationState
+from sph_amd.strategy import NaiveSPHStrategy
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def f32(a):
+    return np.asarray(a, dtype=np.float64).astype(np.float32).astype(np.float64)
+
+
+def upload(strat, pos, vel):
+    dev = strat.device
+    strat.ctx.upload(torch.from_numpy(np.ascontiguousarray(pos, dtype=np.float64)).to(dev),
+                     torch.from_numpy(np.ascontiguousarray(vel, dtype=np.float64)).to(dev))
+
+
+def read(strat, which):
+    return strat.ctx.read_buffer(which).cpu().numpy()
+
+
+def oracle_steps(w, pos, vel, steps, mode="BOX", rng=None):
+    pipe = w.params.pipe.to_numpy() if mode == "PIPE" else None
+    rho = None
+    for _ in range(steps):
+        pos, vel, rho, _, rng = oracle.step(pos, vel, consts=w.constants, ext=w.params.external_force,
+                                            dt=1.0 / w.params.fps, space=w.params.space_size,
+                                            voxel=w.params.voxel_size, mode=mode, pipe=pipe, rng=rng,
+                                            use_grid=True)
+    return pos, vel, rho, rng
+
+
+def test_dam_break_1m_sort_bit_exact(gpu):
+    """voxel_kernels.py:88-105 keys, voxel_sph_strategy.py:84-88 order and :92-107 voxel_begin
+    for all 1,048,576 particles and 4,194,304 cells."""
+    w = W.dam_break_1m()
+    pos = f32(w.state.position)
+    strat = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu)
+    upload(strat, pos, np.zeros_like(pos))
+    dims = oracle.space_dims(w.params.space_size, w.params.voxel_size, True)
+    keys = oracle.voxel_keys(pos, w.params.voxel_size, dims)
+    assert np.array_equal(read(strat, N.BUF_KEYS), keys)
+    strat.ctx.run_stage(N.STAGE_SORT)
+    perm = oracle.sort_map(keys)
+    assert np.array_equal(read(strat, N.BUF_PERM), perm)
+    assert np.array_equal(read(strat, N.BUF_SORTED_KEYS), keys[perm])
+    strat.ctx.run_stage(N.STAGE_TABLE)
+    begin = oracle.voxel_begin(keys[perm], int(np.prod(dims)))
+    assert np.array_equal(read(strat, N.BUF_CELL_BEGIN), begin)
+
+
+def test_dam_break_1m_steps_match_oracle(gpu):
+    """Three steps of the 1M dam break: every particle's density and position within 1e-5
+    relative of the FP64 oracle, and the second upload of the same state gives the same
+    bits (the step is deterministic)."""
+    w = W.dam_break_1m()
+    pos, vel = f32(w.state.position), f32(w.state.velocity)
+    strat = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu)
+    out = strat.advance(SimulationState(pos, vel, np.zeros(len(pos))), 3)
+    again = strat.advance(SimulationState(pos, vel, np.zeros(len(pos))), 3)
+    assert np.array_equal(out.position, again.position) and np.array_equal(out.velocity, again.velocity)
+    rp, rv, rrho, _ = oracle_steps(w, pos, vel, 3)
+    drho = np.abs(out.density - rrho) / rrho
+    dpos = np.abs(out.position - rp) / np.maximum(np.abs(rp), 1e-3)
+    print(f"dam_break_1m 3 steps: rho {drho.max():.2e} pos {dpos.max():.2e}")
+    assert drho.max() <= TOL and dpos.max() <= TOL
+    assert strat.diagnostics()["non_finite"] == 0
+
+
+def test_pipe_512k_step_matches_oracle(gpu):
+    """One step of the 524,288-particle pipe flow, with 400 particles sent through the outlet
+    (re-injected at the inlet from their own streams) and 200 through the inlet (mirrored):
+    streams bit-exact; density within 1e-5 relative; positions within 1e-5 relative except
+    where that measure or the reference's arithmetic is ill-conditioned.  Its wall collision
+    backtracks by d / (sin a + 0.001) (util_kernels.py:113-121), which multiplies the FP32
+    rounding of the wall distance d by up to 1000 for grazing particles (two steps of this
+    workload reach 9e-4), so particles ending within 3 mm of the wall are held to 1e-5
+    absolute after one step; re-injected particles are placed from FP64 draws
+    (r = R sqrt(u), theta = 2 pi u) and are held to 1e-4, as in the small pipe case; mirrored
+    particles end a few mm from x = 0, where a relative bar on x measures the force's error
+    relative to a tiny coordinate, so they are held to 1e-6 absolute."""
+    w = W.pipe_flow()
+    pos, vel = f32(w.state.position), f32(w.state.velocity)
+    pos[:400, 0] = 3.9995
+    vel[:400, 0] = 5.0
+    pos[400:600, 0] = 0.0005
+    vel[400:600, 0] = -3.0
+    strat = NaiveSPHStrategy(w.params, constants=w.constants, mode="PIPE", device=gpu)
+    out = strat.advance(SimulationState(pos, vel, np.zeros(len(pos))), 1)
+    rng0 = oracle.rng_init(16435234, len(pos))
+    rp, rv, rrho, rr = oracle_steps(w, pos, vel, 1, mode="PIPE", rng=rng0)
+    assert np.array_equal(read(strat, N.BUF_RNG).view(np.uint64), rr)
+    reinjected = np.any(rr.reshape(-1, 2) != rng0.reshape(-1, 2), axis=1)
+    assert reinjected.sum() >= 300
+    mirrored = np.zeros(len(pos), dtype=bool)
+    mirrored[400:600] = True
+    tbl = w.params.pipe.to_numpy()
+    r = np.linalg.norm(rp[:, 1:3] - tbl[0, 1:3], axis=1)
+    wall = r > W.pipe_radius_np(tbl, rp[:, 0]) - 3e-3
+    rest = ~wall & ~reinjected & ~mirrored
+    drho = np.abs(out.density - rrho) / rrho
+    dpos = np.abs(out.position - rp) / np.maximum(np.abs(rp), 1e-3)
+    dabs = np.abs(out.position - rp).max(axis=1)
+    print(f"pipe_512k 1 step: rho {drho.max():.2e}; pos away from the wall {dpos[rest].max():.2e} "
+          f"({rest.sum()} particles), at the wall {dabs[wall].max():.2e} ({wall.sum()}), "
+          f"re-injected {dabs[reinjected].max():.2e} ({reinjected.sum()}), mirrored {dabs[mirrored].max():.2e}")
+    assert drho.max() <= TOL
+    assert dpos[rest].max() <= TOL
+    assert dabs[wall].max() <= 1e-5
+    assert dabs[reinjected].max() <= 1e-4
+    assert dabs[mirrored].max() <= 1e-6
+
+
+def test_batch_64_equals_plain_per_instance(gpu):
+    """The 64-instance batch (1.1M particles) with per-instance grids and lanes: the densest,
+    the largest and the smallest instance are bitwise the plain step with the same settings,
+    and every particle stays finite and inside its box."""
+    bw = W.batch_64(steps=300)
+    inst = [(x.params, x.constants) for x in bw.instances]
+    states = [x.state for x in bw.instances]
+    # neighbours per particle of the starting block (particles uniform in [0.1, 0.5]^3 * edge)
+    load = np.array([x.params.particle_count / (0.4 * x.params.space_size[0]) ** 3 * 4.0 / 3.0 * np.pi
+                     * x.constants.h ** 3 for x in bw.instances])
+    lanes = [16 if v >= 500 else (4 if v >= 50 else 1) for v in load]
+    scales = [0.5 if v >= 500 else 1.0 for v in load]
+    assert len(set(lanes)) == 3
+    batch = BatchedSPH(inst, device=gpu, cell_scale=scales, lanes=lanes)
+    batch.upload(states)
+    batch.step(2)
+    got = batch.download()
+    sizes = [x.params.particle_count for x in bw.instances]
+    picks = {int(np.argmax(load)), int(np.argmax(sizes)), int(np.argmin(sizes))}
+    for b in sorted(picks):
+        w = bw.instances[b]
+        plain = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu,
+                                 grid_voxel=W.voxel_for(w.constants.h) * scales[b], lanes=lanes[b])
+        ref = plain.advance(w.state, 2)
+        assert np.array_equal(got[b].position, ref.position), b
+        assert np.array_equal(got[b].velocity, ref.velocity), b
+    for st, w in zip(got, bw.instances):
+        assert np.all(np.isfinite(st.position))
+        assert np.all(st.position >= 0) and np.all(st.position <= np.asarray(w.params.space_size))
