@@ -1,7 +1,9 @@
-// sph_kernels.hip -- gfx950 kernels of one SPH step (everything but the radix sort).
+// sph_kernels.hip -- gfx950 kernels of one SPH step (everything but the sorts).
 //
-// One thread per particle, particles in cell order.  A step is
-//   sort (sph_radix.hip, last pass gathers posB/velB -> posA/velA in cell order)
+// Particles in cell order; one lane (or a group of L lanes) per particle.  A step is
+//   sort         NAIVE: counting sort by cell (sph_cellsort.hip), which also builds the cell
+//                table; VOXEL: radix sort (sph_radix.hip) then cell_table.  The sort's last
+//                pass gathers posB/velB -> posA/velA in cell order
 //   cell_table   cs[c] = first sorted index with key >= c (dense, no empty markers)
 //   density      rho, p = K (rho - rho0) into posA.w and 1/rho into velA.w, neighbour lists
 //   force        pressure + viscosity sums, symplectic Euler, wall collision, then the

@@ -1,4 +1,6 @@
-// sph_radix.hip -- stable LSD radix sort of (voxel key, slot) pairs for gfx950.
+// sph_radix.hip -- stable LSD radix sort of (voxel key, slot) pairs for gfx950 (the VOXEL
+// formulation, whose keys follow the reference's wrapping semantics and may lie outside the
+// grid; the NAIVE formulation's clamped keys use the counting sort of sph_cellsort.hip).
 //
 // Replaces the host-side sort of the voxel map (voxel_sph_strategy.py:81-89), whose
 // result is the lexicographic order on (voxel_id, particle_id).  The input pairs are
