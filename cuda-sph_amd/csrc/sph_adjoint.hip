@@ -41,13 +41,13 @@ constexpr int kAdjThreads = 256;
 // for a particle with more neighbours than list slots, by rescanning its candidates.
 template <typename F>
 __device__ __forceinline__ void walk_neighbours(const StepConsts& C, int i, float4 p, const float4* __restrict__ pos,
-                                                const uint32_t* __restrict__ cs, const uint32_t* __restrict__ nl,
+                                                const uint32_t* __restrict__ cs, const ListEntry* __restrict__ nl,
                                                 const uint32_t* __restrict__ nn, F&& f) {
     const uint32_t cnt = nn[i];
     if (cnt <= (uint32_t)kListCap) {
-        const uint32_t* slot = nl + i;
+        const ListEntry* slot = nl + i;
         const size_t stride = (size_t)C.n;
-        for (uint32_t t = 0; t < cnt; ++t, slot += stride) f(slot[0]);
+        for (uint32_t t = 0; t < cnt; ++t, slot += stride) f(list_decode(slot[0], i));
     } else {
         for_each_candidate(p.x, p.y, p.z, C, cs, [&](uint32_t k) {
             const float4 q = pos[k];
@@ -62,7 +62,7 @@ __device__ __forceinline__ void walk_neighbours(const StepConsts& C, int i, floa
 template <int L, typename F>
 __device__ __forceinline__ void walk_neighbours_lanes(const StepConsts& C, int i, uint32_t l, float4 p,
                                                       const float4* __restrict__ pos, const uint32_t* __restrict__ cs,
-                                                      const uint32_t* __restrict__ nl, const uint32_t* __restrict__ nn,
+                                                      const ListEntry* __restrict__ nl, const uint32_t* __restrict__ nn,
                                                       F&& f) {
     if constexpr (L == 1) {
         walk_neighbours(C, i, p, pos, cs, nl, nn, f);
@@ -70,7 +70,7 @@ __device__ __forceinline__ void walk_neighbours_lanes(const StepConsts& C, int i
         const uint32_t cnt = nn[i];
         if (cnt <= (uint32_t)kListCap) {
             const size_t stride = (size_t)C.n;
-            for (uint32_t t = l; t < cnt; t += L) f(nl[t * stride + i]);
+            for (uint32_t t = l; t < cnt; t += L) f(list_decode(nl[t * stride + i], i));
         } else {
             const float hs = C.h_search;
             const float hs2 = hs * hs;
@@ -110,7 +110,7 @@ __device__ __forceinline__ float comp(const float4& a, int d) { return d == 0 ? 
 template <int L>
 __global__ __launch_bounds__(kAdjThreads) void adjoint_seed_kernel(
     StepConsts C, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
-    const float* __restrict__ rhoA, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ nl,
+    const float* __restrict__ rhoA, const uint32_t* __restrict__ cs, const ListEntry* __restrict__ nl,
     const uint32_t* __restrict__ nn, float4* __restrict__ posOut, float4* __restrict__ velOut,
     uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
     uint32_t* __restrict__ flags, const float4* __restrict__ cotx, const float4* __restrict__ cotv,
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_seed_kernel(
 template <int L>
 __global__ __launch_bounds__(kAdjThreads) void adjoint_pair_kernel(
     StepConsts C, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
-    const uint32_t* __restrict__ cs, const uint32_t* __restrict__ nl, const uint32_t* __restrict__ nn,
+    const uint32_t* __restrict__ cs, const ListEntry* __restrict__ nl, const uint32_t* __restrict__ nn,
     const float4* __restrict__ fb, float4* __restrict__ gxs, const float4* __restrict__ gvs, float* __restrict__ rt,
     float4* __restrict__ cotv) {
     const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_pair_kernel(
 template <int L>
 __global__ __launch_bounds__(kAdjThreads) void adjoint_density_kernel(
     StepConsts C, const float4* __restrict__ pos, const uint32_t* __restrict__ pidA, const uint32_t* __restrict__ cs,
-    const uint32_t* __restrict__ nl, const uint32_t* __restrict__ nn, const float* __restrict__ rt,
+    const ListEntry* __restrict__ nl, const uint32_t* __restrict__ nn, const float* __restrict__ rt,
     const float4* __restrict__ gxs, float4* __restrict__ cotx) {
     const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int k = (int)(gt / L);

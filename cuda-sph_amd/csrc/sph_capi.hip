@@ -101,7 +101,7 @@ Layout plan_layout(const sph_ctx* c) {
     sz[S_RNG] = c->prm.mode == SPH_MODE_PIPE ? 16 * nn : 16;
     sz[S_DBGP] = sz[S_DBGV] = sz[S_DBGF] = c->keep_terms ? 16 * nn : 16;
     sz[S_NBR] = c->prm.formulation == SPH_FORM_VOXEL ? 4 * 33 * nn : 16;
-    sz[S_NL] = c->prm.formulation == SPH_FORM_NAIVE ? 4 * (size_t)sph::kListCap * nn : 16;
+    sz[S_NL] = c->prm.formulation == SPH_FORM_NAIVE ? sizeof(sph::ListEntry) * (size_t)sph::kListCap * nn : 16;
     sz[S_NN] = 4 * nn;
     sz[S_DIAG] = 8 * 8 * 512;
     sz[S_FLAGS] = 64;
@@ -452,7 +452,7 @@ int sph_bind_workspace(sph_ctx* c, void* dev, size_t bytes, void* stream) {
     b.dbgV = (float4*)(base + L.off[S_DBGV]);
     b.dbgF = (float4*)(base + L.off[S_DBGF]);
     b.nbr = (int32_t*)(base + L.off[S_NBR]);
-    b.nl = (uint32_t*)(base + L.off[S_NL]);
+    b.nl = (sph::ListEntry*)(base + L.off[S_NL]);
     b.nn = (uint32_t*)(base + L.off[S_NN]);
     b.diag_partials = (double*)(base + L.off[S_DIAG]);
     b.flags = (uint32_t*)(base + L.off[S_FLAGS]);

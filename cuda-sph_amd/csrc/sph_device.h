@@ -325,6 +325,10 @@ __device__ __forceinline__ void finish_particle(const StepConsts& C, int i, floa
     hash_particle<SLOT_KEYS>(C, pp.x, pp.y, pp.z, pid, (uint32_t)i, keyI, valI, flags);
 }
 
+// Neighbour-list entries (ListEntry): the neighbour's index relative to the particle's own.
+__device__ __forceinline__ ListEntry list_encode(uint32_t k, int i) { return (ListEntry)(uint16_t)((int)k - i); }
+__device__ __forceinline__ uint32_t list_decode(ListEntry e, int i) { return (uint32_t)(i + (int)(int16_t)e); }
+
 // Pressure and viscosity sums of one pair (naive_kernels.py:38-52, :70-77) without the
 // constant factors.  q = (x, y, z, p) and vj = (vx, vy, vz, 1/rho) of the neighbour.
 struct PairSums {
@@ -362,14 +366,14 @@ constexpr int kForceUnroll = SPH_FORCE_UNROLL;
 // particle itself; PairSums gives that pair a zero contribution.
 __device__ __forceinline__ PairSums naive_pair_sums(const StepConsts& C, int i, int n, float4 p, float4 v,
                                                     const float4* __restrict__ pos, const float4* __restrict__ vel,
-                                                    const uint32_t* __restrict__ cs, const uint32_t* __restrict__ nl,
+                                                    const uint32_t* __restrict__ cs, const ListEntry* __restrict__ nl,
                                                     const uint32_t* __restrict__ nn) {
     const __amdgpu_buffer_rsrc_t rpos = make_rsrc(pos, (uint32_t)n * 16u);
     const __amdgpu_buffer_rsrc_t rvel = make_rsrc(vel, (uint32_t)n * 16u);
     PairSums S;
     const uint32_t cnt = nn[i];
     if (cnt <= (uint32_t)kListCap) {
-        const uint32_t* slot = nl + i;
+        const ListEntry* slot = nl + i;
         const size_t stride = (size_t)n;
         uint32_t t = 0;
 #if SPH_FORCE_PREFETCH
@@ -378,7 +382,7 @@ __device__ __forceinline__ PairSums naive_pair_sums(const StepConsts& C, int i, 
         uint32_t kn[kForceUnroll];
         if (cnt >= (uint32_t)kForceUnroll) {
 #pragma unroll
-            for (int u = 0; u < kForceUnroll; ++u) kn[u] = slot[u * stride];
+            for (int u = 0; u < kForceUnroll; ++u) kn[u] = list_decode(slot[u * stride], i);
         }
 #endif
         for (; t + kForceUnroll <= cnt; t += kForceUnroll) {
@@ -390,11 +394,11 @@ __device__ __forceinline__ PairSums naive_pair_sums(const StepConsts& C, int i, 
             slot += kForceUnroll * stride;
             if (t + 2 * kForceUnroll <= cnt) {
 #pragma unroll
-                for (int u = 0; u < kForceUnroll; ++u) kn[u] = slot[u * stride];
+                for (int u = 0; u < kForceUnroll; ++u) kn[u] = list_decode(slot[u * stride], i);
             }
 #else
 #pragma unroll
-            for (int u = 0; u < kForceUnroll; ++u) k[u] = slot[u * stride];
+            for (int u = 0; u < kForceUnroll; ++u) k[u] = list_decode(slot[u * stride], i);
             slot += kForceUnroll * stride;
 #endif
 #pragma unroll
@@ -406,7 +410,7 @@ __device__ __forceinline__ PairSums naive_pair_sums(const StepConsts& C, int i, 
             for (int u = 0; u < kForceUnroll; ++u) S.add(C, p, v, q[u], w[u], k[u] == (uint32_t)i);
         }
         for (; t < cnt; ++t) {
-            const uint32_t k = slot[0];
+            const uint32_t k = list_decode(slot[0], i);
             slot += stride;
             S.add(C, p, v, ld_f4(rpos, k * 16u), ld_f4(rvel, k * 16u), k == (uint32_t)i);
         }
@@ -428,7 +432,7 @@ __device__ __forceinline__ PairSums naive_pair_sums_lanes(const StepConsts& C, i
                                                           float4 v, const float4* __restrict__ pos,
                                                           const float4* __restrict__ vel,
                                                           const uint32_t* __restrict__ cs,
-                                                          const uint32_t* __restrict__ nl,
+                                                          const ListEntry* __restrict__ nl,
                                                           const uint32_t* __restrict__ nn) {
     if constexpr (L == 1) {
         return naive_pair_sums(C, i, n, p, v, pos, vel, cs, nl, nn);
@@ -441,15 +445,15 @@ __device__ __forceinline__ PairSums naive_pair_sums_lanes(const StepConsts& C, i
             const size_t stride = (size_t)n;
 #if SPH_FORCE_PREFETCH
             // the lane's next list slot is loaded while this slot's neighbour is gathered
-            uint32_t kn = l < cnt ? nl[l * stride + i] : 0u;
+            uint32_t kn = l < cnt ? list_decode(nl[l * stride + i], i) : 0u;
             for (uint32_t t = l; t < cnt; t += L) {
                 const uint32_t k = kn;
-                if (t + L < cnt) kn = nl[(t + L) * stride + i];
+                if (t + L < cnt) kn = list_decode(nl[(t + L) * stride + i], i);
                 S.add(C, p, v, ld_f4(rpos, k * 16u), ld_f4(rvel, k * 16u), k == (uint32_t)i);
             }
 #else
             for (uint32_t t = l; t < cnt; t += L) {
-                const uint32_t k = nl[t * stride + i];
+                const uint32_t k = list_decode(nl[t * stride + i], i);
                 S.add(C, p, v, ld_f4(rpos, k * 16u), ld_f4(rvel, k * 16u), k == (uint32_t)i);
             }
 #endif

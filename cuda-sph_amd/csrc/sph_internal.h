@@ -14,11 +14,14 @@
 //   cell_cnt   u32[C]     NAIVE: per-cell counts of the counting sort (kept zero between steps)
 //   rng        u64[2N]    xoroshiro128+ stream per particle id (PIPE mode)
 //   cotx/cotv  float4[N]  reverse mode only: cotangent of the state in particle order
-//   nl/nn      u32[96][N], u32[N]  NAIVE neighbour lists of the density pass, read by the
-//                         force pass.  Column-major: slot t of consecutive particles is
-//                         contiguous.  The density pass appends each lane's first 32 hits to
-//                         a lane-private list in LDS and copies it out slot by slot at the
-//                         end (every store coalesced); hits beyond 32 go straight to HBM.
+//   nl/nn      i16[96][N], u32[N]  NAIVE neighbour lists of the density pass, read by the
+//                         force pass: each entry is the neighbour's index minus the particle's
+//                         own.  Column-major: slot t of consecutive particles is contiguous.
+//                         The density pass appends each lane's first 16 hits to a lane-private
+//                         list in LDS and copies it out slot by slot at the end (every store
+//                         coalesced); hits beyond 16 go straight to HBM.  nn > 96 marks a
+//                         particle whose neighbours the force pass rescans (more than 96, or
+//                         one farther than an int16 offset).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -35,6 +38,11 @@ constexpr int kListCap = 96;        // NAIVE neighbour-list slots per particle; 
 #endif
 constexpr int kLdsListCap = SPH_LDS_LIST_CAP;  // of which staged in LDS by the density pass
 constexpr int kXcds = 8;            // MI355X: 8 accelerator complex dies, one L2 each
+
+// NAIVE neighbour-list entry: the neighbour's cell-ordered index minus the particle's own, as
+// int16 (neighbours in space are near each other in cell order).  A particle with a
+// neighbour farther away than that in cell order is marked for a rescan instead.
+typedef uint16_t ListEntry;
 
 // Everything a step kernel needs, passed by value (fits the kernel-argument segment).
 struct StepConsts {
@@ -95,7 +103,7 @@ struct Buffers {
     uint64_t* rng;
     float4 *dbgP, *dbgV, *dbgF;
     int32_t* nbr;          // VOXEL formulation neighbour lists, N x (1 + 32)
-    uint32_t* nl;          // NAIVE neighbour lists, column-major [kListCap][N] (cell-ordered indices, self excluded)
+    ListEntry* nl;         // NAIVE neighbour lists, column-major [kListCap][N] (index offsets, self included)
     uint32_t* nn;          // NAIVE neighbour count per cell-ordered particle (may exceed kListCap)
     double* diag_partials;
     uint32_t* flags;       // [0] count of non-finite positions seen by the hash

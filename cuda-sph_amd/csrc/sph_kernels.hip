@@ -139,7 +139,7 @@ template <int L>
 __device__ __forceinline__ void density_particle(const StepConsts& G, const StepConsts& C, int i, uint32_t l,
                                                  uint32_t col, uint32_t gbase, float4* pos, float4* vel,
                                                  const uint32_t* __restrict__ cs, float* __restrict__ rhoA,
-                                                 uint32_t* __restrict__ nl, uint32_t* __restrict__ nn,
+                                                 ListEntry* __restrict__ nl, uint32_t* __restrict__ nn,
                                                  DensityList* s_list) {
     const __amdgpu_buffer_rsrc_t rpos = make_rsrc(pos, (uint32_t)G.n * 16u);
     const __amdgpu_buffer_rsrc_t rcs = make_rsrc(cs, ((uint32_t)G.cs_cells + 1u) * 4u);
@@ -147,6 +147,9 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
     const size_t stride = (size_t)G.n;
     float acc = 0.0f;
     uint32_t cnt = 0;
+    // first and one past the last candidate: every list entry fits an int16 offset when the
+    // candidates do (checked once per particle, not per hit)
+    uint32_t klo = 0xffffffffu, khi = 0u;
     const float hs = C.h_search;
     const float hs2 = hs * hs;
     const int ylo = grid_coord(p.y - hs, 1, C), yhi = grid_coord(p.y + hs, 1, C);
@@ -158,7 +161,7 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
         acc += t * t * t;
         if (r2 <= C.h2) {
             if (cnt < (uint32_t)kLdsListCap) s_list[cnt][col] = k;
-            else if (cnt < (uint32_t)kListCap) nl[cnt * stride + i] = k;
+            else if (cnt < (uint32_t)kListCap) nl[cnt * stride + i] = list_encode(k, i);
             ++cnt;
         }
     };
@@ -173,6 +176,8 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
             const uint32_t row = (uint32_t)(C.cell_off + C.dims[0] * (cy + C.dims[1] * cz));
             const uint32_t k0 = ld_u32(rcs, (row + (uint32_t)xlo) * 4u);
             const uint32_t k1 = ld_u32(rcs, (row + (uint32_t)xhi + 1u) * 4u);
+            klo = min(klo, k0);
+            khi = max(khi, k1);
             if constexpr (L == 1) {
                 uint32_t off = k0 * 16u;
                 const uint32_t end = k1 * 16u;
@@ -199,7 +204,7 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
                     if (hit) {
                         const uint32_t slot = cnt + (uint32_t)__popcll(gm & ((1ull << l) - 1ull));
                         if (slot < (uint32_t)kLdsListCap) s_list[slot][col] = k;
-                        else if (slot < (uint32_t)kListCap) nl[slot * stride + i] = k;
+                        else if (slot < (uint32_t)kListCap) nl[slot * stride + i] = list_encode(k, i);
                     }
                     cnt += (uint32_t)__popcll(gm);
                 }
@@ -208,10 +213,13 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
     }
 #pragma unroll
     for (int o = 1; o < L; o <<= 1) acc += __shfl_xor(acc, o);
+    // the lanes of a group walk the same rows, so they agree on this
+    const bool wide = klo < khi && (klo + 32768u < (uint32_t)i || khi > (uint32_t)i + 32768u);
     const uint32_t staged = cnt < (uint32_t)kLdsListCap ? cnt : (uint32_t)kLdsListCap;
-    for (uint32_t t = l; t < staged; t += L) nl[t * stride + i] = s_list[t][col];
+    for (uint32_t t = l; t < staged; t += L) nl[t * stride + i] = list_encode(s_list[t][col], i);
     if (l == 0) {
-        nn[i] = cnt;
+        // a particle with a far neighbour is rescanned by the force pass, as one past the cap
+        nn[i] = wide && cnt <= (uint32_t)kListCap ? (uint32_t)kListCap + 1u : cnt;
         store_density(C, i, acc, pos, vel, rhoA);
     }
 }
@@ -222,7 +230,7 @@ template <int L>
 __global__ __launch_bounds__(kDensityThreads) void naive_density_kernel(StepConsts G, float4* pos, float4* vel,
                                                                          const uint32_t* __restrict__ cs,
                                                                          float* __restrict__ rhoA,
-                                                                         uint32_t* __restrict__ nl,
+                                                                         ListEntry* __restrict__ nl,
                                                                          uint32_t* __restrict__ nn) {
     static_assert(L >= 1 && L <= 64 && (L & (L - 1)) == 0, "lanes per particle: 1, 2, 4, ..., 64");
     __shared__ DensityList s_list[kLdsListCap];
@@ -266,7 +274,7 @@ __device__ __forceinline__ BatchLane batch_lane(const StepConsts& G, uint32_t gt
 __global__ __launch_bounds__(kDensityThreads) void naive_density_batch_kernel(StepConsts G, float4* pos, float4* vel,
                                                                                const uint32_t* __restrict__ cs,
                                                                                float* __restrict__ rhoA,
-                                                                               uint32_t* __restrict__ nl,
+                                                                               ListEntry* __restrict__ nl,
                                                                                uint32_t* __restrict__ nn) {
     __shared__ DensityList s_list[kLdsListCap];
     const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -296,7 +304,7 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_batch_kernel(St
 template <int MODE, int L = 1>
 __global__ __launch_bounds__(256) void naive_force_kernel(
     StepConsts G, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
-    const float* __restrict__ rhoA, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ nl,
+    const float* __restrict__ rhoA, const uint32_t* __restrict__ cs, const ListEntry* __restrict__ nl,
     const uint32_t* __restrict__ nn, float4* __restrict__ posOut, float4* __restrict__ velOut,
     uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
     uint64_t* __restrict__ rng, const PipeTable* __restrict__ pipe, float4* __restrict__ dbgP,
@@ -318,7 +326,7 @@ __global__ __launch_bounds__(256) void naive_force_kernel(
 // Batched form of the force pass (box walls): the thread map of naive_density_batch_kernel.
 __global__ __launch_bounds__(256) void naive_force_batch_kernel(
     StepConsts G, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
-    const float* __restrict__ rhoA, const uint32_t* __restrict__ cs, const uint32_t* __restrict__ nl,
+    const float* __restrict__ rhoA, const uint32_t* __restrict__ cs, const ListEntry* __restrict__ nl,
     const uint32_t* __restrict__ nn, float4* __restrict__ posOut, float4* __restrict__ velOut,
     uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
     float4* __restrict__ dbgP, float4* __restrict__ dbgV, float4* __restrict__ dbgF, uint32_t* __restrict__ flags) {

@@ -352,3 +352,27 @@ def test_set_lanes_rejects_bad_values(gpu):
     v = VoxelSPHStrategy(w.params, constants=w.constants, device=gpu)
     with pytest.raises(RuntimeError, match="VOXEL"):
         v.ctx.set_lanes(4)
+
+
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_far_neighbours_in_cell_order_are_rescanned(gpu, lanes):
+    """List entries are int16 offsets in cell order.  In a thin slab (4 x 4 x 0.1 with four
+    cell layers in z) a particle's neighbours in the next layer are ~75,000 slots away, so
+    the density pass marks it for a rescan and the force pass walks its cells instead: the
+    step still matches the FP64 oracle."""
+    n = 300_000
+    rng = np.random.default_rng(31)
+    pos = f32(rng.uniform([0.0, 0.0, 0.0], [4.0, 4.0, 0.1], size=(n, 3)))
+    vel = f32(rng.normal(0.0, 0.1, size=(n, 3)))
+    c = SPHConstants(h=0.0457, mass=0.02, rho0=1000.0, k=3.0, visc=3.5, damp=0.5)
+    vx = 0.03125
+    params = SimulationParameters(n, np.array([0.0, -9.81, 0.0]), 1, 1000, box_uniform(16).params.pipe,
+                                  np.array([4.0, 4.0, 0.1]), np.array([vx, vx, vx]))
+    strat = NaiveSPHStrategy(params, constants=c, device=gpu, lanes=lanes)
+    out = strat.compute_next_state(SimulationState(pos, vel, np.zeros(n)))
+    rp, rv, rrho, _, _ = oracle.step(pos, vel, consts=c, ext=params.external_force, dt=1e-3, space=params.space_size,
+                                     voxel=params.voxel_size, use_grid=True)
+    drho = np.abs(out.density - rrho) / rrho
+    dpos = np.abs(out.position - rp) / np.maximum(np.abs(rp), 1e-3)
+    print(f"slab lanes {lanes}: rho {drho.max():.2e} pos {dpos.max():.2e}")
+    assert drho.max() <= TOL and dpos.max() <= TOL
