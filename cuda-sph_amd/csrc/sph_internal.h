@@ -17,9 +17,9 @@
 //   nl/nn      i16[96][N], u32[N]  NAIVE neighbour lists of the density pass, read by the
 //                         force pass: each entry is the neighbour's index minus the particle's
 //                         own.  Column-major: slot t of consecutive particles is contiguous.
-//                         The density pass appends each lane's first 16 hits to a lane-private
-//                         list in LDS and copies it out slot by slot at the end (every store
-//                         coalesced); hits beyond 16 go straight to HBM.  nn > 96 marks a
+//                         The density pass appends each lane's first 32 entries to a
+//                         lane-private list in LDS and copies it out slot by slot at the end
+//                         (every store coalesced); hits beyond 32 go straight to HBM.  nn > 96 marks a
 //                         particle whose neighbours the force pass rescans (more than 96, or
 //                         one farther than an int16 offset).
 #pragma once

@@ -124,7 +124,22 @@ constexpr int kDensityThreads = 256;
 #endif
 constexpr int kDensityUnroll = SPH_DENSITY_UNROLL;
 
+// LDS staging of the first list slots: the entries themselves (16-bit offsets) when
+// SPH_LIST_STAGE16, else full indices.
+#ifndef SPH_LIST_STAGE16
+#define SPH_LIST_STAGE16 1
+#endif
+#if SPH_LIST_STAGE16
+typedef ListEntry DensityList[kDensityThreads];
+constexpr int kStageCap = 2 * kLdsListCap;  // the same LDS bytes hold twice the slots
+__device__ __forceinline__ ListEntry stage_value(uint32_t k, int i) { return list_encode(k, i); }
+__device__ __forceinline__ ListEntry staged_entry(ListEntry v, int) { return v; }
+#else
 typedef uint32_t DensityList[kDensityThreads];
+constexpr int kStageCap = kLdsListCap;
+__device__ __forceinline__ uint32_t stage_value(uint32_t k, int) { return k; }
+__device__ __forceinline__ ListEntry staged_entry(uint32_t k, int i) { return list_encode(k, i); }
+#endif
 
 // Density of cell-ordered particle i with L lanes per particle.  L = 1: the lane walks every
 // candidate row, two candidates in flight through 32-bit buffer offsets.  L = 2..16: a group
@@ -160,7 +175,7 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
         const float t = fmaxf(1.0f - r2 * C.inv_h2, 0.0f);
         acc += t * t * t;
         if (r2 <= C.h2) {
-            if (cnt < (uint32_t)kLdsListCap) s_list[cnt][col] = k;
+            if (cnt < (uint32_t)kStageCap) s_list[cnt][col] = stage_value(k, i);
             else if (cnt < (uint32_t)kListCap) nl[cnt * stride + i] = list_encode(k, i);
             ++cnt;
         }
@@ -203,7 +218,7 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
                     const uint64_t gm = (__ballot(hit) >> gbase) & group;
                     if (hit) {
                         const uint32_t slot = cnt + (uint32_t)__popcll(gm & ((1ull << l) - 1ull));
-                        if (slot < (uint32_t)kLdsListCap) s_list[slot][col] = k;
+                        if (slot < (uint32_t)kStageCap) s_list[slot][col] = stage_value(k, i);
                         else if (slot < (uint32_t)kListCap) nl[slot * stride + i] = list_encode(k, i);
                     }
                     cnt += (uint32_t)__popcll(gm);
@@ -215,8 +230,8 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
     for (int o = 1; o < L; o <<= 1) acc += __shfl_xor(acc, o);
     // the lanes of a group walk the same rows, so they agree on this
     const bool wide = klo < khi && (klo + 32768u < (uint32_t)i || khi > (uint32_t)i + 32768u);
-    const uint32_t staged = cnt < (uint32_t)kLdsListCap ? cnt : (uint32_t)kLdsListCap;
-    for (uint32_t t = l; t < staged; t += L) nl[t * stride + i] = list_encode(s_list[t][col], i);
+    const uint32_t staged = cnt < (uint32_t)kStageCap ? cnt : (uint32_t)kStageCap;
+    for (uint32_t t = l; t < staged; t += L) nl[t * stride + i] = staged_entry(s_list[t][col], i);
     if (l == 0) {
         // a particle with a far neighbour is rescanned by the force pass, as one past the cap
         nn[i] = wide && cnt <= (uint32_t)kListCap ? (uint32_t)kListCap + 1u : cnt;
@@ -233,7 +248,7 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_kernel(StepCons
                                                                          ListEntry* __restrict__ nl,
                                                                          uint32_t* __restrict__ nn) {
     static_assert(L >= 1 && L <= 64 && (L & (L - 1)) == 0, "lanes per particle: 1, 2, 4, ..., 64");
-    __shared__ DensityList s_list[kLdsListCap];
+    __shared__ DensityList s_list[kStageCap];
     const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int i = (int)(gt / L);
     if (i >= G.n) return;
@@ -276,7 +291,7 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_batch_kernel(St
                                                                                float* __restrict__ rhoA,
                                                                                ListEntry* __restrict__ nl,
                                                                                uint32_t* __restrict__ nn) {
-    __shared__ DensityList s_list[kLdsListCap];
+    __shared__ DensityList s_list[kStageCap];
     const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const BatchLane t = batch_lane(G, gt);
     if (t.i < 0) return;
