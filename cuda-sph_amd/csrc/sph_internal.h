@@ -32,7 +32,10 @@ namespace sph {
 
 constexpr int kMaxPipeRows = 64;
 constexpr int kMaxNeighbours = 32;  // MAX_NEIGHBOURS, config.py:30
-constexpr int kListCap = 96;        // NAIVE neighbour-list slots per particle; more falls back to a rescan
+#ifndef SPH_LIST_CAP
+#define SPH_LIST_CAP 96
+#endif
+constexpr int kListCap = SPH_LIST_CAP;  // NAIVE neighbour-list slots per particle; more falls back to a rescan
 #ifndef SPH_LDS_LIST_CAP
 #define SPH_LDS_LIST_CAP 16
 #endif
