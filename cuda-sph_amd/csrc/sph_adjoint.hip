@@ -45,9 +45,7 @@ __device__ __forceinline__ void walk_neighbours(const StepConsts& C, int i, floa
                                                 const uint32_t* __restrict__ nn, F&& f) {
     const uint32_t cnt = nn[i];
     if (cnt <= (uint32_t)kListCap) {
-        const ListEntry* slot = nl + i;
-        const size_t stride = (size_t)C.n;
-        for (uint32_t t = 0; t < cnt; ++t, slot += stride) f(list_decode(slot[0], i));
+        for (uint32_t t = 0; t < cnt; ++t) f(list_decode(nl[list_at(C.n, i, t)], i));
     } else {
         for_each_candidate(p.x, p.y, p.z, C, cs, [&](uint32_t k) {
             const float4 q = pos[k];
@@ -69,8 +67,7 @@ __device__ __forceinline__ void walk_neighbours_lanes(const StepConsts& C, int i
     } else {
         const uint32_t cnt = nn[i];
         if (cnt <= (uint32_t)kListCap) {
-            const size_t stride = (size_t)C.n;
-            for (uint32_t t = l; t < cnt; t += L) f(list_decode(nl[t * stride + i], i));
+            for (uint32_t t = l; t < cnt; t += L) f(list_decode(nl[list_at(C.n, i, t)], i));
         } else {
             const float hs = C.h_search;
             const float hs2 = hs * hs;

@@ -326,8 +326,13 @@ __device__ __forceinline__ void finish_particle(const StepConsts& C, int i, floa
 }
 
 // Neighbour-list entries (ListEntry): the neighbour's index relative to the particle's own.
+// Layout: slots t and t + 1 (t even) of particle i share the 32-bit word (t / 2) * n + i, so
+// a lane reads two slots per load and consecutive particles' words are contiguous.
 __device__ __forceinline__ ListEntry list_encode(uint32_t k, int i) { return (ListEntry)(uint16_t)((int)k - i); }
 __device__ __forceinline__ uint32_t list_decode(ListEntry e, int i) { return (uint32_t)(i + (int)(int16_t)e); }
+__device__ __forceinline__ size_t list_at(int n, int i, uint32_t t) {
+    return ((size_t)(t >> 1) * (size_t)n + (size_t)i) * 2u + (t & 1u);
+}
 
 // Pressure and viscosity sums of one pair (naive_kernels.py:38-52, :70-77) without the
 // constant factors.  q = (x, y, z, p) and vj = (vx, vy, vz, 1/rho) of the neighbour.
@@ -373,46 +378,44 @@ __device__ __forceinline__ PairSums naive_pair_sums(const StepConsts& C, int i, 
     PairSums S;
     const uint32_t cnt = nn[i];
     if (cnt <= (uint32_t)kListCap) {
-        const ListEntry* slot = nl + i;
+        // two slots per 32-bit load (list_at); the next four slots are loaded while this
+        // group's neighbours are gathered, so each group waits on one memory latency
+        static_assert(kForceUnroll == 4, "the force pass walks the list four slots at a time");
+        const uint32_t* words = reinterpret_cast<const uint32_t*>(nl) + i;
         const size_t stride = (size_t)n;
         uint32_t t = 0;
-#if SPH_FORCE_PREFETCH
-        // the next group's list slots are loaded while this group's neighbours are gathered,
-        // so each group waits on one memory latency instead of two
-        uint32_t kn[kForceUnroll];
-        if (cnt >= (uint32_t)kForceUnroll) {
-#pragma unroll
-            for (int u = 0; u < kForceUnroll; ++u) kn[u] = list_decode(slot[u * stride], i);
+        uint32_t wn0 = 0u, wn1 = 0u;
+        if (cnt >= 4u) {
+            wn0 = words[0];
+            wn1 = words[stride];
         }
-#endif
-        for (; t + kForceUnroll <= cnt; t += kForceUnroll) {
-            uint32_t k[kForceUnroll];
-            float4 q[kForceUnroll], w[kForceUnroll];
-#if SPH_FORCE_PREFETCH
-#pragma unroll
-            for (int u = 0; u < kForceUnroll; ++u) k[u] = kn[u];
-            slot += kForceUnroll * stride;
-            if (t + 2 * kForceUnroll <= cnt) {
-#pragma unroll
-                for (int u = 0; u < kForceUnroll; ++u) kn[u] = list_decode(slot[u * stride], i);
+        for (; t + 4u <= cnt; t += 4u) {
+            const uint32_t w0 = wn0, w1 = wn1;
+            words += 2 * stride;
+            if (t + 8u <= cnt) {
+                wn0 = words[0];
+                wn1 = words[stride];
             }
-#else
+            const uint32_t k[4] = {list_decode((ListEntry)w0, i), list_decode((ListEntry)(w0 >> 16), i),
+                                   list_decode((ListEntry)w1, i), list_decode((ListEntry)(w1 >> 16), i)};
+            float4 q[4], w[4];
 #pragma unroll
-            for (int u = 0; u < kForceUnroll; ++u) k[u] = list_decode(slot[u * stride], i);
-            slot += kForceUnroll * stride;
-#endif
-#pragma unroll
-            for (int u = 0; u < kForceUnroll; ++u) {
+            for (int u = 0; u < 4; ++u) {
                 q[u] = ld_f4(rpos, k[u] * 16u);
                 w[u] = ld_f4(rvel, k[u] * 16u);
             }
 #pragma unroll
-            for (int u = 0; u < kForceUnroll; ++u) S.add(C, p, v, q[u], w[u], k[u] == (uint32_t)i);
+            for (int u = 0; u < 4; ++u) S.add(C, p, v, q[u], w[u], k[u] == (uint32_t)i);
         }
-        for (; t < cnt; ++t) {
-            const uint32_t k = list_decode(slot[0], i);
-            slot += stride;
-            S.add(C, p, v, ld_f4(rpos, k * 16u), ld_f4(rvel, k * 16u), k == (uint32_t)i);
+        for (; t < cnt; t += 2u) {  // the last one to three slots, a word at a time
+            const uint32_t wd = words[0];
+            words += stride;
+            const uint32_t k0 = list_decode((ListEntry)wd, i);
+            S.add(C, p, v, ld_f4(rpos, k0 * 16u), ld_f4(rvel, k0 * 16u), k0 == (uint32_t)i);
+            if (t + 1u < cnt) {
+                const uint32_t k1 = list_decode((ListEntry)(wd >> 16), i);
+                S.add(C, p, v, ld_f4(rpos, k1 * 16u), ld_f4(rvel, k1 * 16u), k1 == (uint32_t)i);
+            }
         }
     } else {
         for_each_candidate(p.x, p.y, p.z, C, cs, [&](uint32_t k) {
@@ -442,18 +445,17 @@ __device__ __forceinline__ PairSums naive_pair_sums_lanes(const StepConsts& C, i
         PairSums S;
         const uint32_t cnt = nn[i];
         if (cnt <= (uint32_t)kListCap) {
-            const size_t stride = (size_t)n;
 #if SPH_FORCE_PREFETCH
             // the lane's next list slot is loaded while this slot's neighbour is gathered
-            uint32_t kn = l < cnt ? list_decode(nl[l * stride + i], i) : 0u;
+            uint32_t kn = l < cnt ? list_decode(nl[list_at(n, i, l)], i) : 0u;
             for (uint32_t t = l; t < cnt; t += L) {
                 const uint32_t k = kn;
-                if (t + L < cnt) kn = list_decode(nl[(t + L) * stride + i], i);
+                if (t + L < cnt) kn = list_decode(nl[list_at(n, i, t + L)], i);
                 S.add(C, p, v, ld_f4(rpos, k * 16u), ld_f4(rvel, k * 16u), k == (uint32_t)i);
             }
 #else
             for (uint32_t t = l; t < cnt; t += L) {
-                const uint32_t k = list_decode(nl[t * stride + i], i);
+                const uint32_t k = list_decode(nl[list_at(n, i, t)], i);
                 S.add(C, p, v, ld_f4(rpos, k * 16u), ld_f4(rvel, k * 16u), k == (uint32_t)i);
             }
 #endif

@@ -16,7 +16,8 @@
 //   cotx/cotv  float4[N]  reverse mode only: cotangent of the state in particle order
 //   nl/nn      i16[96][N], u32[N]  NAIVE neighbour lists of the density pass, read by the
 //                         force pass: each entry is the neighbour's index minus the particle's
-//                         own.  Column-major: slot t of consecutive particles is contiguous.
+//                         own.  Slots t and t + 1 (t even) of particle i form the 32-bit word
+//                         (t / 2) * N + i, so consecutive particles' words are contiguous.
 //                         The density pass appends each lane's first 32 entries to a
 //                         lane-private list in LDS and copies it out slot by slot at the end
 //                         (every store coalesced); hits beyond 32 go straight to HBM.  nn > 96 marks a

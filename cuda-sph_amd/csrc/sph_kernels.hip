@@ -176,7 +176,7 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
         acc += t * t * t;
         if (r2 <= C.h2) {
             if (cnt < (uint32_t)kStageCap) s_list[cnt][col] = stage_value(k, i);
-            else if (cnt < (uint32_t)kListCap) nl[cnt * stride + i] = list_encode(k, i);
+            else if (cnt < (uint32_t)kListCap) nl[list_at(G.n, i, cnt)] = list_encode(k, i);
             ++cnt;
         }
     };
@@ -219,7 +219,7 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
                     if (hit) {
                         const uint32_t slot = cnt + (uint32_t)__popcll(gm & ((1ull << l) - 1ull));
                         if (slot < (uint32_t)kStageCap) s_list[slot][col] = stage_value(k, i);
-                        else if (slot < (uint32_t)kListCap) nl[slot * stride + i] = list_encode(k, i);
+                        else if (slot < (uint32_t)kListCap) nl[list_at(G.n, i, slot)] = list_encode(k, i);
                     }
                     cnt += (uint32_t)__popcll(gm);
                 }
@@ -231,7 +231,13 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
     // the lanes of a group walk the same rows, so they agree on this
     const bool wide = klo < khi && (klo + 32768u < (uint32_t)i || khi > (uint32_t)i + 32768u);
     const uint32_t staged = cnt < (uint32_t)kStageCap ? cnt : (uint32_t)kStageCap;
-    for (uint32_t t = l; t < staged; t += L) nl[t * stride + i] = staged_entry(s_list[t][col], i);
+    // the staged slots as 32-bit words (two slots each), the group's lanes taking turns
+    uint32_t* words = reinterpret_cast<uint32_t*>(nl);
+    for (uint32_t t = 2 * l; t < staged; t += 2 * L) {
+        const uint32_t lo = staged_entry(s_list[t][col], i);
+        const uint32_t hi = t + 1 < staged ? staged_entry(s_list[t + 1][col], i) : 0u;
+        words[(size_t)(t >> 1) * stride + i] = lo | (hi << 16);
+    }
     if (l == 0) {
         // a particle with a far neighbour is rescanned by the force pass, as one past the cap
         nn[i] = wide && cnt <= (uint32_t)kListCap ? (uint32_t)kListCap + 1u : cnt;
