@@ -100,9 +100,13 @@ def parse() -> argparse.Namespace:
     return ap.parse_args()
 
 
-def tune_workload(w, mode="BOX", scales=(1.0, 0.5), enabled=True):
+def tune_workload(w, mode="BOX", scales=(1.0, 0.5), enabled=True, steps=None):
     """Autotuned (params with the chosen grid, (density, force) block sizes, lanes per
-    particle, summary) for a workload; the untuned defaults when disabled."""
+    particle, summary) for a workload; the untuned defaults when disabled.  Candidates are
+    timed over the first min(100, run length) steps and the best of each of the three best
+    lane counts again over the whole run: the cost of a step changes as the workload evolves
+    (a 20-step window picked 4 lanes for the pipe flow, which is slower than one over its
+    1000 steps)."""
     import dataclasses
 
     import numpy as np
@@ -111,7 +115,9 @@ def tune_workload(w, mode="BOX", scales=(1.0, 0.5), enabled=True):
         return w.params, None, 1, None
     from sph_amd.autotune import tune_strategy
 
-    res = tune_strategy(w.params, w.constants, w.state, mode=mode, scales=scales, log=log)
+    steps = steps or min(100, int(w.steps))
+    res = tune_strategy(w.params, w.constants, w.state, mode=mode, scales=scales, steps=steps,
+                        final_steps=int(w.steps), log=log)
     gv = res.grid_voxel(w.constants.h)
     params = w.params if gv is None else dataclasses.replace(w.params, voxel_size=np.asarray([gv] * 3))
     info = {"cell_scale": res.cell_scale, "grid_voxel": gv, "density_block": res.density_block,

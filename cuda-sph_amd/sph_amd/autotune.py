@@ -159,19 +159,22 @@ def _time_steps(step, reset, steps: int, warmup: int) -> float:
 
 
 def _search(make, scales: Sequence, blocks: Sequence[int], steps: int, warmup: int,
-            log=None, lanes: Sequence = (1,), finalists: int = 3, cross: bool = True) -> TuneResult:
+            log=None, lanes: Sequence = (1,), finalists: int = 3, cross: bool = True,
+            final_steps: Optional[int] = None) -> TuneResult:
     """make(scale) -> (configure(db, fb, lanes), step(k), reset(), close()) for one grid;
     scale and lanes are opaque keys.  With cross, every (grid, lanes) pair is timed with the
     largest workgroups first; otherwise every grid with the middle lanes key, then every
     lanes key on the fastest grid.  Then every block pair is timed on the fastest of those;
-    the best `finalists` are timed once more and the fastest of that second timing wins (the
-    first timing of a run carries warm-up effects).  The very first timing is repeated and
+    the fastest trial of each of the `finalists` best lanes keys is timed once more, over
+    final_steps steps (default: steps; a longer window than the search's when a workload's
+    cost changes as it evolves), and the fastest of that second timing wins (the first
+    timing of a run also carries warm-up effects).  The very first timing is repeated and
     discarded for the same reason."""
     t0 = time.perf_counter()
     trials: List[tuple] = []
     warmed = [False]
 
-    def run(scale, pairs, record):
+    def run(scale, pairs, record, nsteps=steps):
         try:
             configure, step, reset, close = make(scale)
         except RuntimeError as e:  # e.g. a grid too fine for the planner's limits
@@ -188,9 +191,9 @@ def _search(make, scales: Sequence, blocks: Sequence[int], steps: int, warmup: i
                         log(f"autotune: lanes {ln} rejected: {e}")
                     continue
                 if not warmed[0]:
-                    _time_steps(step, reset, steps, warmup)
+                    _time_steps(step, reset, nsteps, warmup)
                     warmed[0] = True
-                ms = _time_steps(step, reset, steps, warmup)
+                ms = _time_steps(step, reset, nsteps, warmup)
                 out.append((scale, db, fb, ln, round(ms, 5)))
                 if log:
                     log(f"autotune: scale {scale} blocks {db}/{fb} lanes {ln}: {ms:.4f} ms/step")
@@ -215,10 +218,13 @@ def _search(make, scales: Sequence, blocks: Sequence[int], steps: int, warmup: i
         raise RuntimeError("autotune: no candidate could be planned")
     s_best, _, _, l_best, _ = min(trials, key=lambda t: t[4])
     run(s_best, [(db, fb, l_best) for db in blocks for fb in blocks if (db, fb) != (top, top)], True)
-    ranked = sorted(trials, key=lambda t: t[4])[:max(1, finalists)]
+    ranked = []
+    for t in sorted(trials, key=lambda t: t[4]):
+        if len(ranked) < max(1, finalists) and all(r[3] != t[3] for r in ranked):
+            ranked.append(t)
     final = []
     for scale in dict.fromkeys(t[0] for t in ranked):
-        final += run(scale, [t[1:4] for t in ranked if t[0] == scale], False)
+        final += run(scale, [t[1:4] for t in ranked if t[0] == scale], False, final_steps or steps)
     best = min(final or ranked, key=lambda t: t[4])
     return TuneResult(best[0], best[1], best[2], best[4], tuple(trials), time.perf_counter() - t0, False, best[3])
 
@@ -227,11 +233,12 @@ def tune_strategy(params: SimulationParameters, constants: SPHConstants, state: 
                   formulation: str = "NAIVE", steps: int = 20, warmup: int = 3,
                   scales: Sequence[float] = (1.0, 0.5), blocks: Sequence[int] = BLOCKS,
                   lanes: Sequence[int] = LANES, cache: Optional[TuneCache] = None, device="cuda:0",
-                  log=None) -> TuneResult:
-    """Tunes the plain step for one simulation.  VOXEL: block sizes only."""
+                  log=None, final_steps: Optional[int] = None) -> TuneResult:
+    """Tunes the plain step for one simulation.  VOXEL: block sizes only.  final_steps: the
+    window over which the finalists are re-timed (see _search)."""
     from .strategy import NaiveSPHStrategy, VoxelSPHStrategy
 
-    key = signature("plain", [(params, constants, state)], mode, formulation, steps, device)
+    key = signature("plain", [(params, constants, state)], mode, formulation, (steps, final_steps), device)
     if cache is not None:
         hit = cache.get(key)
         if hit is not None:
@@ -250,7 +257,8 @@ def tune_strategy(params: SimulationParameters, constants: SPHConstants, state: 
             strat.ctx.set_lanes(ln)
         return configure, strat.ctx.step, lambda: strat.ctx.upload(pos, vel), strat.ctx.close
 
-    res = _search(make, grid_scales, blocks, steps, warmup, log, lanes if formulation == "NAIVE" else (1,))
+    res = _search(make, grid_scales, blocks, steps, warmup, log, lanes if formulation == "NAIVE" else (1,),
+                  final_steps=final_steps)
     if cache is not None:
         cache.put(key, res)
     return res

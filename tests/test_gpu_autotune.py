@@ -21,8 +21,8 @@ def test_tune_strategy_caches_and_keeps_parity(gpu, tmp_path):
     res = T.tune_strategy(w.params, w.constants, w.state, steps=4, warmup=1, scales=(1.0, 0.5, 0.25),
                           blocks=(64, 256), lanes=(1, 4), cache=cache, device=gpu)
     assert len(res.trials) == 3 * 2 + 3 and not res.cached
-    # the winner is one of the three fastest trials, re-timed
-    finalists = sorted(res.trials, key=lambda t: t[4])[:3]
+    # the winner is the fastest trial of one of the lane counts, re-timed
+    finalists = [min((t for t in res.trials if t[3] == ln), key=lambda t: t[4]) for ln in (1, 4)]
     assert (res.cell_scale, res.density_block, res.force_block, res.lanes) in [t[:4] for t in finalists]
     again = T.tune_strategy(w.params, w.constants, w.state, steps=4, warmup=1, scales=(1.0, 0.5, 0.25),
                             blocks=(64, 256), lanes=(1, 4), cache=cache, device=gpu)
