@@ -69,27 +69,14 @@ __device__ __forceinline__ void walk_neighbours_lanes(const StepConsts& C, int i
         if (cnt <= (uint32_t)kListCap) {
             for (uint32_t t = l; t < cnt; t += L) f(list_decode(nl[list_at(C.n, i, t)], i));
         } else {
-            const float hs = C.h_search;
-            const float hs2 = hs * hs;
-            const int ylo = grid_coord(p.y - hs, 1, C), yhi = grid_coord(p.y + hs, 1, C);
-            const int zlo = grid_coord(p.z - hs, 2, C), zhi = grid_coord(p.z + hs, 2, C);
-            for (int cz = zlo; cz <= zhi; ++cz) {
-                const float ez = slab_dist(p.z, cz, 2, C);
-                for (int cy = ylo; cy <= yhi; ++cy) {
-                    const float ey = slab_dist(p.y, cy, 1, C);
-                    const float rem = hs2 - ez * ez - ey * ey;
-                    if (rem < 0.0f) continue;
-                    const float e = sqrtf(rem);
-                    const int xlo = grid_coord(p.x - e, 0, C), xhi = grid_coord(p.x + e, 0, C);
-                    const int row = C.cell_off + C.dims[0] * (cy + C.dims[1] * cz);
-                    const uint32_t k1 = cs[row + xhi + 1];
-                    for (uint32_t k = cs[row + xlo] + l; k < k1; k += L) {
-                        const float4 q = pos[k];
-                        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-                        if (dx * dx + dy * dy + dz * dz <= C.h2) f(k);
-                    }
+            for_each_row(p.x, p.y, p.z, C, [&](uint32_t c0, uint32_t c1) {
+                const uint32_t k1 = cs[c1];
+                for (uint32_t k = cs[c0] + l; k < k1; k += L) {
+                    const float4 q = pos[k];
+                    const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+                    if (dx * dx + dy * dy + dz * dz <= C.h2) f(k);
                 }
-            }
+            });
         }
     }
 }

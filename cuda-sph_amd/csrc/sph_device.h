@@ -122,32 +122,54 @@ __device__ __forceinline__ void hash_particle(const StepConsts& C, float x, floa
 // infinity (positions outside the grid are clamped into them).  The search radius carries
 // a small margin; the exact test r^2 <= h^2 is applied to each candidate.
 // ---------------------------------------------------------------------------------------
-__device__ __forceinline__ float slab_dist(float p, int c, int d, const StepConsts& C) {
-    const float lo = c == 0 ? -INFINITY : (float)c * C.cell[d];
-    const float hi = c == C.dims[d] - 1 ? INFINITY : (float)(c + 1) * C.cell[d];
-    return fmaxf(fmaxf(lo - p, p - hi), 0.0f);
+// The rows of cells that the sphere of radius h_search around (px, py, pz) reaches, in
+// increasing cell order (cz, then cy): f(c0, c1) gets the cell-table indices of the row's
+// first cell and of the cell after its last, so its candidates are cs[c0] .. cs[c1] - 1.
+// Each row's x range is the chord of the sphere through the nearest point of the row's
+// (y, z) slab.  The position is clamped into the grid first; an edge cell holds the
+// particles beyond it, so a particle outside the grid gets a superset of its rows.  The
+// margin of h_search over h covers the rounding of this float arithmetic (a cell it adds or
+// drops holds only particles beyond h).
+template <typename F>
+__device__ __forceinline__ void for_each_row(float px, float py, float pz, const StepConsts& C, F&& f) {
+    const float hs = C.h_search;
+    const float hs2 = hs * hs;
+    const float mx = (float)(C.dims[0] - 1), my = (float)(C.dims[1] - 1), mz = (float)(C.dims[2] - 1);
+    const float gx = fminf(fmaxf(px, 0.0f), (mx + 1.0f) * C.cell[0]);  // NaN -> 0
+    const float gy = fminf(fmaxf(py, 0.0f), (my + 1.0f) * C.cell[1]);
+    const float gz = fminf(fmaxf(pz, 0.0f), (mz + 1.0f) * C.cell[2]);
+    const int ylo = (int)__builtin_amdgcn_fmed3f((gy - hs) * C.inv_cell[1], 0.0f, my);
+    const int yhi = (int)__builtin_amdgcn_fmed3f((gy + hs) * C.inv_cell[1], 0.0f, my);
+    const int zlo = (int)__builtin_amdgcn_fmed3f((gz - hs) * C.inv_cell[2], 0.0f, mz);
+    const int zhi = (int)__builtin_amdgcn_fmed3f((gz + hs) * C.inv_cell[2], 0.0f, mz);
+    const float sy = C.cell[1], sz = C.cell[2];
+    const float fx = gx * C.inv_cell[0];
+    for (int cz = zlo; cz <= zhi; ++cz) {
+        // distance from the particle to the slab [cz sz, (cz + 1) sz), zero inside it
+        const float zf = (float)cz;
+        const float ez = fmaxf(fmaxf(fmaf(zf, sz, -gz), fmaf(-zf, sz, gz - sz)), 0.0f);
+        const float rz = fmaf(-ez, ez, hs2);
+        int row = C.cell_off + C.dims[0] * (ylo + C.dims[1] * cz);
+        for (int cy = ylo; cy <= yhi; ++cy, row += C.dims[0]) {
+            const float yf = (float)cy;
+            const float ey = fmaxf(fmaxf(fmaf(yf, sy, -gy), fmaf(-yf, sy, gy - sy)), 0.0f);
+            const float rem = fmaf(-ey, ey, rz);
+            if (!(rem >= 0.0f)) continue;
+            const float ex = __builtin_amdgcn_sqrtf(rem) * C.inv_cell[0];
+            const int xlo = (int)__builtin_amdgcn_fmed3f(fx - ex, 0.0f, mx);
+            const int xhi = (int)__builtin_amdgcn_fmed3f(fx + ex, 0.0f, mx);
+            f((uint32_t)(row + xlo), (uint32_t)(row + xhi + 1));
+        }
+    }
 }
 
 template <typename F>
 __device__ __forceinline__ void for_each_candidate(float px, float py, float pz, const StepConsts& C,
                                                    const uint32_t* __restrict__ cs, F&& f) {
-    const float hs = C.h_search;
-    const float hs2 = hs * hs;
-    const int ylo = grid_coord(py - hs, 1, C), yhi = grid_coord(py + hs, 1, C);
-    const int zlo = grid_coord(pz - hs, 2, C), zhi = grid_coord(pz + hs, 2, C);
-    for (int cz = zlo; cz <= zhi; ++cz) {
-        const float dz = slab_dist(pz, cz, 2, C);
-        for (int cy = ylo; cy <= yhi; ++cy) {
-            const float dy = slab_dist(py, cy, 1, C);
-            const float rem = hs2 - dz * dz - dy * dy;
-            if (rem < 0.0f) continue;
-            const float e = sqrtf(rem);
-            const int xlo = grid_coord(px - e, 0, C), xhi = grid_coord(px + e, 0, C);
-            const int row = C.cell_off + C.dims[0] * (cy + C.dims[1] * cz);
-            const uint32_t k0 = cs[row + xlo], k1 = cs[row + xhi + 1];
-            for (uint32_t k = k0; k < k1; ++k) f(k);
-        }
-    }
+    for_each_row(px, py, pz, C, [&](uint32_t c0, uint32_t c1) {
+        const uint32_t k1 = cs[c1];
+        for (uint32_t k = cs[c0]; k < k1; ++k) f(k);
+    });
 }
 
 // ---------------------------------------------------------------------------------------
@@ -460,27 +482,14 @@ __device__ __forceinline__ PairSums naive_pair_sums_lanes(const StepConsts& C, i
             }
 #endif
         } else {
-            const float hs = C.h_search;
-            const float hs2 = hs * hs;
-            const int ylo = grid_coord(p.y - hs, 1, C), yhi = grid_coord(p.y + hs, 1, C);
-            const int zlo = grid_coord(p.z - hs, 2, C), zhi = grid_coord(p.z + hs, 2, C);
-            for (int cz = zlo; cz <= zhi; ++cz) {
-                const float dz0 = slab_dist(p.z, cz, 2, C);
-                for (int cy = ylo; cy <= yhi; ++cy) {
-                    const float dy0 = slab_dist(p.y, cy, 1, C);
-                    const float rem = hs2 - dz0 * dz0 - dy0 * dy0;
-                    if (rem < 0.0f) continue;
-                    const float e = sqrtf(rem);
-                    const int xlo = grid_coord(p.x - e, 0, C), xhi = grid_coord(p.x + e, 0, C);
-                    const int row = C.cell_off + C.dims[0] * (cy + C.dims[1] * cz);
-                    const uint32_t k1 = cs[row + xhi + 1];
-                    for (uint32_t k = cs[row + xlo] + l; k < k1; k += L) {
-                        const float4 q = pos[k];
-                        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-                        if (dx * dx + dy * dy + dz * dz <= C.h2 && k != (uint32_t)i) S.add(C, p, v, q, vel[k]);
-                    }
+            for_each_row(p.x, p.y, p.z, C, [&](uint32_t c0, uint32_t c1) {
+                const uint32_t k1 = cs[c1];
+                for (uint32_t k = cs[c0] + l; k < k1; k += L) {
+                    const float4 q = pos[k];
+                    const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+                    if (dx * dx + dy * dy + dz * dz <= C.h2 && k != (uint32_t)i) S.add(C, p, v, q, vel[k]);
                 }
-            }
+            });
         }
 #pragma unroll
         for (int o = 1; o < L; o <<= 1) {

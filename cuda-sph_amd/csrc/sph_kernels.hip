@@ -165,10 +165,6 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
     // first and one past the last candidate: every list entry fits an int16 offset when the
     // candidates do (checked once per particle, not per hit)
     uint32_t klo = 0xffffffffu, khi = 0u;
-    const float hs = C.h_search;
-    const float hs2 = hs * hs;
-    const int ylo = grid_coord(p.y - hs, 1, C), yhi = grid_coord(p.y + hs, 1, C);
-    const int zlo = grid_coord(p.z - hs, 2, C), zhi = grid_coord(p.z + hs, 2, C);
     auto visit = [&](float4 q, uint32_t k) {
         const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
@@ -180,52 +176,44 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
             ++cnt;
         }
     };
-    for (int cz = zlo; cz <= zhi; ++cz) {
-        const float ez = slab_dist(p.z, cz, 2, C);
-        for (int cy = ylo; cy <= yhi; ++cy) {
-            const float ey = slab_dist(p.y, cy, 1, C);
-            const float rem = hs2 - ez * ez - ey * ey;
-            if (rem < 0.0f) continue;
-            const float e = __builtin_amdgcn_sqrtf(rem);  // within 1 ulp; hs carries the margin
-            const int xlo = grid_coord(p.x - e, 0, C), xhi = grid_coord(p.x + e, 0, C);
-            const uint32_t row = (uint32_t)(C.cell_off + C.dims[0] * (cy + C.dims[1] * cz));
-            const uint32_t k0 = ld_u32(rcs, (row + (uint32_t)xlo) * 4u);
-            const uint32_t k1 = ld_u32(rcs, (row + (uint32_t)xhi + 1u) * 4u);
-            klo = min(klo, k0);
-            khi = max(khi, k1);
-            if constexpr (L == 1) {
-                uint32_t off = k0 * 16u;
-                const uint32_t end = k1 * 16u;
-                for (; off + 16u * (kDensityUnroll - 1) < end; off += 16u * kDensityUnroll) {
-                    float4 q[kDensityUnroll];
+    // candidates k0 .. k1 - 1 of one row
+    auto run_row = [&](uint32_t k0, uint32_t k1) {
+        klo = min(klo, k0);
+        khi = max(khi, k1);
+        if constexpr (L == 1) {
+            uint32_t off = k0 * 16u;
+            const uint32_t end = k1 * 16u;
+            for (; off + 16u * (kDensityUnroll - 1) < end; off += 16u * kDensityUnroll) {
+                float4 q[kDensityUnroll];
 #pragma unroll
-                    for (int u = 0; u < kDensityUnroll; ++u) q[u] = ld_f4(rpos, off + 16u * u);
+                for (int u = 0; u < kDensityUnroll; ++u) q[u] = ld_f4(rpos, off + 16u * u);
 #pragma unroll
-                    for (int u = 0; u < kDensityUnroll; ++u) visit(q[u], (off >> 4) + (uint32_t)u);
+                for (int u = 0; u < kDensityUnroll; ++u) visit(q[u], (off >> 4) + (uint32_t)u);
+            }
+            for (; off < end; off += 16u) visit(ld_f4(rpos, off), off >> 4);
+        } else {
+            for (uint32_t base = k0; base < k1; base += L) {
+                const uint32_t k = base + l;
+                const bool valid = k < k1;
+                const float4 q = ld_f4(rpos, (valid ? k : k0) * 16u);
+                const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+                const float r2 = valid ? dx * dx + dy * dy + dz * dz : INFINITY;
+                const float t = fmaxf(1.0f - r2 * C.inv_h2, 0.0f);
+                acc += t * t * t;
+                const bool hit = r2 <= C.h2;
+                constexpr uint64_t group = L == 64 ? ~0ull : (1ull << L) - 1ull;
+                const uint64_t gm = (__ballot(hit) >> gbase) & group;
+                if (hit) {
+                    const uint32_t slot = cnt + (uint32_t)__popcll(gm & ((1ull << l) - 1ull));
+                    if (slot < (uint32_t)kStageCap) s_list[slot][col] = stage_value(k, i);
+                    else if (slot < (uint32_t)kListCap) nl[list_at(G.n, i, slot)] = list_encode(k, i);
                 }
-                for (; off < end; off += 16u) visit(ld_f4(rpos, off), off >> 4);
-            } else {
-                for (uint32_t base = k0; base < k1; base += L) {
-                    const uint32_t k = base + l;
-                    const bool valid = k < k1;
-                    const float4 q = ld_f4(rpos, (valid ? k : k0) * 16u);
-                    const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-                    const float r2 = valid ? dx * dx + dy * dy + dz * dz : INFINITY;
-                    const float t = fmaxf(1.0f - r2 * C.inv_h2, 0.0f);
-                    acc += t * t * t;
-                    const bool hit = r2 <= C.h2;
-                    constexpr uint64_t group = L == 64 ? ~0ull : (1ull << L) - 1ull;
-                    const uint64_t gm = (__ballot(hit) >> gbase) & group;
-                    if (hit) {
-                        const uint32_t slot = cnt + (uint32_t)__popcll(gm & ((1ull << l) - 1ull));
-                        if (slot < (uint32_t)kStageCap) s_list[slot][col] = stage_value(k, i);
-                        else if (slot < (uint32_t)kListCap) nl[list_at(G.n, i, slot)] = list_encode(k, i);
-                    }
-                    cnt += (uint32_t)__popcll(gm);
-                }
+                cnt += (uint32_t)__popcll(gm);
             }
         }
-    }
+    };
+    for_each_row(p.x, p.y, p.z, C,
+                 [&](uint32_t c0, uint32_t c1) { run_row(ld_u32(rcs, c0 * 4u), ld_u32(rcs, c1 * 4u)); });
 #pragma unroll
     for (int o = 1; o < L; o <<= 1) acc += __shfl_xor(acc, o);
     // the lanes of a group walk the same rows, so they agree on this
