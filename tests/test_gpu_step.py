@@ -128,6 +128,65 @@ def test_density_and_force_stages(gpu):
     assert eP2.max() <= 1e-4
 
 
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_density_of_particles_outside_the_grid(gpu, lanes):
+    """naive_kernels.py:7-21 sums over all pairs, wherever the particles are.  The grid keeps
+    particles beyond its faces in its edge cells and searches from their clamped positions;
+    clusters of mutual neighbours beyond a face, straddling the far face and beyond a corner
+    must get the all-pairs density, and so must the particles inside."""
+    w = box_uniform(4096, seed=15)
+    c = w.constants
+    L = np.asarray(w.params.space_size, dtype=np.float64)
+    pos = f32(w.state.position)
+    rng = np.random.default_rng(15)
+    centres = [np.array([-0.2, 0.3, 0.3]) * L, np.array([L[0], 0.45 * L[1], 0.4 * L[2]]),
+               np.array([-0.1, -0.1, 1.1]) * L, np.array([0.5, 1.0 + c.h / L[1], 0.5]) * L]
+    k = 0
+    for ctr in centres:
+        for _ in range(6):
+            pos[k] = ctr + rng.uniform(-0.25, 0.25, size=3) * c.h  # mates within 0.87 h
+            k += 1
+    pos = f32(pos)
+    strat = NaiveSPHStrategy(w.params, constants=c, device=gpu, lanes=lanes)
+    upload(strat, pos, np.zeros_like(pos))
+    for st in (N.STAGE_SORT, N.STAGE_TABLE, N.STAGE_REORDER, N.STAGE_DENSITY):
+        strat.ctx.run_stage(st)
+    rho = read(strat, N.BUF_DENSITY).astype(np.float64)
+    rho_ref = oracle.naive_density(pos, c)
+    rel = np.abs(rho - rho_ref) / rho_ref
+    print(f"lanes {lanes}: outside clusters rel err {rel[:k].max():.3e}, all {rel.max():.3e}")
+    # every cluster particle has its five mates within h: (1 - r^2/h^2)^3 >= 0.25^3 each
+    w_norm = c.mass * 315.0 / (64.0 * np.pi * c.h ** 3)
+    assert np.all(rho_ref[:k] >= w_norm * (1.0 + 5 * 0.25 ** 3) * (1 - 1e-6))
+    assert rel.max() <= TOL
+
+
+def test_non_finite_particles_are_counted_not_summed(gpu):
+    """naive_kernels.py:18 skips a pair only when norm_squared > h^2, so in the reference a
+    NaN position makes every density NaN.  This build differs on purpose: a non-finite pair
+    adds nothing (the pair test r^2 <= h^2 fails and (1 - r^2/h^2)^3 is clamped at zero), so
+    a NaN or infinite particle has density 0, the finite particles get the all-pairs density
+    of the finite particles alone, and the resident state counts the non-finite ones."""
+    w, pos, vel = _force_case(seed=21)
+    c = w.constants
+    pos[5] = np.nan
+    pos[6, 1] = np.inf
+    pos[7, 2] = -np.inf
+    pos[8, 0] = np.nan
+    bad = ~np.isfinite(pos).all(axis=1)
+    strat = NaiveSPHStrategy(w.params, constants=c, device=gpu)
+    upload(strat, pos, vel)
+    assert strat.diagnostics()["non_finite"] == int(bad.sum()) == 4
+    for st in (N.STAGE_SORT, N.STAGE_TABLE, N.STAGE_REORDER, N.STAGE_DENSITY):
+        strat.ctx.run_stage(st)
+    rho = read(strat, N.BUF_DENSITY).astype(np.float64)
+    assert np.all(rho[bad] == 0.0)
+    rho_ref = oracle.naive_density(pos[~bad], c)
+    rel = np.abs(rho[~bad] - rho_ref) / rho_ref
+    print(f"finite particles' density rel err {rel.max():.3e}")
+    assert rel.max() <= TOL
+
+
 def test_full_step_matches_oracle(gpu):
     """compute_next_state (abstract_sph_strategy.py:31-46), NAIVE, BOX."""
     w, pos, vel = _force_case(seed=14)
