@@ -122,43 +122,68 @@ __device__ __forceinline__ void hash_particle(const StepConsts& C, float x, floa
 // infinity (positions outside the grid are clamped into them).  The search radius carries
 // a small margin; the exact test r^2 <= h^2 is applied to each candidate.
 // ---------------------------------------------------------------------------------------
-// The rows of cells that the sphere of radius h_search around (px, py, pz) reaches, in
-// increasing cell order (cz, then cy): f(c0, c1) gets the cell-table indices of the row's
-// first cell and of the cell after its last, so its candidates are cs[c0] .. cs[c1] - 1.
-// Each row's x range is the chord of the sphere through the nearest point of the row's
-// (y, z) slab.  The position is clamped into the grid first; an edge cell holds the
+// The rows of cells that the sphere of radius h_search around a particle reaches, in
+// increasing cell order (cz, then cy): a row is the run of cells along x that the sphere's
+// chord through the nearest point of the row's (y, z) slab covers, named by the cell-table
+// indices of its first cell and of the cell after its last (its candidates are cs[c0] ..
+// cs[c1] - 1).  The position is clamped into the grid first; an edge cell holds the
 // particles beyond it, so a particle outside the grid gets a superset of its rows.  The
 // margin of h_search over h covers the rounding of this float arithmetic (a cell it adds or
 // drops holds only particles beyond h).
+struct RowGrid {
+    float gx, gy, gz, fx, hs2, mx;
+    int ylo, yhi, zlo, zhi;
+
+    __device__ __forceinline__ RowGrid(float px, float py, float pz, const StepConsts& C) {
+        const float hs = C.h_search;
+        hs2 = hs * hs;
+        mx = (float)(C.dims[0] - 1);
+        const float my = (float)(C.dims[1] - 1), mz = (float)(C.dims[2] - 1);
+        gx = fminf(fmaxf(px, 0.0f), (mx + 1.0f) * C.cell[0]);  // NaN -> 0
+        gy = fminf(fmaxf(py, 0.0f), (my + 1.0f) * C.cell[1]);
+        gz = fminf(fmaxf(pz, 0.0f), (mz + 1.0f) * C.cell[2]);
+        ylo = (int)__builtin_amdgcn_fmed3f((gy - hs) * C.inv_cell[1], 0.0f, my);
+        yhi = (int)__builtin_amdgcn_fmed3f((gy + hs) * C.inv_cell[1], 0.0f, my);
+        zlo = (int)__builtin_amdgcn_fmed3f((gz - hs) * C.inv_cell[2], 0.0f, mz);
+        zhi = (int)__builtin_amdgcn_fmed3f((gz + hs) * C.inv_cell[2], 0.0f, mz);
+        fx = gx * C.inv_cell[0];
+    }
+
+    // distance from the particle to the slab [c s, (c + 1) s), zero inside it
+    __device__ __forceinline__ static float slab(float g, int c, float s) {
+        const float f = (float)c;
+        return fmaxf(fmaxf(fmaf(f, s, -g), fmaf(-f, s, g - s)), 0.0f);
+    }
+
+    // h_search^2 less the squared distance to the slab of cz
+    __device__ __forceinline__ float rest_z(int cz, const StepConsts& C) const {
+        const float ez = slab(gz, cz, C.cell[2]);
+        return fmaf(-ez, ez, hs2);
+    }
+
+    // the row at cy of the slab of cz (row0: the table index of that row's cell at x = 0,
+    // rz: rest_z(cz)); false if the sphere misses the row's slab
+    __device__ __forceinline__ bool row(int cy, int row0, float rz, const StepConsts& C, uint32_t& c0,
+                                        uint32_t& c1) const {
+        const float ey = slab(gy, cy, C.cell[1]);
+        const float rem = fmaf(-ey, ey, rz);
+        if (!(rem >= 0.0f)) return false;
+        const float ex = __builtin_amdgcn_sqrtf(rem) * C.inv_cell[0];
+        c0 = (uint32_t)(row0 + (int)__builtin_amdgcn_fmed3f(fx - ex, 0.0f, mx));
+        c1 = (uint32_t)(row0 + (int)__builtin_amdgcn_fmed3f(fx + ex, 0.0f, mx) + 1);
+        return true;
+    }
+};
+
 template <typename F>
 __device__ __forceinline__ void for_each_row(float px, float py, float pz, const StepConsts& C, F&& f) {
-    const float hs = C.h_search;
-    const float hs2 = hs * hs;
-    const float mx = (float)(C.dims[0] - 1), my = (float)(C.dims[1] - 1), mz = (float)(C.dims[2] - 1);
-    const float gx = fminf(fmaxf(px, 0.0f), (mx + 1.0f) * C.cell[0]);  // NaN -> 0
-    const float gy = fminf(fmaxf(py, 0.0f), (my + 1.0f) * C.cell[1]);
-    const float gz = fminf(fmaxf(pz, 0.0f), (mz + 1.0f) * C.cell[2]);
-    const int ylo = (int)__builtin_amdgcn_fmed3f((gy - hs) * C.inv_cell[1], 0.0f, my);
-    const int yhi = (int)__builtin_amdgcn_fmed3f((gy + hs) * C.inv_cell[1], 0.0f, my);
-    const int zlo = (int)__builtin_amdgcn_fmed3f((gz - hs) * C.inv_cell[2], 0.0f, mz);
-    const int zhi = (int)__builtin_amdgcn_fmed3f((gz + hs) * C.inv_cell[2], 0.0f, mz);
-    const float sy = C.cell[1], sz = C.cell[2];
-    const float fx = gx * C.inv_cell[0];
-    for (int cz = zlo; cz <= zhi; ++cz) {
-        // distance from the particle to the slab [cz sz, (cz + 1) sz), zero inside it
-        const float zf = (float)cz;
-        const float ez = fmaxf(fmaxf(fmaf(zf, sz, -gz), fmaf(-zf, sz, gz - sz)), 0.0f);
-        const float rz = fmaf(-ez, ez, hs2);
-        int row = C.cell_off + C.dims[0] * (ylo + C.dims[1] * cz);
-        for (int cy = ylo; cy <= yhi; ++cy, row += C.dims[0]) {
-            const float yf = (float)cy;
-            const float ey = fmaxf(fmaxf(fmaf(yf, sy, -gy), fmaf(-yf, sy, gy - sy)), 0.0f);
-            const float rem = fmaf(-ey, ey, rz);
-            if (!(rem >= 0.0f)) continue;
-            const float ex = __builtin_amdgcn_sqrtf(rem) * C.inv_cell[0];
-            const int xlo = (int)__builtin_amdgcn_fmed3f(fx - ex, 0.0f, mx);
-            const int xhi = (int)__builtin_amdgcn_fmed3f(fx + ex, 0.0f, mx);
-            f((uint32_t)(row + xlo), (uint32_t)(row + xhi + 1));
+    const RowGrid R(px, py, pz, C);
+    for (int cz = R.zlo; cz <= R.zhi; ++cz) {
+        const float rz = R.rest_z(cz, C);
+        int row0 = C.cell_off + C.dims[0] * (R.ylo + C.dims[1] * cz);
+        for (int cy = R.ylo; cy <= R.yhi; ++cy, row0 += C.dims[0]) {
+            uint32_t c0, c1;
+            if (R.row(cy, row0, rz, C, c0, c1)) f(c0, c1);
         }
     }
 }
