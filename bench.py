@@ -100,9 +100,11 @@ def parse() -> argparse.Namespace:
     return ap.parse_args()
 
 
-def tune_workload(w, mode="BOX", scales=(1.0, 0.5), enabled=True, steps=None):
+def tune_workload(w, mode="BOX", scales=(2.0, 1.0, 0.5), enabled=True, steps=None):
     """Autotuned (params with the chosen grid, (density, force) block sizes, lanes per
-    particle, summary) for a workload; the untuned defaults when disabled.  Candidates are
+    particle, summary) for a workload; the untuned defaults when disabled.  The grid edge is
+    tried at 2, 1 and 1/2 times the largest power of two <= h (the coarse grid suits small
+    problems with many lanes per particle: fewer cells to sort, fewer rows).  Candidates are
     timed over the first min(100, run length) steps and the best of each of the three best
     lane counts again over the whole run: the cost of a step changes as the workload evolves
     (a 20-step window picked 4 lanes for the pipe flow, which is slower than one over its

@@ -252,6 +252,49 @@ __global__ __launch_bounds__(CT) void cell_order_kernel(int n, const uint32_t* _
     }
 }
 
+// Small problems (at most kLdsSortCells cells and kLdsSortMax particles): count, scan and
+// scatter in one workgroup with the counts in LDS, in place of the four kernels above, whose
+// launches dominate at these sizes.  The LDS atomics rank the particles of a cell in an
+// arbitrary order, as the global ones do; cell_order_kernel makes the order.  The global
+// per-cell counts are not used (they stay zero).  Measured on 4,096 particles: 20.2 against
+// 23.1 us per step with 4,096 cells; with 32,768 cells (or 16,384 particles) the one
+// workgroup's sweep costs more than the launches it saves.
+constexpr int LT = 1024;                // threads of the one workgroup
+constexpr int kLdsSortCells = 8192;
+constexpr int kLdsSortMax = 8192;
+constexpr int LWAVES = LT / 64;
+
+__global__ __launch_bounds__(LT) void cell_sort_lds_kernel(int n, int m, const uint32_t* __restrict__ keyI,
+                                                           uint32_t* __restrict__ rank, uint32_t* __restrict__ cs,
+                                                           uint32_t* __restrict__ tmp) {
+    __shared__ uint32_t cnt[kLdsSortCells + kLdsSortCells / 32];  // index c at cpad(c)
+    __shared__ uint32_t lw[LWAVES];
+    for (int c = threadIdx.x; c < m; c += LT) cnt[cpad(c)] = 0u;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += LT) rank[i] = atomicAdd(&cnt[cpad((int)keyI[i])], 1u);
+    __syncthreads();
+    // exclusive scan: thread t owns the cells [t per, (t + 1) per)
+    const int per = (m + LT - 1) / LT;
+    const int c0 = min((int)threadIdx.x * per, m), c1 = min(c0 + per, m);
+    uint32_t sum = 0;
+    for (int c = c0; c < c1; ++c) sum += cnt[cpad(c)];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t incl = wave_incl_scan(sum);
+    if (lane == 63) lw[w] = incl;
+    __syncthreads();
+    uint32_t run = incl - sum;
+    for (int k = 0; k < w; ++k) run += lw[k];
+    for (int c = c0; c < c1; ++c) {
+        const uint32_t v = cnt[cpad(c)];
+        cnt[cpad(c)] = run;
+        run += v;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < m; c += LT) cs[c] = cnt[cpad(c)];
+    if (threadIdx.x == 0) cs[m] = (uint32_t)n;
+    for (int i = threadIdx.x; i < n; i += LT) tmp[cnt[cpad((int)keyI[i])] + rank[i]] = (uint32_t)i;
+}
+
 inline int cblocks(int64_t n) { return (int)((n + CT - 1) / CT); }
 
 }  // namespace
@@ -266,6 +309,20 @@ hipError_t launch_cell_sort(sph_ctx* c, bool gather, hipStream_t s) {
     uint32_t* rank = b.keyS[0];
     uint32_t* tmp = b.valS[0];
     const int tiles = (int)cell_scan_tiles(m);
+    if (m <= kLdsSortCells && n <= kLdsSortMax) {
+        cell_sort_lds_kernel<<<1, LT, 0, s>>>(n, m, b.keyI, rank, b.cs, tmp);
+        if (n > 0) {
+            if (gather)
+                cell_order_kernel<true><<<cblocks(n), CT, 0, s>>>(n, tmp, b.keyI, b.cs, b.keyS[1], b.valS[1], b.posB,
+                                                                   b.velB, b.pidB, b.posA, b.velA, b.pidA);
+            else
+                cell_order_kernel<false><<<cblocks(n), CT, 0, s>>>(n, tmp, b.keyI, b.cs, b.keyS[1], b.valS[1],
+                                                                    nullptr, nullptr, b.pidB, nullptr, nullptr, nullptr);
+        }
+        c->sorted_keys = b.keyS[1];
+        c->sorted_vals = b.valS[1];
+        return hipGetLastError();
+    }
     if (n > 0) cell_count_kernel<<<cblocks(n), CT, 0, s>>>(n, b.keyI, b.cell_cnt, rank);
     cell_scan_reduce<<<tiles, CT, 0, s>>>(b.cell_cnt, m, b.cell_part);
     if (tiles <= kMergedTiles) {

@@ -88,6 +88,35 @@ def test_grid_keys_match_reference_inside_box(gpu):
     assert np.array_equal(read(strat, N.BUF_PERM), np.argsort(ref_keys, kind="stable"))
 
 
+def test_small_grid_sort_in_one_workgroup(gpu):
+    """A problem of at most 8,192 cells and particles is counted, scanned and scattered by one
+    workgroup with the counts in LDS: keys, order and cell table must still be the
+    reference's (voxel_kernels.py:88-105, voxel_sph_strategy.py:84-107) bit for bit, on the
+    grid the step searches (2^-4 here, 4,096 cells), and the step must match the oracle."""
+    w = box_uniform(4096, seed=16)
+    pos = f32(w.state.position)
+    vel = f32(np.random.default_rng(16).normal(0.0, 0.5, size=pos.shape))
+    grid = 2.0 ** -4
+    strat = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu, grid_voxel=grid)
+    upload(strat, pos, vel)
+    voxel = np.array([grid] * 3)
+    dims = oracle.space_dims(w.params.space_size, voxel, True)
+    assert int(np.prod(dims)) == 4096
+    keys = oracle.voxel_keys(pos, voxel, dims)
+    assert np.array_equal(read(strat, N.BUF_KEYS), keys)
+    strat.ctx.run_stage(N.STAGE_SORT)
+    perm = oracle.sort_map(keys)
+    assert np.array_equal(read(strat, N.BUF_PERM), perm)
+    assert np.array_equal(read(strat, N.BUF_SORTED_KEYS), keys[perm])
+    strat.ctx.run_stage(N.STAGE_TABLE)
+    assert np.array_equal(read(strat, N.BUF_CELL_BEGIN), oracle.voxel_begin(keys[perm], int(np.prod(dims))))
+    out = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu, grid_voxel=grid).compute_next_state(
+        SimulationState(pos, vel, np.zeros(len(pos))))
+    rp, rv, rrho, _, _ = oracle_step(w, pos, vel)
+    assert np.max(np.abs(out.density - rrho) / rrho) <= TOL
+    assert np.max(np.abs(out.position - rp) / np.maximum(np.abs(rp), 1e-3)) <= TOL
+
+
 def _force_case(seed=13, n=4096):
     w = box_uniform(n, seed=seed)
     rng = np.random.default_rng(seed)
