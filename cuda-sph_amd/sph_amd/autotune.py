@@ -293,10 +293,9 @@ def neighbour_load(instances: Sequence[Tuple[SimulationParameters, SPHConstants]
 
 
 def lanes_for_load(load: np.ndarray, divisor: float, cap: int = 16) -> Tuple[int, ...]:
-    """Largest power of two in 1..cap not above load / divisor, per instance.  Every lane of a
-    group walks all of the particle's cell rows, so beyond 16 lanes the row walk outgrows the
-    candidates each lane saves (measured on batch_64: 64 lanes for its densest instance made
-    the batch slower)."""
+    """Largest power of two in 1..cap not above load / divisor, per instance.  Beyond 16 lanes
+    a group's lanes outnumber the candidates of most of its rows (measured on batch_64 over its
+    first 100 steps: 64 lanes for the densest instances, 1.76 against 1.42 ms per step)."""
     out = []
     for x in np.asarray(load, dtype=np.float64):
         q = x / divisor if np.isfinite(x) else 1.0
