@@ -69,9 +69,9 @@ __device__ __forceinline__ void walk_neighbours_lanes(const StepConsts& C, int i
         if (cnt <= (uint32_t)kListCap) {
             for (uint32_t t = l; t < cnt; t += L) f(list_decode(nl[list_at(C.n, i, t)], i));
         } else {
-            for_each_row(p.x, p.y, p.z, C, [&](uint32_t c0, uint32_t c1) {
-                const uint32_t k1 = cs[c1];
-                for (uint32_t k = cs[c0] + l; k < k1; k += L) {
+            const __amdgpu_buffer_rsrc_t rcs = make_rsrc(cs, ((uint32_t)C.cell_off + (uint32_t)C.n_cells + 1u) * 4u);
+            for_each_row_range<L>(p.x, p.y, p.z, C, rcs, l, [&](uint32_t k0, uint32_t k1) {
+                for (uint32_t k = k0 + l; k < k1; k += L) {
                     const float4 q = pos[k];
                     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
                     if (dx * dx + dy * dy + dz * dz <= C.h2) f(k);

@@ -188,6 +188,41 @@ __device__ __forceinline__ void for_each_row(float px, float py, float pz, const
     }
 }
 
+// for_each_row for a group of L lanes sharing one particle (groups aligned in the
+// wavefront, l the lane's index in its group): the particle's (cy, cz) pairs are numbered
+// t = 0, 1, ... in for_each_row's order, lane l works out the rows of pairs l, l + L, ...
+// (geometry and cell-table entries), and every lane of the group then gets f(k0, k1) for
+// each row in order, the row's candidate range taken from the lane that owns it.  Rows the
+// sphere misses are skipped; rows without candidates are passed on (k0 == k1).
+template <int L, typename F>
+__device__ __forceinline__ void for_each_row_range(float px, float py, float pz, const StepConsts& C,
+                                                   __amdgpu_buffer_rsrc_t rcs, uint32_t l, F&& f) {
+    const int gbase = (int)((threadIdx.x & 63u) & ~(uint32_t)(L - 1));
+    const RowGrid R(px, py, pz, C);
+    const int ny = R.yhi - R.ylo + 1;
+    const int npairs = ny * (R.zhi - R.zlo + 1);
+    const float inv_ny = 1.0f / (float)ny;
+    for (int t0 = 0; t0 < npairs; t0 += L) {
+        uint32_t a0 = 0xffffffffu, a1 = 0u;  // an empty range unless the row exists
+        const int t = t0 + (int)l;
+        if (t < npairs) {
+            const int q = (int)(((float)t + 0.5f) * inv_ny);  // exact: t and ny are small
+            const int cz = R.zlo + q, cy = R.ylo + (t - q * ny);
+            const int row0 = C.cell_off + C.dims[0] * (cy + C.dims[1] * cz);
+            uint32_t c0, c1;
+            if (R.row(cy, row0, R.rest_z(cz, C), C, c0, c1)) {
+                a0 = ld_u32(rcs, c0 * 4u);
+                a1 = ld_u32(rcs, c1 * 4u);
+            }
+        }
+        const int last = min(L, npairs - t0);
+        for (int j = 0; j < last; ++j) {
+            const uint32_t k0 = __shfl(a0, gbase + j), k1 = __shfl(a1, gbase + j);
+            if (k0 <= k1) f(k0, k1);
+        }
+    }
+}
+
 template <typename F>
 __device__ __forceinline__ void for_each_candidate(float px, float py, float pz, const StepConsts& C,
                                                    const uint32_t* __restrict__ cs, F&& f) {
@@ -507,12 +542,12 @@ __device__ __forceinline__ PairSums naive_pair_sums_lanes(const StepConsts& C, i
             }
 #endif
         } else {
-            for_each_row(p.x, p.y, p.z, C, [&](uint32_t c0, uint32_t c1) {
-                const uint32_t k1 = cs[c1];
-                for (uint32_t k = cs[c0] + l; k < k1; k += L) {
-                    const float4 q = pos[k];
+            const __amdgpu_buffer_rsrc_t rcs = make_rsrc(cs, ((uint32_t)C.cell_off + (uint32_t)C.n_cells + 1u) * 4u);
+            for_each_row_range<L>(p.x, p.y, p.z, C, rcs, l, [&](uint32_t k0, uint32_t k1) {
+                for (uint32_t k = k0 + l; k < k1; k += L) {
+                    const float4 q = ld_f4(rpos, k * 16u);
                     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-                    if (dx * dx + dy * dy + dz * dz <= C.h2 && k != (uint32_t)i) S.add(C, p, v, q, vel[k]);
+                    if (dx * dx + dy * dy + dz * dz <= C.h2 && k != (uint32_t)i) S.add(C, p, v, q, ld_f4(rvel, k * 16u));
                 }
             });
         }

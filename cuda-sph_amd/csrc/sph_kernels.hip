@@ -216,33 +216,8 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
         for_each_row(p.x, p.y, p.z, C,
                      [&](uint32_t c0, uint32_t c1) { run_row(ld_u32(rcs, c0 * 4u), ld_u32(rcs, c1 * 4u)); });
     } else {
-        // The group shares the walk: the particle's (cy, cz) pairs are numbered t = 0, 1, ...
-        // in for_each_row's order, lane l works out the rows of pairs l, l + L, ... (geometry
-        // and cell-table entries), and the group visits every row in order, taking its
-        // candidate range from the lane that owns it.
-        const RowGrid R(p.x, p.y, p.z, C);
-        const int ny = R.yhi - R.ylo + 1;
-        const int npairs = ny * (R.zhi - R.zlo + 1);
-        const float inv_ny = 1.0f / (float)ny;
-        for (int t0 = 0; t0 < npairs; t0 += L) {
-            uint32_t a0 = 0xffffffffu, a1 = 0u;  // an empty range unless the row exists
-            const int t = t0 + (int)l;
-            if (t < npairs) {
-                const int q = (int)(((float)t + 0.5f) * inv_ny);  // exact: t and ny are small
-                const int cz = R.zlo + q, cy = R.ylo + (t - q * ny);
-                const int row0 = C.cell_off + C.dims[0] * (cy + C.dims[1] * cz);
-                uint32_t c0, c1;
-                if (R.row(cy, row0, R.rest_z(cz, C), C, c0, c1)) {
-                    a0 = ld_u32(rcs, c0 * 4u);
-                    a1 = ld_u32(rcs, c1 * 4u);
-                }
-            }
-            const int last = min(L, npairs - t0);
-            for (int j = 0; j < last; ++j) {
-                const uint32_t k0 = __shfl(a0, (int)gbase + j), k1 = __shfl(a1, (int)gbase + j);
-                if (k0 <= k1) run_row(k0, k1);  // rows without candidates still bound klo, khi
-            }
-        }
+        // the group shares the row walk; rows without candidates still bound klo, khi
+        for_each_row_range<L>(p.x, p.y, p.z, C, rcs, l, run_row);
     }
 #pragma unroll
     for (int o = 1; o < L; o <<= 1) acc += __shfl_xor(acc, o);
