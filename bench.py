@@ -17,6 +17,7 @@ Also measured in the same run:
                 the force kernel, and per stage and per step in "kernels"/"step_roofline");
   repeats       the timed region repeated five times, and the median;
   pipe          the pipe workload (524,288 particles, 1000 steps);
+  voxel_formulation  the reference's VOXEL formulation on the same dam break (500 steps);
   box_4k        the reference's CPU-sized case (4,096 particles, 100 steps) on the GPU;
   frames        frame output to disk in the reference's Saver format, synchronous
                 (the reference's loop) against the asynchronous writer;
@@ -88,6 +89,8 @@ def parse() -> argparse.Namespace:
                     help="also measure frame output to disk in the reference's format")
     ap.add_argument("--pipe", choices=["on", "off"], default="on",
                     help="also measure the pipe workload (524,288 particles, 1000 steps)")
+    ap.add_argument("--voxel-form", choices=["on", "off"], default="on",
+                    help="also measure the VOXEL formulation on the dam break (500 steps)")
     ap.add_argument("--small", choices=["on", "off"], default="on",
                     help="also measure box_4k (4,096 particles, 100 steps) on the GPU")
     ap.add_argument("--graph", choices=["on", "off"], default="off",
@@ -349,6 +352,42 @@ def bench_pipe(steps: int = 1000, warmup: int = 5, tune: bool = True, graph: boo
     return {"workload": "pipe_512k", "particles": n, "steps": steps, "value": round(n * steps / (ms / 1e3), 1),
             "unit": "particle-updates/s", "ms_per_step": round(ms / steps, 5), "h": w.constants.h,
             "non_finite": d["non_finite"], "graph": graph, "tuning": tuning}
+
+
+def bench_voxel_form(steps: int = 500, warmup: int = 5):
+    """The reference's VOXEL formulation (voxel_sph_strategy.py: stable radix sort of the
+    voxel keys, neighbour lists of at most 32 from the 27 voxels, its own density and force
+    formulas) on dam_break_1m with voxels of 2^-6 (128 x 64 x 64): its 27-voxel search
+    reaches h only when the voxel edge is at least h (0.01298 here).  Particle-updates/s over
+    500 steps launched one by one."""
+    import dataclasses
+
+    import numpy as np
+    import torch
+
+    from sph_amd import workloads
+    from sph_amd.strategy import VoxelSPHStrategy
+
+    w = workloads.dam_break_1m()
+    w.params = dataclasses.replace(w.params, voxel_size=np.asarray([2.0 ** -6] * 3))
+    n = w.params.particle_count
+    strat = VoxelSPHStrategy(w.params, constants=w.constants)
+    pos = torch.from_numpy(np.ascontiguousarray(w.state.position)).cuda()
+    vel = torch.from_numpy(np.ascontiguousarray(w.state.velocity)).cuda()
+    strat.ctx.upload(pos, vel)
+    strat.ctx.step(warmup)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    strat.ctx.step(steps)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b)
+    d = strat.diagnostics()
+    log(f"voxel formulation: {steps} steps of {n} particles in {ms:.1f} ms")
+    return {"workload": "dam_break_1m", "formulation": "VOXEL", "particles": n, "steps": steps,
+            "voxel": [float(x) for x in w.params.voxel_size], "value": round(n * steps / (ms / 1e3), 1),
+            "unit": "particle-updates/s", "ms_per_step": round(ms / steps, 5), "non_finite": d["non_finite"]}
 
 
 def timed_steps(strat, steps: int, graph: bool, pos, vel):
@@ -621,6 +660,7 @@ def main() -> int:
         log("copy calibration: 5 copies of 256 MiB")
 
     pipe = bench_pipe(tune=args.tune == "on", graph=graph) if args.pipe == "on" else None
+    voxel_form = bench_voxel_form() if args.voxel_form == "on" else None
     frames = bench_frames() if args.frames == "on" else None
     small = bench_small(tune=args.tune == "on") if args.small == "on" else None
     batched = bench_batched(args.batch_steps, tune=args.tune == "on") if args.batched == "on" else None
@@ -647,6 +687,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "tuning": tuning,
         "pipe": pipe,
+        "voxel_formulation": voxel_form,
         "box_4k": small,
         "frames": frames,
         "stages_ms": {k: round(v, 4) for k, v in st.items()},

@@ -372,25 +372,48 @@ __global__ __launch_bounds__(256) void naive_force_batch_kernel(
 // VOXEL formulation (voxel_kernels.py).  get_neighbours (:29-85): 27 cells in (dx, dy, dz)
 // order with dz innermost, bounds from trunc(space/voxel), each cell's range runs to the
 // next non-empty cell (or to N when none follows), accept sqrt(r^2) <= h, self included,
-// stop at 32.  The list is built once per step and reused by the force pass.
+// stop at 32.  The list is built once per step and reused by the force pass.  A cell whose
+// nearest point lies beyond h_search holds no particle within h (h_search's margin covers
+// the rounding of the distance), so skipping it leaves every list as it is; the density is
+// summed while the list is built, in list order.  Lists are column-major: the count of
+// cell-ordered particle i at nbr[i], its t-th neighbour at nbr[(1 + t) N + i].
 // ---------------------------------------------------------------------------------------
 constexpr int kNbrStride = 1 + kMaxNeighbours;
+
+// squared distance from p to the key cells v - 1, v, v + 1 of one axis; cell 0 also holds
+// the positions in (-voxel, 0), which trunc maps to it, so its slab is open below
+__device__ __forceinline__ void voxel_slab2(float p, int v, float s, float e2[3]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int c = v - 1 + d;
+        const float lo = c == 0 ? -INFINITY : (float)c * s;
+        const float e = fmaxf(fmaxf(lo - p, p - (float)(c + 1) * s), 0.0f);
+        e2[d] = e * e;
+    }
+}
 
 __global__ __launch_bounds__(256) void voxel_density_kernel(StepConsts C, float4* pos, float4* vel,
                                                              const uint32_t* __restrict__ cs,
                                                              float* __restrict__ rhoA, int32_t* __restrict__ nbr) {
     const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= C.n) return;
+    const size_t n = (size_t)C.n;
     const float4 p = pos[i];
     const int vx = ref_coord(p.x, 0, C), vy = ref_coord(p.y, 1, C), vz = ref_coord(p.z, 2, C);
-    int32_t* list = nbr + (int64_t)i * kNbrStride + 1;
+    float ex[3], ey[3], ez[3];
+    voxel_slab2(p.x, vx, C.cell[0], ex);
+    voxel_slab2(p.y, vy, C.cell[1], ey);
+    voxel_slab2(p.z, vz, C.cell[2], ez);
+    const float hs2 = C.h_search * C.h_search;
     int cnt = 0;
+    float acc = 0.0f;
     const uint32_t cs_end = cs[C.n_cells];
     for (int dx = -1; dx < 2 && cnt < kMaxNeighbours; ++dx)
         for (int dy = -1; dy < 2 && cnt < kMaxNeighbours; ++dy)
             for (int dz = -1; dz < 2 && cnt < kMaxNeighbours; ++dz) {
                 const int nx = vx + dx, ny = vy + dy, nz = vz + dz;
                 if (nx < 0 || nx >= C.tdims[0] || ny < 0 || ny >= C.tdims[1] || nz < 0 || nz >= C.tdims[2]) continue;
+                if (ex[dx + 1] + ey[dy + 1] + ez[dz + 1] > hs2) continue;
                 const int ci = nx + ny * C.tdims[0] + nz * C.tdims[0] * C.tdims[1];
                 const uint32_t start = cs[ci], next = cs[ci + 1];
                 if (start == next) continue;  // voxel_begin == -1
@@ -398,19 +421,18 @@ __global__ __launch_bounds__(256) void voxel_density_kernel(StepConsts C, float4
                 for (uint32_t e = start; e < end && cnt < kMaxNeighbours; ++e) {
                     const float4 q = pos[e];
                     const float ddx = p.x - q.x, ddy = p.y - q.y, ddz = p.z - q.z;
-                    if (sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) <= C.h) list[cnt++] = (int32_t)e;
+                    const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                    if (sqrtf(r2) <= C.h) {
+                        nbr[(size_t)(1 + cnt) * n + i] = (int32_t)e;
+                        ++cnt;
+                        if (e != (uint32_t)i) {  // voxel_kernels.py:127-130: rho skips the particle itself
+                            const float t = 1.0f - r2 * C.inv_h2;
+                            acc += t * t * t;
+                        }
+                    }
                 }
             }
-    nbr[(int64_t)i * kNbrStride] = cnt;
-    float acc = 0.0f;
-    for (int a = 0; a < cnt; ++a) {
-        const int j = list[a];
-        if (j == i) continue;
-        const float4 q = pos[j];
-        const float ddx = p.x - q.x, ddy = p.y - q.y, ddz = p.z - q.z;
-        const float t = 1.0f - (ddx * ddx + ddy * ddy + ddz * ddz) * C.inv_h2;
-        acc += t * t * t;
-    }
+    nbr[i] = cnt;
     store_density(C, i, acc, pos, vel, rhoA);
 }
 
@@ -424,31 +446,30 @@ __global__ __launch_bounds__(256) void voxel_force_kernel(
     float4* __restrict__ dbgF, uint32_t* __restrict__ flags) {
     const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= C.n) return;
+    const size_t n = (size_t)C.n;
     const float4 p = pos[i], v = vel[i];
-    const int32_t* list = nbr + (int64_t)i * kNbrStride;
-    const int cnt = list[0];
+    const int cnt = nbr[i];
     const float rho_i = rhoA[i];
     const float fi = p.w / (rho_i * rho_i);
+    const float s = C.mass * C.visc / rho_i;
     float ax = 0.f, ay = 0.f, az = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
     for (int t = 0; t < cnt; ++t) {
-        const int j = list[1 + t];
+        const int j = nbr[(size_t)(1 + t) * n + i];
         if (j == i) continue;
-        const float4 q = pos[j], vj = vel[j];
-        const float rho_j = rhoA[j];
+        const float4 q = pos[j], vj = vel[j];  // q.w = p_j, vj.w = 1 / rho_j (store_density)
         const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         const float r = sqrtf(dx * dx + dy * dy + dz * dz);
         const float hr = C.h - r;
         // voxel_kernels.py:162-168: (p_i/rho_i^2 + p_j/rho_j^2) * grad W
-        const float cp = (fi + q.w / (rho_j * rho_j)) * C.grad_c * hr * hr / r;
+        const float cp = (fi + q.w * vj.w * vj.w) * C.grad_c * hr * hr * (1.0f / r);
         ax += cp * dx;
         ay += cp * dy;
         az += cp * dz;
         // voxel_kernels.py:203-208: (v_j - v_i) / rho_j * lap W * m * mu / rho_i
-        const float lap = C.lap_c * hr;
-        const float s = C.mass * C.visc / rho_i;
-        bx += (vj.x - v.x) / rho_j * lap * s;
-        by += (vj.y - v.y) / rho_j * lap * s;
-        bz += (vj.z - v.z) / rho_j * lap * s;
+        const float cv = vj.w * (C.lap_c * hr) * s;
+        bx += (vj.x - v.x) * cv;
+        by += (vj.y - v.y) * cv;
+        bz += (vj.z - v.z) * cv;
     }
     // voxel_kernels.py:211 min(1, .)
     const Vec3 V{fminf(1.0f, bx), fminf(1.0f, by), fminf(1.0f, bz)};
@@ -539,13 +560,13 @@ __global__ __launch_bounds__(256) void read_buffer_kernel(StepConsts C, int whic
             ((float*)out)[3 * id + 2] = t.z;
             break;
         }
-        case SPH_BUF_NEIGHBOURS: {
+        case SPH_BUF_NEIGHBOURS: {  // row-major in particle order: count, then particle ids
             const int64_t id = pidA[i];
-            const int32_t* src = nbr + i * kNbrStride;
             int32_t* dst = (int32_t*)out + id * kNbrStride;
-            const int cnt = src[0];
+            const int cnt = nbr[i];
             dst[0] = cnt;
-            for (int t = 0; t < kMaxNeighbours; ++t) dst[1 + t] = t < cnt ? (int32_t)pidA[src[1 + t]] : -1;
+            for (int t = 0; t < kMaxNeighbours; ++t)
+                dst[1 + t] = t < cnt ? (int32_t)pidA[nbr[(int64_t)(1 + t) * C.n + i]] : -1;
             break;
         }
         default:
