@@ -170,6 +170,14 @@ int plan_instance(const sph_params_t& P, const sph_constants_t& K, sph::StepCons
     S.h = (float)K.h;
     S.h2 = (float)(K.h * K.h);
     S.inv_h2 = (float)(1.0 / (K.h * K.h));
+    // sqrtf is correctly rounded and monotone, so sqrtf(r2) <= h holds exactly for
+    // r2 <= r2_accept: the VOXEL pair test without a square root per candidate
+    {
+        float t = S.h * S.h;
+        while (sqrtf(nextafterf(t, INFINITY)) <= S.h) t = nextafterf(t, INFINITY);
+        while (sqrtf(t) > S.h) t = nextafterf(t, 0.0f);
+        S.r2_accept = t;
+    }
     double maxsp = fmax(P.space_size[0], fmax(P.space_size[1], P.space_size[2]));
     S.h_search = (float)(K.h * (1.0 + 1e-5) + 4e-7 * maxsp);
     S.mass = (float)K.mass;

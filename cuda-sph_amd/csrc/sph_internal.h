@@ -63,6 +63,7 @@ struct StepConsts {
     float cell[3];
     double dcell[3];  // voxel edges as given (used when pow2 == 0)
     float h, h2, inv_h2;
+    float r2_accept;  // VOXEL: the largest FP32 r^2 whose correctly rounded sqrt is <= h
     float h_search;   // h plus a rounding margin, for choosing cells (the pair test uses h2)
     float mass, rho0, k, visc, damp, dt;
     float w_norm;     // mass * 315 / (64 pi h^3): rho = w_norm * sum (1 - r^2/h^2)^3

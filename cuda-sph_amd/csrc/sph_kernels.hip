@@ -415,14 +415,16 @@ __global__ __launch_bounds__(256) void voxel_density_kernel(StepConsts C, float4
                 if (nx < 0 || nx >= C.tdims[0] || ny < 0 || ny >= C.tdims[1] || nz < 0 || nz >= C.tdims[2]) continue;
                 if (ex[dx + 1] + ey[dy + 1] + ez[dz + 1] > hs2) continue;
                 const int ci = nx + ny * C.tdims[0] + nz * C.tdims[0] * C.tdims[1];
-                const uint32_t start = cs[ci], next = cs[ci + 1];
+                uint2 se;  // cs[ci] and cs[ci + 1] in one load
+                __builtin_memcpy(&se, cs + ci, sizeof(se));
+                const uint32_t start = se.x, next = se.y;
                 if (start == next) continue;  // voxel_begin == -1
                 const uint32_t end = next < cs_end ? next : (uint32_t)C.n;
                 for (uint32_t e = start; e < end && cnt < kMaxNeighbours; ++e) {
                     const float4 q = pos[e];
                     const float ddx = p.x - q.x, ddy = p.y - q.y, ddz = p.z - q.z;
                     const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    if (sqrtf(r2) <= C.h) {
+                    if (r2 <= C.r2_accept) {  // sqrtf(r2) <= h, exactly
                         nbr[(size_t)(1 + cnt) * n + i] = (int32_t)e;
                         ++cnt;
                         if (e != (uint32_t)i) {  // voxel_kernels.py:127-130: rho skips the particle itself
