@@ -420,16 +420,25 @@ __global__ __launch_bounds__(256) void voxel_density_kernel(StepConsts C, float4
                 const uint32_t start = se.x, next = se.y;
                 if (start == next) continue;  // voxel_begin == -1
                 const uint32_t end = next < cs_end ? next : (uint32_t)C.n;
-                for (uint32_t e = start; e < end && cnt < kMaxNeighbours; ++e) {
-                    const float4 q = pos[e];
-                    const float ddx = p.x - q.x, ddy = p.y - q.y, ddz = p.z - q.z;
-                    const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    if (r2 <= C.r2_accept) {  // sqrtf(r2) <= h, exactly
-                        nbr[(size_t)(1 + cnt) * n + i] = (int32_t)e;
-                        ++cnt;
-                        if (e != (uint32_t)i) {  // voxel_kernels.py:127-130: rho skips the particle itself
-                            const float t = 1.0f - r2 * C.inv_h2;
-                            acc += t * t * t;
+                // four candidates loaded at a time (the stop at 32 would otherwise hold each
+                // load until the previous test is done), then tested in order
+                for (uint32_t e0 = start; e0 < end && cnt < kMaxNeighbours; e0 += 4) {
+                    float4 q[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) q[u] = pos[min(e0 + (uint32_t)u, end - 1u)];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const uint32_t e = e0 + (uint32_t)u;
+                        if (e >= end || cnt >= kMaxNeighbours) break;
+                        const float ddx = p.x - q[u].x, ddy = p.y - q[u].y, ddz = p.z - q[u].z;
+                        const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        if (r2 <= C.r2_accept) {  // sqrtf(r2) <= h, exactly
+                            nbr[(size_t)(1 + cnt) * n + i] = (int32_t)e;
+                            ++cnt;
+                            if (e != (uint32_t)i) {  // voxel_kernels.py:127-130: rho skips the particle itself
+                                const float t = 1.0f - r2 * C.inv_h2;
+                                acc += t * t * t;
+                            }
                         }
                     }
                 }
