@@ -7,6 +7,8 @@ density and positions within 1e-5 relative; the pipe's xoroshiro128+ streams bit
 The batch is compared with the plain step per instance (bitwise), and a full-size step is
 checked to be deterministic.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 import torch
@@ -16,7 +18,7 @@ from sph_amd import _native as N
 from sph_amd import workloads as W
 from sph_amd.batched import BatchedSPH
 from sph_amd.data_classes import SimulationState
-from sph_amd.strategy import NaiveSPHStrategy
+from sph_amd.strategy import NaiveSPHStrategy, VoxelSPHStrategy
 
 pytestmark = pytest.mark.gpu
 
@@ -83,6 +85,41 @@ def test_dam_break_1m_steps_match_oracle(gpu):
     print(f"dam_break_1m 3 steps: rho {drho.max():.2e} pos {dpos.max():.2e}")
     assert drho.max() <= TOL and dpos.max() <= TOL
     assert strat.diagnostics()["non_finite"] == 0
+
+
+def test_dam_break_1m_voxel_step_matches_oracle(gpu):
+    """One step of the 1M dam break in the VOXEL formulation (voxels of 2^-6, at least h, so
+    the 27-voxel search reaches h): density and positions within 1e-5 relative of the FP64
+    oracle's VOXEL step (voxel_kernels.py: its voxel order, the stop at 32 neighbours and its
+    formulas).  A pair within 1e-5 h of the cut-off may fall either side in FP32 and change
+    a list, which changes that particle's density and, through it, its neighbours' forces: a
+    particle with such a pair is left out of the density check, and it and its neighbours
+    out of the position check (found with a k-d tree)."""
+    from scipy.spatial import cKDTree
+
+    w = W.dam_break_1m()
+    vox = np.array([2.0 ** -6] * 3)
+    params = dataclasses.replace(w.params, voxel_size=vox)
+    pos, vel = f32(w.state.position), f32(w.state.velocity)
+    strat = VoxelSPHStrategy(params, constants=w.constants, device=gpu)
+    out = strat.advance(SimulationState(pos, vel, np.zeros(len(pos))), 1)
+    rp, rv, rrho, _, _ = oracle.step(pos, vel, consts=w.constants, ext=params.external_force,
+                                     dt=1.0 / params.fps, space=params.space_size, voxel=vox,
+                                     formulation="VOXEL")
+    h = w.constants.h
+    d, j = cKDTree(pos).query(pos, k=64, distance_upper_bound=1.5 * h, workers=-1)
+    assert np.all(d[:, -1] > h * (1 + 1e-5)), "more than 63 particles within h of one particle"
+    near = (np.abs(d - h) < 1e-5 * h).any(axis=1)
+    rows, cols = np.nonzero(d <= h * (1 + 1e-5))
+    hop = near.copy()
+    hop[rows[near[j[rows, cols]]]] = True
+    ok_rho, ok_pos = ~near, ~hop
+    drho = np.abs(out.density - rrho)[ok_rho] / np.abs(rrho[ok_rho])
+    dpos = (np.abs(out.position - rp) / np.maximum(np.abs(rp), 1e-3))[ok_pos]
+    print(f"voxel dam_break_1m 1 step: rho {drho.max():.2e} ({ok_rho.sum()} particles), "
+          f"pos {dpos.max():.2e} ({ok_pos.sum()} particles)")
+    assert ok_pos.sum() > 0.9 * len(pos)
+    assert drho.max() <= TOL and dpos.max() <= TOL
 
 
 def test_pipe_512k_step_matches_oracle(gpu):
