@@ -267,14 +267,17 @@ def test_box_collisions_all_walls(gpu):
     assert out.velocity[0, 0] > 0 and out.velocity[1, 0] < 0 and abs(out.velocity[0, 0]) < 2
 
 
-@pytest.mark.parametrize("n", [0, 1, 2, 63, 1000, 2049])
-def test_small_and_ragged_sizes(gpu, n):
+@pytest.mark.parametrize("grid", [None, 2.0 ** -4])
+@pytest.mark.parametrize("n", [0, 1, 2, 63, 1000, 2049, 8192, 8193])
+def test_small_and_ragged_sizes(gpu, n, grid):
+    """Empty, single and ragged sizes; with the 2^-4 grid (4,096 cells) up to 8,192 particles
+    go through the one-workgroup sort and 8,193 through the multi-kernel one."""
     w = box_uniform(max(n, 1), seed=16)
     params = SimulationParameters(n, w.params.external_force, 1, 1000, w.params.pipe, w.params.space_size,
                                   w.params.voxel_size)
     pos = f32(w.state.position[:n])
     vel = np.zeros((n, 3))
-    strat = NaiveSPHStrategy(params, constants=w.constants, device=gpu)
+    strat = NaiveSPHStrategy(params, constants=w.constants, device=gpu, grid_voxel=grid)
     out = strat.advance(SimulationState(pos, vel, np.zeros(n)), 3)
     rp, rv = pos, vel
     for _ in range(3):
