@@ -73,6 +73,30 @@ def test_finalists_are_retimed(monkeypatch):
     assert (res.density_block, res.force_block, res.lanes, res.ms_per_step) == (128, 128, 1, 1.0)
 
 
+def test_finalists_are_one_per_lane_count_and_retimed_over_the_run(monkeypatch):
+    """The three fastest search trials are all one-lane block pairs; the finalists are the
+    fastest trial of each lane count instead, re-timed over final_steps, where 4 lanes win
+    (a workload whose cost per step changes as it evolves)."""
+    search = {1: {(128, 128): 1.0, (64, 64): 1.1, (64, 128): 1.2, (128, 64): 1.3}, 4: {(128, 128): 1.5}, 2: {(128, 128): 1.4}}
+    final = {1: 2.0, 2: 2.2, 4: 1.8}
+    cfg, steps_seen = {}, []
+
+    def make(scale):
+        def configure(db, fb, ln):
+            cfg["k"] = (db, fb, ln)
+        return configure, lambda k: None, lambda: None, lambda: None
+
+    def fake_time(step, reset, steps, warmup):
+        db, fb, ln = cfg["k"]
+        steps_seen.append(steps)
+        return final[ln] if steps == 50 else search[ln][(db, fb)]
+
+    monkeypatch.setattr(T, "_time_steps", fake_time)
+    res = T._search(make, [1.0], (64, 128), 5, 1, lanes=(1, 2, 4), final_steps=50)
+    assert (res.lanes, res.launch, res.ms_per_step) == (4, (128, 128), 1.8)
+    assert steps_seen.count(50) == 3  # one finalist per lane count
+
+
 def test_grid_voxel_is_power_of_two():
     res = T.TuneResult(0.5, 64, 64, 1.0)
     g = res.grid_voxel(0.0457)
