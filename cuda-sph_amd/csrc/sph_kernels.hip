@@ -464,23 +464,35 @@ __global__ __launch_bounds__(256) void voxel_force_kernel(
     const float fi = p.w / (rho_i * rho_i);
     const float s = C.mass * C.visc / rho_i;
     float ax = 0.f, ay = 0.f, az = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
-    for (int t = 0; t < cnt; ++t) {
-        const int j = nbr[(size_t)(1 + t) * n + i];
-        if (j == i) continue;
-        const float4 q = pos[j], vj = vel[j];  // q.w = p_j, vj.w = 1 / rho_j (store_density)
-        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-        const float r = sqrtf(dx * dx + dy * dy + dz * dz);
-        const float hr = C.h - r;
-        // voxel_kernels.py:162-168: (p_i/rho_i^2 + p_j/rho_j^2) * grad W
-        const float cp = (fi + q.w * vj.w * vj.w) * C.grad_c * hr * hr * (1.0f / r);
-        ax += cp * dx;
-        ay += cp * dy;
-        az += cp * dz;
-        // voxel_kernels.py:203-208: (v_j - v_i) / rho_j * lap W * m * mu / rho_i
-        const float cv = vj.w * (C.lap_c * hr) * s;
-        bx += (vj.x - v.x) * cv;
-        by += (vj.y - v.y) * cv;
-        bz += (vj.z - v.z) * cv;
+    // four list entries and their gathers in flight, then summed in list order
+    for (int t0 = 0; t0 < cnt; t0 += 4) {
+        int jj[4];
+        float4 qq[4], vv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) jj[u] = nbr[(size_t)(1 + min(t0 + u, cnt - 1)) * n + i];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            qq[u] = pos[jj[u]];  // q.w = p_j, vj.w = 1 / rho_j (store_density)
+            vv[u] = vel[jj[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (t0 + u >= cnt || jj[u] == i) continue;
+            const float4 q = qq[u], vj = vv[u];
+            const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+            const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+            const float hr = C.h - r;
+            // voxel_kernels.py:162-168: (p_i/rho_i^2 + p_j/rho_j^2) * grad W
+            const float cp = (fi + q.w * vj.w * vj.w) * C.grad_c * hr * hr * (1.0f / r);
+            ax += cp * dx;
+            ay += cp * dy;
+            az += cp * dz;
+            // voxel_kernels.py:203-208: (v_j - v_i) / rho_j * lap W * m * mu / rho_i
+            const float cv = vj.w * (C.lap_c * hr) * s;
+            bx += (vj.x - v.x) * cv;
+            by += (vj.y - v.y) * cv;
+            bz += (vj.z - v.z) * cv;
+        }
     }
     // voxel_kernels.py:211 min(1, .)
     const Vec3 V{fminf(1.0f, bx), fminf(1.0f, by), fminf(1.0f, bz)};
