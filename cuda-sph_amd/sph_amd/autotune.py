@@ -125,6 +125,23 @@ def _device_name(device) -> str:
         return "unknown"
 
 
+_LIBRARY_ID: Optional[str] = None
+
+
+def _library_id() -> str:
+    """Content hash of the built library: a rebuilt kernel set invalidates cached choices."""
+    global _LIBRARY_ID
+    if _LIBRARY_ID is None:
+        from ._build import LIB_PATH
+
+        try:
+            with open(LIB_PATH, "rb") as f:
+                _LIBRARY_ID = hashlib.sha1(f.read()).hexdigest()[:16]
+        except OSError:
+            _LIBRARY_ID = "missing"
+    return _LIBRARY_ID
+
+
 def _fingerprint(parts: Iterable) -> str:
     h = hashlib.sha1()
     for p in parts:
@@ -141,7 +158,7 @@ def _instance_key(p: SimulationParameters, c: SPHConstants, state: SimulationSta
 
 def signature(kind: str, instances: Sequence[Tuple[SimulationParameters, SPHConstants, SimulationState]],
               mode: str, formulation: str, steps: int, device) -> str:
-    parts = [kind, mode, formulation, steps, _device_name(device)]
+    parts = [kind, mode, formulation, steps, _device_name(device), _library_id()]
     parts += [_instance_key(p, c, s) for p, c, s in instances]
     return f"{kind}:{formulation}:{mode}:{len(instances)}:{_fingerprint(parts)}"
 

@@ -30,6 +30,17 @@ def test_signature_tracks_workload_and_order():
     assert kab != kba
 
 
+def test_signature_tracks_the_built_library(monkeypatch):
+    """A rebuilt library (other kernels) must not reuse choices cached for the old one."""
+    w = W.box_uniform(256, seed=1)
+    inst = [(w.params, w.constants, w.state)]
+    monkeypatch.setattr(T, "_LIBRARY_ID", "build-a")
+    ka = T.signature("plain", inst, "BOX", "NAIVE", 20, "cpu")
+    monkeypatch.setattr(T, "_LIBRARY_ID", "build-b")
+    kb = T.signature("plain", inst, "BOX", "NAIVE", 20, "cpu")
+    assert ka != kb
+
+
 def test_search_picks_fastest_and_skips_rejected_grids(monkeypatch):
     # phase 1: (grid, lanes) at 128/128; phase 2: the other block pairs at the best of them
     cost = {(1.0, 128, 128, 1): 5.0, (1.0, 128, 128, 4): 4.5, (2.0, 128, 128, 1): 6.0, (2.0, 128, 128, 4): 7.0,
