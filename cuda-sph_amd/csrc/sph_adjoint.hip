@@ -306,6 +306,84 @@ void launch_pullback_lanes(sph_ctx* c, hipStream_t s) {
                                                           b.cotx);
 }
 
+namespace {
+
+// Step snapshots: the buffers the pullback reads of the step's forward pass (cell-ordered
+// state with p and 1/rho in .w, pid, rho, counts, lists, cell table), each at a 256-byte
+// aligned offset of the snapshot, copied in one launch.
+struct CopyRegions {
+    int count;
+    uint32_t* dst[8];
+    const uint32_t* src[8];
+    uint32_t words[8];
+};
+
+__global__ __launch_bounds__(256) void copy_regions_kernel(CopyRegions R) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (int r = 0; r < R.count; ++r) {
+        const uint32_t w = R.words[r];
+        const uint32_t q = w / 4u;  // 16-byte pieces (every region starts 256-byte aligned)
+        uint4* d4 = reinterpret_cast<uint4*>(R.dst[r]);
+        const uint4* s4 = reinterpret_cast<const uint4*>(R.src[r]);
+        for (uint32_t k = t; k < q; k += stride) d4[k] = s4[k];
+        for (uint32_t k = 4u * q + t; k < w; k += stride) R.dst[r][k] = R.src[r][k];
+    }
+}
+
+struct SnapshotPart {
+    void* buf;
+    size_t bytes;
+};
+
+int snapshot_parts(const sph_ctx* c, SnapshotPart* parts) {
+    const size_t n = (size_t)c->n;
+    const Buffers& b = c->b;
+    int k = 0;
+    parts[k++] = {b.posA, 16 * n};
+    parts[k++] = {b.velA, 16 * n};
+    parts[k++] = {b.pidA, 4 * n};
+    parts[k++] = {b.rhoA, 4 * n};
+    parts[k++] = {b.nn, 4 * n};
+    parts[k++] = {b.nl, sizeof(ListEntry) * (size_t)kListCap * n};
+    parts[k++] = {b.cs, 4 * ((size_t)c->sc.cs_cells + 1)};
+    return k;
+}
+
+inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+}  // namespace
+
+int64_t adjoint_snapshot_bytes(const sph_ctx* c) {
+    if (!c->adjoint || !c->cell_sort) return 0;
+    SnapshotPart parts[8];
+    const int k = snapshot_parts(c, parts);
+    size_t total = 0;
+    for (int i = 0; i < k; ++i) total += align256(parts[i].bytes);
+    return (int64_t)total;
+}
+
+hipError_t launch_adjoint_snapshot(sph_ctx* c, void* snap, bool save, hipStream_t s) {
+    SnapshotPart parts[8];
+    const int k = snapshot_parts(c, parts);
+    CopyRegions R{};
+    R.count = k;
+    size_t off = 0, most = 0;
+    for (int i = 0; i < k; ++i) {
+        uint32_t* at = reinterpret_cast<uint32_t*>(static_cast<char*>(snap) + off);
+        uint32_t* buf = static_cast<uint32_t*>(parts[i].buf);
+        R.dst[i] = save ? at : buf;
+        R.src[i] = save ? buf : at;
+        R.words[i] = (uint32_t)(parts[i].bytes / 4);
+        most = parts[i].bytes > most ? parts[i].bytes : most;
+        off += align256(parts[i].bytes);
+    }
+    const size_t want = (most / 16 + 255) / 256 + 1;
+    const int blocks = (int)(want < 2048 ? want : 2048);
+    copy_regions_kernel<<<blocks, 256, 0, s>>>(R);
+    return hipGetLastError();
+}
+
 // After sort, cell table and density of the step: the seed, pair, density and mu kernels.
 hipError_t launch_adjoint_pullback(sph_ctx* c, hipStream_t s) {
     const int64_t n = c->n;

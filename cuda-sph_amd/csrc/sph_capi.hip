@@ -733,6 +733,35 @@ int sph_adjoint_step(sph_ctx* c, void* stream) {
     return SPH_OK;
 }
 
+int64_t sph_adjoint_snapshot_bytes(const sph_ctx* c) { return c ? sph::adjoint_snapshot_bytes(c) : 0; }
+
+int sph_adjoint_save(sph_ctx* c, void* snapshot, void* stream) {
+    int rc = ready(c, true);
+    if (rc) return rc;
+    if (sph::adjoint_snapshot_bytes(c) == 0)
+        return fail(SPH_E_STATE, "snapshots need a NAIVE context created with SPH_FLAG_ADJOINT");
+    if (c->last_stage != SPH_STAGE_FORCE) return fail(SPH_E_STATE, "no completed step to save (run sph_step first)");
+    if (!snapshot || ((uintptr_t)snapshot & 255u)) return fail(SPH_E_INVALID, "snapshot must be 256-byte aligned device memory");
+    return hip_check(sph::launch_adjoint_snapshot(c, snapshot, true, (hipStream_t)stream), "adjoint save");
+}
+
+int sph_adjoint_step_from(sph_ctx* c, const void* snapshot, void* stream) {
+    int rc = ready(c, false);
+    if (rc) return rc;
+    if (sph::adjoint_snapshot_bytes(c) == 0)
+        return fail(SPH_E_STATE, "snapshots need a NAIVE context created with SPH_FLAG_ADJOINT");
+    if (!c->adj_seeded) return fail(SPH_E_STATE, "no cotangent (call sph_adjoint_seed)");
+    if (c->last_stage != 0 && c->last_stage != SPH_STAGE_FORCE)
+        return fail(SPH_E_STATE, "a staged step is in progress (run it to SPH_STAGE_FORCE first)");
+    if (!snapshot || ((uintptr_t)snapshot & 255u)) return fail(SPH_E_INVALID, "snapshot must be 256-byte aligned device memory");
+    hipStream_t s = (hipStream_t)stream;
+    if ((rc = hip_check(sph::launch_adjoint_snapshot(c, const_cast<void*>(snapshot), false, s), "adjoint restore")))
+        return rc;
+    if ((rc = hip_check(sph::launch_adjoint_pullback(c, s), "adjoint"))) return rc;
+    c->last_stage = SPH_STAGE_FORCE;
+    return SPH_OK;
+}
+
 int sph_adjoint_read(sph_ctx* c, float* gx, float* gv, float* gmu, void* stream) {
     int rc = ready(c, false);
     if (rc) return rc;

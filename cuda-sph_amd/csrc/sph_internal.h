@@ -186,4 +186,7 @@ int radix_num_tiles(int64_t n);
 hipError_t launch_adjoint_set(sph_ctx* c, const float* gx, const float* gv, hipStream_t s);
 hipError_t launch_adjoint_get(sph_ctx* c, float* gx, float* gv, float* gmu, hipStream_t s);
 hipError_t launch_adjoint_pullback(sph_ctx* c, hipStream_t s);
+int64_t adjoint_snapshot_bytes(const sph_ctx* c);
+// save: the step's buffers -> snap; otherwise snap -> the buffers
+hipError_t launch_adjoint_snapshot(sph_ctx* c, void* snap, bool save, hipStream_t s);
 }  // namespace sph

@@ -38,10 +38,15 @@ class Tape:
 class DifferentiableSPH:
     """graphs: replay the forward rollout and each backward segment as HIP graphs (one
     launch instead of hundreds; captured once per viscosity value, since the step's
-    constants are kernel arguments)."""
+    constants are kernel arguments).  snapshots: the backward's recompute pass keeps each
+    step's cell-ordered state, densities, lists and cell table (sph_adjoint_save), so that
+    reversing the step does not sort and recompute it again; it costs checkpoint_every
+    snapshots of device memory (about 240 bytes per particle each) and gives the same
+    gradients bit for bit."""
 
     def __init__(self, params: SimulationParameters, constants: SPHConstants, steps: int, checkpoint_every: int = 20,
-                 device="cuda:0", graphs: bool = True, launch: Optional[Tuple[int, int]] = None, lanes: int = 1):
+                 device="cuda:0", graphs: bool = True, launch: Optional[Tuple[int, int]] = None, lanes: int = 1,
+                 snapshots: bool = True):
         if steps < 1 or checkpoint_every < 1:
             raise ValueError("steps and checkpoint_every must be positive")
         self.params = params
@@ -59,7 +64,12 @@ class DifferentiableSPH:
         if lanes != 1:
             self.ctx.set_lanes(lanes)
         f32 = dict(dtype=torch.float32, device=self.device)
-        self._seg = torch.empty((self.every, 6, self.n), **f32)       # states of the segment being reversed
+        self.snapshots = bool(snapshots)
+        # states of the segment being reversed (with snapshots, only its first)
+        self._seg = torch.empty((1 if self.snapshots else self.every, 6, self.n), **f32)
+        if self.snapshots:
+            nbytes = self.ctx.adjoint_snapshot_bytes()
+            self._snap = torch.empty((self.every, nbytes), dtype=torch.uint8, device=self.device)
         self._in = torch.empty((6, self.n), **f32)                    # rollout input (x0, v0 rows)
         self._ck = torch.empty((self.num_checkpoints, 6, self.n), **f32)
         self._out = torch.empty((6, self.n), **f32)
@@ -100,6 +110,13 @@ class DifferentiableSPH:
         """Reverses one segment whose first state is in _seg[0]: recomputes its states, then
         pulls the cotangent back through its steps, last to first."""
         self._load(self._seg[0])
+        if self.snapshots:
+            for a in range(length):
+                self.ctx.step(1)
+                self.ctx.adjoint_save(self._snap[a])
+            for a in reversed(range(length)):
+                self.ctx.adjoint_step_from(self._snap[a])
+            return
         for a in range(1, length):
             self.ctx.step(1)
             self._save(self._seg[a])

@@ -242,6 +242,22 @@ int sph_adjoint_step(sph_ctx* ctx, void* stream);
  * pointer may be NULL. */
 int sph_adjoint_read(sph_ctx* ctx, float* dev_gx, float* dev_gv, float* dev_gmu, void* stream);
 
+/* Step snapshots (no reference counterpart): what sph_adjoint_step recomputes before its
+ * pullback -- the step's cell-ordered state, densities, neighbour lists and cell table --
+ * saved right after the forward step that produced them, so that a checkpointed backward
+ * can reverse each step without sorting and recomputing it a second time.
+ *   sph_adjoint_snapshot_bytes  bytes of one snapshot of this context (0 if it has no
+ *                               reverse-mode buffers)
+ *   sph_adjoint_save            after sph_step (or sph_run_stage through SPH_STAGE_FORCE):
+ *                               copies that step's snapshot to dev_snapshot
+ *   sph_adjoint_step_from       sph_adjoint_step for the step whose snapshot dev_snapshot
+ *                               holds (the resident state is not read); the cotangent and
+ *                               dL/dmu come out bitwise as sph_adjoint_step gives them
+ * dev_snapshot: device memory of sph_adjoint_snapshot_bytes, 256-byte aligned. */
+int64_t sph_adjoint_snapshot_bytes(const sph_ctx* ctx);
+int sph_adjoint_save(sph_ctx* ctx, void* dev_snapshot, void* stream);
+int sph_adjoint_step_from(sph_ctx* ctx, const void* dev_snapshot, void* stream);
+
 /* Message of the last error on this thread (always NUL-terminated). */
 int sph_last_error(char* buf, size_t len);
 

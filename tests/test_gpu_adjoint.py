@@ -229,3 +229,26 @@ def test_graph_replay_equals_eager_launches(gpu):
         seed = torch.ones_like(xg)
         for s, t in zip(g.backward(tg, seed, None), e.backward(te, seed, None)):
             assert torch.equal(s, t)
+
+
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_snapshots_give_the_recomputed_gradients(gpu, lanes):
+    """Reversing a segment from the recompute pass's step snapshots (sph_adjoint_save /
+    sph_adjoint_step_from) gives bitwise the gradients of re-sorting and recomputing each
+    step (sph_adjoint_step), on the wall case (bounces, and particles whose lists overflow
+    into the rescan), with a ragged last segment, in graphs and eagerly."""
+    w, x, v = case("walls")
+    for graphs in (True, False):
+        a = DifferentiableSPH(w.params, w.constants, steps=11, checkpoint_every=4, device=gpu, lanes=lanes,
+                              graphs=graphs, snapshots=True)
+        b = DifferentiableSPH(w.params, w.constants, steps=11, checkpoint_every=4, device=gpu, lanes=lanes,
+                              graphs=graphs, snapshots=False)
+        assert a.ctx.adjoint_snapshot_bytes() % 256 == 0 and a.ctx.adjoint_snapshot_bytes() >= 236 * 3000
+        for mu in (3.5, 5.0):
+            xa, va, ta = a.forward(dev(x, gpu), dev(v, gpu), mu)
+            xb, vb, tb = b.forward(dev(x, gpu), dev(v, gpu), mu)
+            assert torch.equal(xa, xb) and torch.equal(va, vb)
+            gx = torch.sin(xa * 7.0)
+            gv = torch.cos(va * 3.0)
+            for s, t in zip(a.backward(ta, gx, gv), b.backward(tb, gx, gv)):
+                assert torch.equal(s, t)
