@@ -467,3 +467,72 @@ def test_far_neighbours_in_cell_order_are_rescanned(gpu, lanes):
     dpos = np.abs(out.position - rp) / np.maximum(np.abs(rp), 1e-3)
     print(f"slab lanes {lanes}: rho {drho.max():.2e} pos {dpos.max():.2e}")
     assert drho.max() <= TOL and dpos.max() <= TOL
+
+
+def _same_after(make, upload_state, steps):
+    """Two contexts from make(): one runs `steps` steps in one sph_step call, the other in
+    single-step calls; returns both contexts."""
+    a, b = make(), make()
+    upload_state(a)
+    upload_state(b)
+    a.ctx.step(steps)
+    for _ in range(steps):
+        b.ctx.step(1)
+    return a, b
+
+
+@pytest.mark.parametrize("mode,lanes", [("BOX", 1), ("BOX", 4), ("PIPE", 1)])
+def test_steps_in_one_call_equal_single_calls(gpu, mode, lanes):
+    """One sph_step(k) call and k single-step calls give bitwise the same state, keys,
+    permutation and cell table (the per-cell counts are cleared between sorts whichever way
+    the steps are launched).  20,000 particles on 32,768 cells or more (the counting sort's
+    multi-workgroup path, which keeps counts in device memory), particles outside the grid
+    (BOX) and re-injection (PIPE)."""
+    if mode == "BOX":
+        w = box_uniform(20000, seed=31)
+        pos = outside_points(f32(w.state.position), w.params.space_size)
+        grid = 2.0 ** -5
+    else:
+        w = pipe_flow(n=20000, seed=32)
+        pos = f32(w.state.position)
+        pos[:300, 0] = 3.9995
+        grid = None
+    vel = f32(np.random.default_rng(33).normal(0.0, 0.5, size=pos.shape))
+    if mode == "PIPE":
+        vel[:300, 0] = 5.0
+
+    def make():
+        return NaiveSPHStrategy(w.params, constants=w.constants, mode=mode, device=gpu, grid_voxel=grid, lanes=lanes)
+
+    a, b = _same_after(make, lambda s: upload(s, pos, vel), 7)
+    info = a.ctx.grid_info()
+    print(f"{mode} lanes {lanes}: grid {info}")
+    for ta, tb in zip(a.ctx.download(), b.ctx.download()):
+        assert torch.equal(ta, tb)
+    for which in (N.BUF_KEYS, N.BUF_PERM, N.BUF_SORTED_KEYS, N.BUF_CELL_BEGIN, N.BUF_DENSITY):
+        assert torch.equal(a.ctx.read_buffer(which), b.ctx.read_buffer(which)), which
+    if mode == "PIPE":
+        assert torch.equal(a.ctx.read_buffer(N.BUF_RNG), b.ctx.read_buffer(N.BUF_RNG))
+    # and a further single step after the k-step call
+    a.ctx.step(1)
+    b.ctx.step(1)
+    for ta, tb in zip(a.ctx.download(), b.ctx.download()):
+        assert torch.equal(ta, tb)
+
+
+def test_batch_steps_in_one_call_equal_single_calls(gpu):
+    """The batched form (per-instance cell offsets in one counting sort): one sph_step(6) call
+    equals six single-step calls bitwise, instances with 1 and 4 lanes."""
+    from sph_amd.batched import BatchedSPH
+
+    ws = [box_uniform(n, seed=40 + k, h=h) for k, (n, h) in enumerate([(9000, 0.0457), (3000, 0.03), (12000, 0.05)])]
+
+    def make():
+        return BatchedSPH([(x.params, x.constants) for x in ws], device=gpu, cell_scale=0.5, lanes=[1, 4, 1])
+
+    states = [SimulationState(f32(x.state.position), f32(np.random.default_rng(41).normal(0.0, 0.5, (len(x.state.position), 3))),
+                              np.zeros(len(x.state.position))) for x in ws]
+    a, b = _same_after(make, lambda s: s.upload(states), 6)
+    for ta, tb in zip(a.ctx.download(), b.ctx.download()):
+        assert torch.equal(ta, tb)
+    assert torch.equal(a.ctx.read_buffer(N.BUF_CELL_BEGIN), b.ctx.read_buffer(N.BUF_CELL_BEGIN))
