@@ -536,6 +536,7 @@ def bench_adjoint(steps: int = 200, every: int = 20, tune: bool = True):
     t_f, t_b = a.elapsed_time(b) / 1e3, c.elapsed_time(d) / 1e3
     log(f"adjoint: forward {t_f * 1e3:.1f} ms, backward {t_b * 1e3:.1f} ms for {steps} steps of {n} particles")
     return {"workload": "adjoint_16k", "particles": n, "steps": steps, "checkpoint_every": every,
+            "keeps_every_step": sim.keep_all, "snapshot_mb": round(sim.ctx.adjoint_snapshot_bytes() / 1e6, 2),
             "value": round(n * steps / (t_f + t_b), 1), "unit": "particle-updates/s",
             "ms_forward": round(t_f * 1e3, 3), "ms_backward": round(t_b * 1e3, 3),
             "ms_per_adjoint_step": round(t_b * 1e3 / steps, 4),

@@ -7,6 +7,8 @@ FP32 by libsph_amd.  The backward pass is hand-written (libsph_amd's sph_adjoint
 the rollout keeps the state every `checkpoint_every` steps, and for each segment, last to
 first, recomputes the segment's states from its checkpoint and pulls the cotangent back
 through the steps one at a time.  Memory: (T / k + k) states of 24 bytes per particle.
+When T step snapshots fit in KEEP_ALL_BYTES of device memory (about 240 bytes per particle
+each), the rollout keeps them all and the backward pass runs no forward step at all.
 
     sim = DifferentiableSPH(params, constants, steps=200, checkpoint_every=20)
     xT, vT = rollout(sim, x0, v0, mu)          # x0, v0 (N, 3) float32 cuda; mu a tensor
@@ -26,13 +28,19 @@ from .constants import SPHConstants
 from .data_classes import SimulationParameters
 
 
-class Tape:
-    """What the backward pass needs of one forward rollout."""
+# Device memory the rollout may spend on keeping every step's snapshot (keep_all=None).
+KEEP_ALL_BYTES = 16 << 30
 
-    def __init__(self, checkpoints: torch.Tensor, visc: float, steps: int):
+
+class Tape:
+    """What the backward pass needs of one forward rollout.  generation: which forward call
+    made it (the simulator's per-step snapshots belong to its latest forward call only)."""
+
+    def __init__(self, checkpoints: torch.Tensor, visc: float, steps: int, generation: int = 0):
         self.checkpoints = checkpoints  # (n_ck, 6, N) float32: x, y, z, vx, vy, vz rows
         self.visc = visc
         self.steps = steps
+        self.generation = generation
 
 
 class DifferentiableSPH:
@@ -42,11 +50,14 @@ class DifferentiableSPH:
     step's cell-ordered state, densities, lists and cell table (sph_adjoint_save), so that
     reversing the step does not sort and recompute it again; it costs checkpoint_every
     snapshots of device memory (about 240 bytes per particle each) and gives the same
-    gradients bit for bit."""
+    gradients bit for bit.  keep_all: the forward rollout itself keeps all `steps` snapshots
+    (steps times that memory), so that the backward of the latest forward call reverses
+    every step from them without recomputing any (an older tape falls back to its
+    checkpoints); None keeps them when they fit in KEEP_ALL_BYTES."""
 
     def __init__(self, params: SimulationParameters, constants: SPHConstants, steps: int, checkpoint_every: int = 20,
                  device="cuda:0", graphs: bool = True, launch: Optional[Tuple[int, int]] = None, lanes: int = 1,
-                 snapshots: bool = True):
+                 snapshots: bool = True, keep_all: Optional[bool] = None):
         if steps < 1 or checkpoint_every < 1:
             raise ValueError("steps and checkpoint_every must be positive")
         self.params = params
@@ -67,14 +78,23 @@ class DifferentiableSPH:
         self.snapshots = bool(snapshots)
         # states of the segment being reversed (with snapshots, only its first)
         self._seg = torch.empty((1 if self.snapshots else self.every, 6, self.n), **f32)
+        nbytes = self.ctx.adjoint_snapshot_bytes()
+        if nbytes <= 0:
+            raise RuntimeError("libsph_amd gives no step snapshots for this context")
+        if keep_all is None:
+            keep_all = self.snapshots and self.steps * nbytes <= KEEP_ALL_BYTES
+        self.keep_all = bool(keep_all)
         if self.snapshots:
-            nbytes = self.ctx.adjoint_snapshot_bytes()
             self._snap = torch.empty((self.every, nbytes), dtype=torch.uint8, device=self.device)
+        if self.keep_all:
+            self._all = torch.empty((self.steps, nbytes), dtype=torch.uint8, device=self.device)
+        self._generation = 0  # forward calls so far; _all holds the latest one's snapshots
         self._in = torch.empty((6, self.n), **f32)                    # rollout input (x0, v0 rows)
         self._ck = torch.empty((self.num_checkpoints, 6, self.n), **f32)
         self._out = torch.empty((6, self.n), **f32)
         self._fwd_graphs = {}
         self._seg_graphs = {}
+        self._rev_graphs = {}
 
     @property
     def num_checkpoints(self) -> int:
@@ -97,14 +117,25 @@ class DifferentiableSPH:
 
     # -- passes ------------------------------------------------------------------------
     def _forward_body(self) -> None:
-        """_in -> _ck (every `every` steps, _ck[0] = _in) and _out."""
+        """_in -> _ck (every `every` steps, _ck[0] = _in), _out and, with keep_all, the
+        snapshot of every step in _all."""
         self._load(self._in)
         for t in range(1, self.steps):
             self.ctx.step(1)
+            if self.keep_all:
+                self.ctx.adjoint_save(self._all[t - 1])
             if t % self.every == 0:
                 self._save(self._ck[t // self.every])
         self.ctx.step(1)
+        if self.keep_all:
+            self.ctx.adjoint_save(self._all[self.steps - 1])
         self._save(self._out)
+
+    def _reverse_all_body(self) -> None:
+        """Pulls the cotangent back through every step of the latest rollout, last to first,
+        from its snapshots."""
+        for t in reversed(range(self.steps)):
+            self.ctx.adjoint_step_from(self._all[t])
 
     def _segment_body(self, length: int) -> None:
         """Reverses one segment whose first state is in _seg[0]: recomputes its states, then
@@ -143,10 +174,11 @@ class DifferentiableSPH:
         self.set_viscosity(mu)
         self._in.copy_(self._rows(x0, v0))
         self._run(self._fwd_graphs, mu, self._forward_body)
+        self._generation += 1
         ck = self._ck.clone()
         ck[0] = self._in
         out = self._out
-        return out[:3].t().contiguous(), out[3:].t().contiguous(), Tape(ck, mu, self.steps)
+        return out[:3].t().contiguous(), out[3:].t().contiguous(), Tape(ck, mu, self.steps, self._generation)
 
     def backward(self, tape: Tape, gx: Optional[torch.Tensor], gv: Optional[torch.Tensor]):
         """Cotangents of x_T, v_T (float32 (N, 3) or None) -> (dL/dx0, dL/dv0, dL/dmu)."""
@@ -155,6 +187,9 @@ class DifferentiableSPH:
         gx = None if gx is None else gx.detach().to(**f32).reshape(self.n, 3).contiguous()
         gv = None if gv is None else gv.detach().to(**f32).reshape(self.n, 3).contiguous()
         self.ctx.adjoint_seed(gx, gv)
+        if self.keep_all and tape.generation == self._generation:
+            self._run(self._rev_graphs, tape.visc, self._reverse_all_body)
+            return self.ctx.adjoint_read()
         for seg in reversed(range(self.num_checkpoints)):
             length = min(self.every, tape.steps - seg * self.every)
             self._seg[0].copy_(tape.checkpoints[seg])

@@ -233,22 +233,34 @@ def test_graph_replay_equals_eager_launches(gpu):
 
 @pytest.mark.parametrize("lanes", [1, 4])
 def test_snapshots_give_the_recomputed_gradients(gpu, lanes):
-    """Reversing a segment from the recompute pass's step snapshots (sph_adjoint_save /
-    sph_adjoint_step_from) gives bitwise the gradients of re-sorting and recomputing each
-    step (sph_adjoint_step), on the wall case (bounces, and particles whose lists overflow
-    into the rescan), with a ragged last segment, in graphs and eagerly."""
+    """Reversing steps from step snapshots (sph_adjoint_save / sph_adjoint_step_from), taken
+    by each segment's recompute pass or kept for every step by the rollout itself, gives
+    bitwise the gradients of re-sorting and recomputing each step (sph_adjoint_step): on the
+    wall case (bounces, and particles whose lists overflow into the rescan), with a ragged
+    last segment, in graphs and eagerly; a tape from an earlier forward call (whose per-step
+    snapshots have been overwritten) falls back to its checkpoints with the same result."""
     w, x, v = case("walls")
+    kinds = [dict(snapshots=False, keep_all=False), dict(snapshots=True, keep_all=False),
+             dict(snapshots=True, keep_all=True)]
     for graphs in (True, False):
-        a = DifferentiableSPH(w.params, w.constants, steps=11, checkpoint_every=4, device=gpu, lanes=lanes,
-                              graphs=graphs, snapshots=True)
-        b = DifferentiableSPH(w.params, w.constants, steps=11, checkpoint_every=4, device=gpu, lanes=lanes,
-                              graphs=graphs, snapshots=False)
-        assert a.ctx.adjoint_snapshot_bytes() % 256 == 0 and a.ctx.adjoint_snapshot_bytes() >= 236 * 3000
+        sims = [DifferentiableSPH(w.params, w.constants, steps=11, checkpoint_every=4, device=gpu, lanes=lanes,
+                                  graphs=graphs, **k) for k in kinds]
+        assert [s.keep_all for s in sims] == [False, False, True]
+        nbytes = sims[1].ctx.adjoint_snapshot_bytes()
+        assert nbytes % 256 == 0 and nbytes >= 236 * 3000
+        ref = None
         for mu in (3.5, 5.0):
-            xa, va, ta = a.forward(dev(x, gpu), dev(v, gpu), mu)
-            xb, vb, tb = b.forward(dev(x, gpu), dev(v, gpu), mu)
-            assert torch.equal(xa, xb) and torch.equal(va, vb)
-            gx = torch.sin(xa * 7.0)
-            gv = torch.cos(va * 3.0)
-            for s, t in zip(a.backward(ta, gx, gv), b.backward(tb, gx, gv)):
-                assert torch.equal(s, t)
+            outs = [s.forward(dev(x, gpu), dev(v, gpu), mu) for s in sims]
+            for xa, va, _ in outs[1:]:
+                assert torch.equal(xa, outs[0][0]) and torch.equal(va, outs[0][1])
+            gx = torch.sin(outs[0][0] * 7.0)
+            gv = torch.cos(outs[0][1] * 3.0)
+            grads = [s.backward(t, gx, gv) for s, (_, _, t) in zip(sims, outs)]
+            for g in grads[1:]:
+                for a, b in zip(g, grads[0]):
+                    assert torch.equal(a, b)
+            if ref is None:
+                ref = (outs[2][2], gx, gv, grads[0])
+        old_tape, gx, gv, want = ref  # the mu = 3.5 tape, after the mu = 5.0 forward call
+        for a, b in zip(sims[2].backward(old_tape, gx, gv), want):
+            assert torch.equal(a, b)
