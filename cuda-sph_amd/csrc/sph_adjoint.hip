@@ -294,8 +294,7 @@ hipError_t launch_adjoint_get(sph_ctx* c, float* gx, float* gv, float* gmu, hipS
 
 // The seed, pair and density pullbacks with L lanes per particle (the step's own count).
 template <int L>
-void launch_pullback_lanes(sph_ctx* c, hipStream_t s) {
-    const Buffers& b = c->b;
+void launch_pullback_lanes(sph_ctx* c, const Buffers& b, hipStream_t s) {
     const int nb = adj_blocks(c->n * L);
     adjoint_seed_kernel<L><<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.velA, b.pidA, b.rhoA, b.cs, b.nl, b.nn, b.posB,
                                                        b.velB, b.pidB, b.keyI, b.valI, b.flags, b.cotx, b.cotv,
@@ -310,7 +309,9 @@ namespace {
 
 // Step snapshots: the buffers the pullback reads of the step's forward pass (cell-ordered
 // state with p and 1/rho in .w, pid, rho, counts, lists, cell table), each at a 256-byte
-// aligned offset of the snapshot, copied in one launch.
+// aligned offset of the snapshot.  A step run with its buffers pointed into a snapshot
+// (snapshot_view) fills it directly; sph_adjoint_save copies them there in one launch after
+// an ordinary step; the pullback reads them in place.
 struct CopyRegions {
     int count;
     uint32_t* dst[8];
@@ -331,52 +332,77 @@ __global__ __launch_bounds__(256) void copy_regions_kernel(CopyRegions R) {
     }
 }
 
-struct SnapshotPart {
-    void* buf;
-    size_t bytes;
-};
-
-int snapshot_parts(const sph_ctx* c, SnapshotPart* parts) {
-    const size_t n = (size_t)c->n;
-    const Buffers& b = c->b;
-    int k = 0;
-    parts[k++] = {b.posA, 16 * n};
-    parts[k++] = {b.velA, 16 * n};
-    parts[k++] = {b.pidA, 4 * n};
-    parts[k++] = {b.rhoA, 4 * n};
-    parts[k++] = {b.nn, 4 * n};
-    parts[k++] = {b.nl, sizeof(ListEntry) * (size_t)kListCap * n};
-    parts[k++] = {b.cs, 4 * ((size_t)c->sc.cs_cells + 1)};
-    return k;
-}
+// Snapshot layout: the seven buffers in this order, each at a 256-byte aligned offset.
+constexpr int kSnapParts = 7;
 
 inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+void snapshot_sizes(const sph_ctx* c, size_t* bytes) {
+    const size_t n = (size_t)c->n;
+    bytes[0] = 16 * n;                                     // posA (x, y, z, p)
+    bytes[1] = 16 * n;                                     // velA (vx, vy, vz, 1/rho)
+    bytes[2] = 4 * n;                                      // pidA
+    bytes[3] = 4 * n;                                      // rhoA
+    bytes[4] = 4 * n;                                      // nn
+    bytes[5] = sizeof(ListEntry) * (size_t)kListCap * n;   // nl
+    bytes[6] = 4 * ((size_t)c->sc.cs_cells + 1);           // cs
+}
+
+// The seven buffers of b, in layout order.
+void snapshot_buffers(const Buffers& b, void** out) {
+    out[0] = b.posA;
+    out[1] = b.velA;
+    out[2] = b.pidA;
+    out[3] = b.rhoA;
+    out[4] = b.nn;
+    out[5] = b.nl;
+    out[6] = b.cs;
+}
 
 }  // namespace
 
 int64_t adjoint_snapshot_bytes(const sph_ctx* c) {
     if (!c->adjoint || !c->cell_sort) return 0;
-    SnapshotPart parts[8];
-    const int k = snapshot_parts(c, parts);
-    size_t total = 0;
-    for (int i = 0; i < k; ++i) total += align256(parts[i].bytes);
+    size_t bytes[kSnapParts], total = 0;
+    snapshot_sizes(c, bytes);
+    for (int i = 0; i < kSnapParts; ++i) total += align256(bytes[i]);
     return (int64_t)total;
 }
 
-hipError_t launch_adjoint_snapshot(sph_ctx* c, void* snap, bool save, hipStream_t s) {
-    SnapshotPart parts[8];
-    const int k = snapshot_parts(c, parts);
+Buffers snapshot_view(const sph_ctx* c, void* snap) {
+    size_t bytes[kSnapParts], off[kSnapParts], at = 0;
+    snapshot_sizes(c, bytes);
+    for (int i = 0; i < kSnapParts; ++i) {
+        off[i] = at;
+        at += align256(bytes[i]);
+    }
+    char* p = static_cast<char*>(snap);
+    Buffers v = c->b;
+    v.posA = reinterpret_cast<float4*>(p + off[0]);
+    v.velA = reinterpret_cast<float4*>(p + off[1]);
+    v.pidA = reinterpret_cast<uint32_t*>(p + off[2]);
+    v.rhoA = reinterpret_cast<float*>(p + off[3]);
+    v.nn = reinterpret_cast<uint32_t*>(p + off[4]);
+    v.nl = reinterpret_cast<ListEntry*>(p + off[5]);
+    v.cs = reinterpret_cast<uint32_t*>(p + off[6]);
+    return v;
+}
+
+hipError_t launch_adjoint_save(sph_ctx* c, void* snap, hipStream_t s) {
+    size_t bytes[kSnapParts];
+    snapshot_sizes(c, bytes);
+    void* own[kSnapParts];
+    void* in_snap[kSnapParts];
+    snapshot_buffers(c->b, own);
+    snapshot_buffers(snapshot_view(c, snap), in_snap);
     CopyRegions R{};
-    R.count = k;
-    size_t off = 0, most = 0;
-    for (int i = 0; i < k; ++i) {
-        uint32_t* at = reinterpret_cast<uint32_t*>(static_cast<char*>(snap) + off);
-        uint32_t* buf = static_cast<uint32_t*>(parts[i].buf);
-        R.dst[i] = save ? at : buf;
-        R.src[i] = save ? buf : at;
-        R.words[i] = (uint32_t)(parts[i].bytes / 4);
-        most = parts[i].bytes > most ? parts[i].bytes : most;
-        off += align256(parts[i].bytes);
+    R.count = kSnapParts;
+    size_t most = 0;
+    for (int i = 0; i < kSnapParts; ++i) {
+        R.dst[i] = static_cast<uint32_t*>(in_snap[i]);
+        R.src[i] = static_cast<const uint32_t*>(own[i]);
+        R.words[i] = (uint32_t)(bytes[i] / 4);
+        most = bytes[i] > most ? bytes[i] : most;
     }
     const size_t want = (most / 16 + 255) / 256 + 1;
     const int blocks = (int)(want < 2048 ? want : 2048);
@@ -385,18 +411,17 @@ hipError_t launch_adjoint_snapshot(sph_ctx* c, void* snap, bool save, hipStream_
 }
 
 // After sort, cell table and density of the step: the seed, pair, density and mu kernels.
-hipError_t launch_adjoint_pullback(sph_ctx* c, hipStream_t s) {
+hipError_t launch_adjoint_pullback(sph_ctx* c, const Buffers& b, hipStream_t s) {
     const int64_t n = c->n;
     if (n == 0) return hipGetLastError();
-    const Buffers& b = c->b;
     switch (c->lanes) {
-        case 2: launch_pullback_lanes<2>(c, s); break;
-        case 4: launch_pullback_lanes<4>(c, s); break;
-        case 8: launch_pullback_lanes<8>(c, s); break;
-        case 16: launch_pullback_lanes<16>(c, s); break;
-        case 32: launch_pullback_lanes<32>(c, s); break;
-        case 64: launch_pullback_lanes<64>(c, s); break;
-        default: launch_pullback_lanes<1>(c, s); break;
+        case 2: launch_pullback_lanes<2>(c, b, s); break;
+        case 4: launch_pullback_lanes<4>(c, b, s); break;
+        case 8: launch_pullback_lanes<8>(c, b, s); break;
+        case 16: launch_pullback_lanes<16>(c, b, s); break;
+        case 32: launch_pullback_lanes<32>(c, b, s); break;
+        case 64: launch_pullback_lanes<64>(c, b, s); break;
+        default: launch_pullback_lanes<1>(c, b, s); break;
     }
     adjoint_mu_kernel<<<1, kAdjThreads, 0, s>>>((int)n, b.adj_gx, b.mu_acc);
     return hipGetLastError();

@@ -509,6 +509,7 @@ int sph_upload_state(sph_ctx* c, const double* pos, const double* vel, void* str
     if ((rc = hip_check(sph::launch_upload_f64(c, pos, vel, (hipStream_t)stream), "upload"))) return rc;
     c->have_state = true;
     c->last_stage = 0;
+    c->a_in_snapshot = false;
     return SPH_OK;
 }
 
@@ -520,6 +521,7 @@ int sph_upload_state_f32(sph_ctx* c, const float* x, const float* y, const float
     if ((rc = hip_check(sph::launch_upload_f32(c, x, y, z, vx, vy, vz, (hipStream_t)stream), "upload"))) return rc;
     c->have_state = true;
     c->last_stage = 0;
+    c->a_in_snapshot = false;
     return SPH_OK;
 }
 
@@ -529,8 +531,10 @@ int sph_step(sph_ctx* c, int32_t nsteps, void* stream) {
     if (nsteps < 0) return fail(SPH_E_INVALID, "nsteps < 0");
     if (c->last_stage != 0 && c->last_stage != SPH_STAGE_FORCE)
         return fail(SPH_E_STATE, "a staged step is in progress (run it to SPH_STAGE_FORCE first)");
-    for (int32_t k = 0; k < nsteps; ++k)
+    for (int32_t k = 0; k < nsteps; ++k) {
         if ((rc = run_step_stages(c, (hipStream_t)stream, true))) return rc;
+        c->a_in_snapshot = false;
+    }
     return SPH_OK;
 }
 
@@ -555,6 +559,7 @@ int sph_run_stage(sph_ctx* c, int32_t stage, void* stream) {
     }
     if (rc) return rc;
     c->last_stage = stage;
+    c->a_in_snapshot = false;
     return SPH_OK;
 }
 
@@ -594,6 +599,8 @@ int sph_read_buffer(sph_ctx* c, int32_t which, void* out, void* stream) {
     if (rc) return rc;
     if (!out) return fail(SPH_E_INVALID, "null output");
     hipStream_t s = (hipStream_t)stream;
+    if (c->a_in_snapshot && which != SPH_BUF_KEYS && which != SPH_BUF_RNG && which != SPH_BUF_SORTED_KEYS)
+        return fail(SPH_E_STATE, "the last step was run by sph_step_save: its buffers are in the snapshot");
     switch (which) {
         case SPH_BUF_PRESSURE:
         case SPH_BUF_VISCOSITY:
@@ -728,24 +735,52 @@ int sph_adjoint_step(sph_ctx* c, void* stream) {
         return fail(SPH_E_STATE, "a staged step is in progress (run it to SPH_STAGE_FORCE first)");
     hipStream_t s = (hipStream_t)stream;
     if ((rc = run_density_stages(c, s, true))) return rc;
-    if ((rc = hip_check(sph::launch_adjoint_pullback(c, s), "adjoint"))) return rc;
+    c->a_in_snapshot = false;
+    if ((rc = hip_check(sph::launch_adjoint_pullback(c, c->b, s), "adjoint"))) return rc;
     c->last_stage = SPH_STAGE_FORCE;
     return SPH_OK;
 }
 
 int64_t sph_adjoint_snapshot_bytes(const sph_ctx* c) { return c ? sph::adjoint_snapshot_bytes(c) : 0; }
 
-int sph_adjoint_save(sph_ctx* c, void* snapshot, void* stream) {
+static int check_snapshot(const sph_ctx* c, const void* snapshot, size_t bytes) {
+    if (!snapshot || ((uintptr_t)snapshot & 255u)) return fail(SPH_E_INVALID, "snapshot must be 256-byte aligned device memory");
+    const int64_t need = sph::adjoint_snapshot_bytes(c);
+    if ((int64_t)bytes < need) return fail(SPH_E_INVALID, "snapshot of %zu bytes, %lld needed", bytes, (long long)need);
+    return SPH_OK;
+}
+
+int sph_adjoint_save(sph_ctx* c, void* snapshot, size_t bytes, void* stream) {
     int rc = ready(c, true);
     if (rc) return rc;
     if (sph::adjoint_snapshot_bytes(c) == 0)
         return fail(SPH_E_STATE, "snapshots need a NAIVE context created with SPH_FLAG_ADJOINT");
     if (c->last_stage != SPH_STAGE_FORCE) return fail(SPH_E_STATE, "no completed step to save (run sph_step first)");
-    if (!snapshot || ((uintptr_t)snapshot & 255u)) return fail(SPH_E_INVALID, "snapshot must be 256-byte aligned device memory");
-    return hip_check(sph::launch_adjoint_snapshot(c, snapshot, true, (hipStream_t)stream), "adjoint save");
+    if (c->a_in_snapshot) return fail(SPH_E_STATE, "the last step was run by sph_step_save: its snapshot is written");
+    if ((rc = check_snapshot(c, snapshot, bytes))) return rc;
+    return hip_check(sph::launch_adjoint_save(c, snapshot, (hipStream_t)stream), "adjoint save");
 }
 
-int sph_adjoint_step_from(sph_ctx* c, const void* snapshot, void* stream) {
+int sph_step_save(sph_ctx* c, void* snapshot, size_t bytes, void* stream) {
+    int rc = ready(c, true);
+    if (rc) return rc;
+    if (sph::adjoint_snapshot_bytes(c) == 0)
+        return fail(SPH_E_STATE, "snapshots need a NAIVE context created with SPH_FLAG_ADJOINT");
+    if (c->last_stage != 0 && c->last_stage != SPH_STAGE_FORCE)
+        return fail(SPH_E_STATE, "a staged step is in progress (run it to SPH_STAGE_FORCE first)");
+    if ((rc = check_snapshot(c, snapshot, bytes))) return rc;
+    // the step's sorted state, densities, lists and cell table are written into the
+    // snapshot instead of the context's own buffers; everything else is the context's
+    const sph::Buffers own = c->b;
+    c->b = sph::snapshot_view(c, snapshot);
+    rc = run_step_stages(c, (hipStream_t)stream, true);
+    c->b = own;
+    if (rc) return rc;
+    c->a_in_snapshot = true;
+    return SPH_OK;
+}
+
+int sph_adjoint_step_from(sph_ctx* c, const void* snapshot, size_t bytes, void* stream) {
     int rc = ready(c, false);
     if (rc) return rc;
     if (sph::adjoint_snapshot_bytes(c) == 0)
@@ -753,11 +788,11 @@ int sph_adjoint_step_from(sph_ctx* c, const void* snapshot, void* stream) {
     if (!c->adj_seeded) return fail(SPH_E_STATE, "no cotangent (call sph_adjoint_seed)");
     if (c->last_stage != 0 && c->last_stage != SPH_STAGE_FORCE)
         return fail(SPH_E_STATE, "a staged step is in progress (run it to SPH_STAGE_FORCE first)");
-    if (!snapshot || ((uintptr_t)snapshot & 255u)) return fail(SPH_E_INVALID, "snapshot must be 256-byte aligned device memory");
+    if ((rc = check_snapshot(c, snapshot, bytes))) return rc;
     hipStream_t s = (hipStream_t)stream;
-    if ((rc = hip_check(sph::launch_adjoint_snapshot(c, const_cast<void*>(snapshot), false, s), "adjoint restore")))
+    if ((rc = hip_check(sph::launch_adjoint_pullback(c, sph::snapshot_view(c, const_cast<void*>(snapshot)), s),
+                        "adjoint")))
         return rc;
-    if ((rc = hip_check(sph::launch_adjoint_pullback(c, s), "adjoint"))) return rc;
     c->last_stage = SPH_STAGE_FORCE;
     return SPH_OK;
 }

@@ -156,6 +156,7 @@ struct sph_ctx {
     bool bound;
     bool have_state;
     int last_stage;          // last stage run in the current step (0 = hashed, ready)
+    bool a_in_snapshot;      // the last step's sorted state, densities and table went to a snapshot
     uint64_t* rng_host;      // seeded on the host, uploaded at bind
     // batched form (empty for a plain step)
     std::vector<sph::StepConsts> inst;
@@ -185,8 +186,10 @@ int radix_num_tiles(int64_t n);
 // sph_adjoint.hip
 hipError_t launch_adjoint_set(sph_ctx* c, const float* gx, const float* gv, hipStream_t s);
 hipError_t launch_adjoint_get(sph_ctx* c, float* gx, float* gv, float* gmu, hipStream_t s);
-hipError_t launch_adjoint_pullback(sph_ctx* c, hipStream_t s);
+// the pullback of the step whose sorted state, densities, lists and cell table b holds
+hipError_t launch_adjoint_pullback(sph_ctx* c, const Buffers& b, hipStream_t s);
 int64_t adjoint_snapshot_bytes(const sph_ctx* c);
-// save: the step's buffers -> snap; otherwise snap -> the buffers
-hipError_t launch_adjoint_snapshot(sph_ctx* c, void* snap, bool save, hipStream_t s);
+// c's buffers with the seven a snapshot holds pointed at their places in snap
+Buffers snapshot_view(const sph_ctx* c, void* snap);
+hipError_t launch_adjoint_save(sph_ctx* c, void* snap, hipStream_t s);  // copy c's seven buffers into snap
 }  // namespace sph

@@ -59,7 +59,7 @@ EXPORTED = [
     "sph_upload_state", "sph_upload_state_f32", "sph_step", "sph_download_state", "sph_download_state_f32",
     "sph_run_stage", "sph_buffer_elems", "sph_read_buffer", "sph_diagnostics", "sph_grid_info", "sph_set_launch",
     "sph_create_batch", "sph_batch_info", "sph_set_constants", "sph_set_lanes", "sph_adjoint_seed", "sph_adjoint_step",
-    "sph_adjoint_read", "sph_set_batch_lanes", "sph_adjoint_snapshot_bytes", "sph_adjoint_save", "sph_adjoint_step_from",
+    "sph_adjoint_read", "sph_set_batch_lanes", "sph_adjoint_snapshot_bytes", "sph_adjoint_save", "sph_step_save", "sph_adjoint_step_from",
 ]
 
 _lib = None
@@ -102,8 +102,9 @@ def load_library() -> ctypes.CDLL:
         "sph_adjoint_step": ([vp, vp], i32),
         "sph_adjoint_read": ([vp, vp, vp, vp, vp], i32),
         "sph_adjoint_snapshot_bytes": ([vp], i64),
-        "sph_adjoint_save": ([vp, vp, vp], i32),
-        "sph_adjoint_step_from": ([vp, vp, vp], i32),
+        "sph_adjoint_save": ([vp, vp, c_size_t, vp], i32),
+        "sph_step_save": ([vp, vp, c_size_t, vp], i32),
+        "sph_adjoint_step_from": ([vp, vp, c_size_t, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -275,13 +276,26 @@ class NativeContext:
     def adjoint_snapshot_bytes(self) -> int:
         return int(self.lib.sph_adjoint_snapshot_bytes(self.handle))
 
+    def _snapshot(self, snapshot: torch.Tensor) -> int:
+        if snapshot.dtype != torch.uint8 or snapshot.device != self.device or not snapshot.is_contiguous():
+            raise ValueError("a snapshot is a contiguous uint8 tensor on the context's device")
+        return snapshot.numel()
+
     def adjoint_save(self, snapshot: torch.Tensor):
-        """The last step's snapshot into `snapshot` (uint8 device tensor of
+        """The last step's snapshot into `snapshot` (uint8 device tensor of at least
         adjoint_snapshot_bytes, 256-byte aligned)."""
-        check(self.lib.sph_adjoint_save(self.handle, _ptr(snapshot), _stream(self.device)), "sph_adjoint_save")
+        nbytes = self._snapshot(snapshot)
+        check(self.lib.sph_adjoint_save(self.handle, _ptr(snapshot), nbytes, _stream(self.device)),
+              "sph_adjoint_save")
+
+    def step_save(self, snapshot: torch.Tensor):
+        """One step whose snapshot is written straight into `snapshot`."""
+        nbytes = self._snapshot(snapshot)
+        check(self.lib.sph_step_save(self.handle, _ptr(snapshot), nbytes, _stream(self.device)), "sph_step_save")
 
     def adjoint_step_from(self, snapshot: torch.Tensor):
-        check(self.lib.sph_adjoint_step_from(self.handle, _ptr(snapshot), _stream(self.device)),
+        nbytes = self._snapshot(snapshot)
+        check(self.lib.sph_adjoint_step_from(self.handle, _ptr(snapshot), nbytes, _stream(self.device)),
               "sph_adjoint_step_from")
 
     def adjoint_read(self):

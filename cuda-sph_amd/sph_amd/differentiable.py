@@ -47,7 +47,7 @@ class DifferentiableSPH:
     """graphs: replay the forward rollout and each backward segment as HIP graphs (one
     launch instead of hundreds; captured once per viscosity value, since the step's
     constants are kernel arguments).  snapshots: the backward's recompute pass keeps each
-    step's cell-ordered state, densities, lists and cell table (sph_adjoint_save), so that
+    step's cell-ordered state, densities, lists and cell table (sph_step_save), so that
     reversing the step does not sort and recompute it again; it costs checkpoint_every
     snapshots of device memory (about 240 bytes per particle each) and gives the same
     gradients bit for bit.  keep_all: the forward rollout itself keeps all `steps` snapshots
@@ -120,15 +120,13 @@ class DifferentiableSPH:
         """_in -> _ck (every `every` steps, _ck[0] = _in), _out and, with keep_all, the
         snapshot of every step in _all."""
         self._load(self._in)
-        for t in range(1, self.steps):
-            self.ctx.step(1)
+        for t in range(1, self.steps + 1):
             if self.keep_all:
-                self.ctx.adjoint_save(self._all[t - 1])
-            if t % self.every == 0:
+                self.ctx.step_save(self._all[t - 1])
+            else:
+                self.ctx.step(1)
+            if t % self.every == 0 and t < self.steps:
                 self._save(self._ck[t // self.every])
-        self.ctx.step(1)
-        if self.keep_all:
-            self.ctx.adjoint_save(self._all[self.steps - 1])
         self._save(self._out)
 
     def _reverse_all_body(self) -> None:
@@ -143,8 +141,7 @@ class DifferentiableSPH:
         self._load(self._seg[0])
         if self.snapshots:
             for a in range(length):
-                self.ctx.step(1)
-                self.ctx.adjoint_save(self._snap[a])
+                self.ctx.step_save(self._snap[a])
             for a in reversed(range(length)):
                 self.ctx.adjoint_step_from(self._snap[a])
             return

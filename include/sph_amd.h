@@ -250,13 +250,21 @@ int sph_adjoint_read(sph_ctx* ctx, float* dev_gx, float* dev_gv, float* dev_gmu,
  *                               reverse-mode buffers)
  *   sph_adjoint_save            after sph_step (or sph_run_stage through SPH_STAGE_FORCE):
  *                               copies that step's snapshot to dev_snapshot
+ *   sph_step_save               sph_step(ctx, 1) writing the step's sorted state,
+ *                               densities, lists and cell table into dev_snapshot instead
+ *                               of the context (no copy); the state it produces is the
+ *                               same, but until the next step sph_read_buffer refuses the
+ *                               buffers the snapshot holds and sph_adjoint_save refuses
  *   sph_adjoint_step_from       sph_adjoint_step for the step whose snapshot dev_snapshot
- *                               holds (the resident state is not read); the cotangent and
- *                               dL/dmu come out bitwise as sph_adjoint_step gives them
- * dev_snapshot: device memory of sph_adjoint_snapshot_bytes, 256-byte aligned. */
+ *                               holds, read in place (the resident state is not read); the
+ *                               cotangent and dL/dmu come out bitwise as sph_adjoint_step
+ *                               gives them
+ * dev_snapshot: 256-byte aligned device memory of `bytes` bytes, at least
+ * sph_adjoint_snapshot_bytes (SPH_E_INVALID otherwise). */
 int64_t sph_adjoint_snapshot_bytes(const sph_ctx* ctx);
-int sph_adjoint_save(sph_ctx* ctx, void* dev_snapshot, void* stream);
-int sph_adjoint_step_from(sph_ctx* ctx, const void* dev_snapshot, void* stream);
+int sph_adjoint_save(sph_ctx* ctx, void* dev_snapshot, size_t bytes, void* stream);
+int sph_step_save(sph_ctx* ctx, void* dev_snapshot, size_t bytes, void* stream);
+int sph_adjoint_step_from(sph_ctx* ctx, const void* dev_snapshot, size_t bytes, void* stream);
 
 /* Message of the last error on this thread (always NUL-terminated). */
 int sph_last_error(char* buf, size_t len);

@@ -264,3 +264,73 @@ def test_snapshots_give_the_recomputed_gradients(gpu, lanes):
         old_tape, gx, gv, want = ref  # the mu = 3.5 tape, after the mu = 5.0 forward call
         for a, b in zip(sims[2].backward(old_tape, gx, gv), want):
             assert torch.equal(a, b)
+
+
+def _snapshot_regions(n, cells, cap=96):
+    """(offset, bytes) of the seven parts of a step snapshot: posA, velA, pid, rho, list
+    counts, lists (16-bit slots in pairs, [cap / 2][n][2]), cell table (cells + 1 entries)."""
+    sizes = [16 * n, 16 * n, 4 * n, 4 * n, 4 * n, 2 * cap * n, 4 * (cells + 1)]
+    out, at = [], 0
+    for b in sizes:
+        out.append((at, b))
+        at += (b + 255) // 256 * 256
+    return out, at
+
+
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_step_save_writes_what_adjoint_save_copies(gpu, lanes):
+    """sph_step_save (the step's sorted state, densities, lists and cell table written into
+    the snapshot) leaves the same snapshot as sph_step followed by sph_adjoint_save, part by
+    part (list slots past each particle's count are unused and not compared), the same
+    resident state, and the same pullback through sph_adjoint_step_from; and the entry
+    points refuse what they cannot serve."""
+    from sph_amd import _native as N
+
+    w, x, v = case("walls")
+    sims = [DifferentiableSPH(w.params, w.constants, steps=1, device=gpu, lanes=lanes, graphs=False)
+            for _ in range(2)]
+    a, b = sims[0].ctx, sims[1].ctx
+    nbytes = a.adjoint_snapshot_bytes()
+    cells = int(np.prod(a.grid_info()["dims"]))
+    regions, total = _snapshot_regions(a.n, cells)
+    assert total == nbytes
+    snaps = [torch.zeros(nbytes, dtype=torch.uint8, device=gpu), torch.full((nbytes,), 7, dtype=torch.uint8, device=gpu)]
+    xs, vs = dev(x, gpu), dev(v, gpu)
+    for ctx in (a, b):
+        ctx.upload_f32(*[t.contiguous() for t in xs.t()], *[t.contiguous() for t in vs.t()])
+    a.step(1)
+    a.adjoint_save(snaps[0])
+    b.step_save(snaps[1])
+    torch.cuda.synchronize()
+    for ta, tb in zip(a.download(), b.download()):
+        assert torch.equal(ta, tb)
+    s0, s1 = snaps[0].cpu().numpy(), snaps[1].cpu().numpy()
+    for k, (off, size) in enumerate(regions):
+        if k == 5:
+            continue
+        assert np.array_equal(s0[off:off + size], s1[off:off + size]), k
+    nn = s0[regions[4][0]:regions[4][0] + regions[4][1]].view(np.uint32)
+    # slot t of particle i is 16-bit word ((t >> 1) * n + i) * 2 + (t & 1)
+    lists = [s[regions[5][0]:regions[5][0] + regions[5][1]].view(np.uint16).reshape(48, a.n, 2)
+             .transpose(0, 2, 1).reshape(96, a.n) for s in (s0, s1)]
+    used = np.arange(96)[:, None] < np.minimum(nn, 96)[None, :]
+    assert used.sum() > 0 and np.array_equal(lists[0][used], lists[1][used])
+    # the pullback from either snapshot
+    g = torch.sin(xs)
+    for ctx, snap in ((a, snaps[0]), (b, snaps[1])):
+        ctx.adjoint_seed(g, None)
+        ctx.adjoint_step_from(snap)
+    for ta, tb in zip(a.adjoint_read(), b.adjoint_read()):
+        assert torch.equal(ta, tb)
+    # refusals (host-side checks, before any launch)
+    b.step_save(snaps[1])
+    with pytest.raises(RuntimeError, match="sph_step_save"):
+        b.adjoint_save(snaps[0])
+    with pytest.raises(RuntimeError, match="snapshot"):
+        b.read_buffer(N.BUF_DENSITY)
+    with pytest.raises(RuntimeError, match="needed"):
+        a.adjoint_step_from(snaps[0][: nbytes - 256])
+    with pytest.raises(RuntimeError, match="aligned"):
+        a.adjoint_step_from(snaps[0][1:])
+    b.step(1)
+    b.read_buffer(N.BUF_DENSITY)
