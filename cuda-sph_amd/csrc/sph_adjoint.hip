@@ -25,10 +25,12 @@
 // Kernels, in order, all in the cell order of the step:
 //   adjoint_seed_kernel     the force pass itself (same code, so the step's output is
 //                           written bitwise as sph_step writes it) plus the collision and
-//                           integration pullback
+//                           integration pullback; each particle's dL/dmu term is added to
+//                           its FP64 running sum (indexed by particle id)
 //   adjoint_pair_kernel     pressure / viscosity pullback: gv final, rt, gx partial
 //   adjoint_density_kernel  density pullback: gx final
-//   adjoint_mu_kernel       dL/dmu of the step, summed in a fixed order (deterministic)
+// and when the gradient is read, adjoint_mu_total_kernel sums the running sums in a fixed
+// order (deterministic).
 // The derivation is checked against an FP64 restatement and central finite differences.
 #include "sph_device.h"
 
@@ -98,7 +100,7 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_seed_kernel(
     const uint32_t* __restrict__ nn, float4* __restrict__ posOut, float4* __restrict__ velOut,
     uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
     uint32_t* __restrict__ flags, const float4* __restrict__ cotx, const float4* __restrict__ cotv,
-    float4* __restrict__ fb, float4* __restrict__ gxs, float4* __restrict__ gvs) {
+    float4* __restrict__ fb, float4* __restrict__ gxs, float4* __restrict__ gvs, double* __restrict__ mu_part) {
     // L lanes per particle, as the step's force pass uses, so the recomputed sums are its own
     const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int i = (int)(gt / L);
@@ -129,7 +131,8 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_seed_kernel(
     // dV/dmu = m Lc sum_j (h - r) iota_j (v_j - v_i)
     const float mu_term = C.mass * C.lap_c * (fbx * S.bx + fby * S.by + fbz * S.bz);
     fb[i] = make_float4(fbx, fby, fbz, rb);
-    gxs[i] = make_float4(gxp[0], gxp[1], gxp[2], mu_term);
+    mu_part[pid] += (double)mu_term;  // one writer per particle id and step
+    gxs[i] = make_float4(gxp[0], gxp[1], gxp[2], 0.0f);
     gvs[i] = make_float4(gvp[0], gvp[1], gvp[2], 0.0f);
 }
 
@@ -234,29 +237,32 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_density_kernel(
     cotx[pidA[k]] = make_float4(gx.x + s * ax, gx.y + s * ay, gx.z + s * az, 0.0f);
 }
 
-// One workgroup: each lane sums a strided range in order, then a fixed tree; the step's
-// dL/dmu is added to acc[0].
-__global__ __launch_bounds__(kAdjThreads) void adjoint_mu_kernel(int n, const float4* __restrict__ gxs,
-                                                                  float* __restrict__ acc) {
-    __shared__ float part[kAdjThreads];
-    float s = 0.0f;
-    for (int i = threadIdx.x; i < n; i += kAdjThreads) s += gxs[i].w;
+// dL/dmu: the per-particle sums over the reversed steps (mu_part, FP64), summed by one
+// workgroup in a fixed order (each lane a strided range in order, then a fixed tree) into
+// acc[0] when the gradient is read.
+__global__ __launch_bounds__(kAdjThreads) void adjoint_mu_total_kernel(int n, const double* __restrict__ mu_part,
+                                                                        float* __restrict__ acc) {
+    __shared__ double part[kAdjThreads];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += kAdjThreads) s += mu_part[i];
     part[threadIdx.x] = s;
     __syncthreads();
     for (int w = kAdjThreads / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) acc[0] += part[0];
+    if (threadIdx.x == 0) acc[0] = (float)part[0];
 }
 
 __global__ __launch_bounds__(kAdjThreads) void adjoint_set_kernel(int n, const float* __restrict__ gx,
                                                                    const float* __restrict__ gv,
                                                                    float4* __restrict__ cotx,
-                                                                   float4* __restrict__ cotv, float* __restrict__ acc) {
+                                                                   float4* __restrict__ cotv, float* __restrict__ acc,
+                                                                   double* __restrict__ mu_part) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) acc[0] = 0.0f;
     if (i >= n) return;
+    mu_part[i] = 0.0;
     const int64_t o = 3 * (int64_t)i;
     cotx[i] = gx ? make_float4(gx[o], gx[o + 1], gx[o + 2], 0.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
     cotv[i] = gv ? make_float4(gv[o], gv[o + 1], gv[o + 2], 0.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -281,12 +287,13 @@ inline int adj_blocks(int64_t n) { return (int)((n + kAdjThreads - 1) / kAdjThre
 hipError_t launch_adjoint_set(sph_ctx* c, const float* gx, const float* gv, hipStream_t s) {
     const int64_t n = c->n;
     adjoint_set_kernel<<<adj_blocks(n > 0 ? n : 1), kAdjThreads, 0, s>>>((int)n, gx, gv, c->b.cotx, c->b.cotv,
-                                                                         c->b.mu_acc);
+                                                                         c->b.mu_acc, c->b.mu_part);
     return hipGetLastError();
 }
 
 hipError_t launch_adjoint_get(sph_ctx* c, float* gx, float* gv, float* gmu, hipStream_t s) {
     const int64_t n = c->n;
+    adjoint_mu_total_kernel<<<1, kAdjThreads, 0, s>>>((int)n, c->b.mu_part, c->b.mu_acc);
     adjoint_get_kernel<<<adj_blocks(n > 0 ? n : 1), kAdjThreads, 0, s>>>((int)n, c->b.cotx, c->b.cotv, c->b.mu_acc,
                                                                          gx, gv, gmu);
     return hipGetLastError();
@@ -298,7 +305,7 @@ void launch_pullback_lanes(sph_ctx* c, const Buffers& b, hipStream_t s) {
     const int nb = adj_blocks(c->n * L);
     adjoint_seed_kernel<L><<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.velA, b.pidA, b.rhoA, b.cs, b.nl, b.nn, b.posB,
                                                        b.velB, b.pidB, b.keyI, b.valI, b.flags, b.cotx, b.cotv,
-                                                       b.adj_fb, b.adj_gx, b.adj_gv);
+                                                       b.adj_fb, b.adj_gx, b.adj_gv, b.mu_part);
     adjoint_pair_kernel<L><<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.velA, b.pidA, b.cs, b.nl, b.nn, b.adj_fb,
                                                        b.adj_gx, b.adj_gv, b.adj_rt, b.cotv);
     adjoint_density_kernel<L><<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.pidA, b.cs, b.nl, b.nn, b.adj_rt, b.adj_gx,
@@ -423,7 +430,6 @@ hipError_t launch_adjoint_pullback(sph_ctx* c, const Buffers& b, hipStream_t s) 
         case 64: launch_pullback_lanes<64>(c, b, s); break;
         default: launch_pullback_lanes<1>(c, b, s); break;
     }
-    adjoint_mu_kernel<<<1, kAdjThreads, 0, s>>>((int)n, b.adj_gx, b.mu_acc);
     return hipGetLastError();
 }
 

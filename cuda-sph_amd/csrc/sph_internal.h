@@ -125,6 +125,7 @@ struct Buffers {
     float4 *adj_fb, *adj_gx, *adj_gv;
     float* adj_rt;
     float* mu_acc;
+    double* mu_part;       // dL/dmu per particle id, summed over the reversed steps
 };
 
 }  // namespace sph

@@ -2,7 +2,7 @@
 
 Bars (FP32 kernels against FP64 references on the same FP32-rounded inputs):
   one step           max |g_gpu - g_ref| <= 1e-5 * max |g_ref| for the position and the
-                     velocity cotangent; dL/dmu within 1e-5 relative (measured <= 7e-7)
+                     velocity cotangent; dL/dmu within 1e-5 relative (measured <= 1.4e-6)
   30-step rollout    relative L2 error of dL/dv0 <= 5e-5 (measured 6e-6), dL/dmu within
                      2e-5 (measured 5e-7), loss within 1e-6
   FD, 40 steps       16,384 particles: 8 velocity components within 5e-4 (measured 5e-5) and
