@@ -93,21 +93,14 @@ __device__ __forceinline__ float group_sum(float v) {
 __device__ __forceinline__ float comp(const Vec3& a, int d) { return d == 0 ? a.x : (d == 1 ? a.y : a.z); }
 __device__ __forceinline__ float comp(const float4& a, int d) { return d == 0 ? a.x : (d == 1 ? a.y : a.z); }
 
-template <int L>
-__global__ __launch_bounds__(kAdjThreads) void adjoint_seed_kernel(
-    StepConsts C, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
-    const float* __restrict__ rhoA, const uint32_t* __restrict__ cs, const ListEntry* __restrict__ nl,
-    const uint32_t* __restrict__ nn, float4* __restrict__ posOut, float4* __restrict__ velOut,
+// Everything of the seed after the force pass's sums: the step's output (bitwise sph_step's)
+// and the collision / integration pullback of particle i.
+__device__ __forceinline__ void seed_particle(
+    const StepConsts& C, int i, float4 p, float4 v, const PairSums& S, const uint32_t* __restrict__ pidA,
+    const float* __restrict__ rhoA, float4* __restrict__ posOut, float4* __restrict__ velOut,
     uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
     uint32_t* __restrict__ flags, const float4* __restrict__ cotx, const float4* __restrict__ cotv,
     float4* __restrict__ fb, float4* __restrict__ gxs, float4* __restrict__ gvs, double* __restrict__ mu_part) {
-    // L lanes per particle, as the step's force pass uses, so the recomputed sums are its own
-    const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const int i = (int)(gt / L);
-    if (i >= C.n) return;
-    const float4 p = pos[i], v = vel[i];
-    const PairSums S = naive_pair_sums_lanes<L>(C, i, gt & (L - 1), C.n, p, v, pos, vel, cs, nl, nn);
-    if (L > 1 && (gt & (L - 1)) != 0) return;
     const float ps = 0.5f * C.mass * C.grad_c;
     const float vs = C.mass * C.visc * C.lap_c;
     const Vec3 P{ps * S.ax, ps * S.ay, ps * S.az}, V{vs * S.bx, vs * S.by, vs * S.bz};
@@ -134,6 +127,48 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_seed_kernel(
     mu_part[pid] += (double)mu_term;  // one writer per particle id and step
     gxs[i] = make_float4(gxp[0], gxp[1], gxp[2], 0.0f);
     gvs[i] = make_float4(gvp[0], gvp[1], gvp[2], 0.0f);
+}
+
+template <int L>
+__global__ __launch_bounds__(kAdjThreads) void adjoint_seed_kernel(
+    StepConsts C, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
+    const float* __restrict__ rhoA, const uint32_t* __restrict__ cs, const ListEntry* __restrict__ nl,
+    const uint32_t* __restrict__ nn, float4* __restrict__ posOut, float4* __restrict__ velOut,
+    uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
+    uint32_t* __restrict__ flags, const float4* __restrict__ cotx, const float4* __restrict__ cotv,
+    float4* __restrict__ fb, float4* __restrict__ gxs, float4* __restrict__ gvs, double* __restrict__ mu_part) {
+    // L lanes per particle, as the step's force pass uses, so the recomputed sums are its own
+    const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int i = (int)(gt / L);
+    if (i >= C.n) return;
+    const float4 p = pos[i], v = vel[i];
+    const PairSums S = naive_pair_sums_lanes<L>(C, i, gt & (L - 1), C.n, p, v, pos, vel, cs, nl, nn);
+    if (L > 1 && (gt & (L - 1)) != 0) return;
+    seed_particle(C, i, p, v, S, pidA, rhoA, posOut, velOut, pidOut, keyI, valI, flags, cotx, cotv, fb, gxs, gvs,
+                  mu_part);
+}
+
+// The seed from the sums the step's own force pass kept (sums[i], sums[n + i]): the same
+// values, without the neighbour walk; one thread per particle.
+__global__ __launch_bounds__(kAdjThreads) void adjoint_seed_sums_kernel(
+    StepConsts C, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
+    const float* __restrict__ rhoA, const float4* __restrict__ sums, float4* __restrict__ posOut,
+    float4* __restrict__ velOut, uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI,
+    uint32_t* __restrict__ valI, uint32_t* __restrict__ flags, const float4* __restrict__ cotx,
+    const float4* __restrict__ cotv, float4* __restrict__ fb, float4* __restrict__ gxs, float4* __restrict__ gvs,
+    double* __restrict__ mu_part) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= C.n) return;
+    const float4 a = sums[i], b = sums[C.n + i];
+    PairSums S;
+    S.ax = a.x;
+    S.ay = a.y;
+    S.az = a.z;
+    S.bx = b.x;
+    S.by = b.y;
+    S.bz = b.z;
+    seed_particle(C, i, pos[i], vel[i], S, pidA, rhoA, posOut, velOut, pidOut, keyI, valI, flags, cotx, cotv, fb, gxs,
+                  gvs, mu_part);
 }
 
 template <int L>
@@ -301,9 +336,14 @@ hipError_t launch_adjoint_get(sph_ctx* c, float* gx, float* gv, float* gmu, hipS
 
 // The seed, pair and density pullbacks with L lanes per particle (the step's own count).
 template <int L>
-void launch_pullback_lanes(sph_ctx* c, const Buffers& b, hipStream_t s) {
+void launch_pullback_lanes(sph_ctx* c, const Buffers& b, hipStream_t s, bool from_sums) {
     const int nb = adj_blocks(c->n * L);
-    adjoint_seed_kernel<L><<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.velA, b.pidA, b.rhoA, b.cs, b.nl, b.nn, b.posB,
+    if (from_sums)
+        adjoint_seed_sums_kernel<<<adj_blocks(c->n), kAdjThreads, 0, s>>>(
+            c->sc, b.posA, b.velA, b.pidA, b.rhoA, b.sums, b.posB, b.velB, b.pidB, b.keyI, b.valI, b.flags, b.cotx,
+            b.cotv, b.adj_fb, b.adj_gx, b.adj_gv, b.mu_part);
+    else
+        adjoint_seed_kernel<L><<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.velA, b.pidA, b.rhoA, b.cs, b.nl, b.nn, b.posB,
                                                        b.velB, b.pidB, b.keyI, b.valI, b.flags, b.cotx, b.cotv,
                                                        b.adj_fb, b.adj_gx, b.adj_gv, b.mu_part);
     adjoint_pair_kernel<L><<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.velA, b.pidA, b.cs, b.nl, b.nn, b.adj_fb,
@@ -315,7 +355,8 @@ void launch_pullback_lanes(sph_ctx* c, const Buffers& b, hipStream_t s) {
 namespace {
 
 // Step snapshots: the buffers the pullback reads of the step's forward pass (cell-ordered
-// state with p and 1/rho in .w, pid, rho, counts, lists, cell table), each at a 256-byte
+// state with p and 1/rho in .w, pid, rho, counts, lists, cell table, the force pass's
+// sums), each at a 256-byte
 // aligned offset of the snapshot.  A step run with its buffers pointed into a snapshot
 // (snapshot_view) fills it directly; sph_adjoint_save copies them there in one launch after
 // an ordinary step; the pullback reads them in place.
@@ -339,8 +380,8 @@ __global__ __launch_bounds__(256) void copy_regions_kernel(CopyRegions R) {
     }
 }
 
-// Snapshot layout: the seven buffers in this order, each at a 256-byte aligned offset.
-constexpr int kSnapParts = 7;
+// Snapshot layout: the eight buffers in this order, each at a 256-byte aligned offset.
+constexpr int kSnapParts = 8;
 
 inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
@@ -353,9 +394,10 @@ void snapshot_sizes(const sph_ctx* c, size_t* bytes) {
     bytes[4] = 4 * n;                                      // nn
     bytes[5] = sizeof(ListEntry) * (size_t)kListCap * n;   // nl
     bytes[6] = 4 * ((size_t)c->sc.cs_cells + 1);           // cs
+    bytes[7] = 32 * n;                                     // sums (the force pass's, [2][N])
 }
 
-// The seven buffers of b, in layout order.
+// The eight buffers of b, in layout order.
 void snapshot_buffers(const Buffers& b, void** out) {
     out[0] = b.posA;
     out[1] = b.velA;
@@ -364,6 +406,7 @@ void snapshot_buffers(const Buffers& b, void** out) {
     out[4] = b.nn;
     out[5] = b.nl;
     out[6] = b.cs;
+    out[7] = b.sums;
 }
 
 }  // namespace
@@ -392,6 +435,7 @@ Buffers snapshot_view(const sph_ctx* c, void* snap) {
     v.nn = reinterpret_cast<uint32_t*>(p + off[4]);
     v.nl = reinterpret_cast<ListEntry*>(p + off[5]);
     v.cs = reinterpret_cast<uint32_t*>(p + off[6]);
+    v.sums = reinterpret_cast<float4*>(p + off[7]);
     return v;
 }
 
@@ -418,17 +462,17 @@ hipError_t launch_adjoint_save(sph_ctx* c, void* snap, hipStream_t s) {
 }
 
 // After sort, cell table and density of the step: the seed, pair, density and mu kernels.
-hipError_t launch_adjoint_pullback(sph_ctx* c, const Buffers& b, hipStream_t s) {
+hipError_t launch_adjoint_pullback(sph_ctx* c, const Buffers& b, hipStream_t s, bool from_sums) {
     const int64_t n = c->n;
     if (n == 0) return hipGetLastError();
     switch (c->lanes) {
-        case 2: launch_pullback_lanes<2>(c, b, s); break;
-        case 4: launch_pullback_lanes<4>(c, b, s); break;
-        case 8: launch_pullback_lanes<8>(c, b, s); break;
-        case 16: launch_pullback_lanes<16>(c, b, s); break;
-        case 32: launch_pullback_lanes<32>(c, b, s); break;
-        case 64: launch_pullback_lanes<64>(c, b, s); break;
-        default: launch_pullback_lanes<1>(c, b, s); break;
+        case 2: launch_pullback_lanes<2>(c, b, s, from_sums); break;
+        case 4: launch_pullback_lanes<4>(c, b, s, from_sums); break;
+        case 8: launch_pullback_lanes<8>(c, b, s, from_sums); break;
+        case 16: launch_pullback_lanes<16>(c, b, s, from_sums); break;
+        case 32: launch_pullback_lanes<32>(c, b, s, from_sums); break;
+        case 64: launch_pullback_lanes<64>(c, b, s, from_sums); break;
+        default: launch_pullback_lanes<1>(c, b, s, from_sums); break;
     }
     return hipGetLastError();
 }

@@ -81,7 +81,7 @@ size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 enum Slice {
     S_POSA, S_VELA, S_POSB, S_VELB, S_PIDA, S_PIDB, S_RHOA, S_KEYI, S_VALI, S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_CS,
     S_HIST, S_DTOT, S_RNG, S_DBGP, S_DBGV, S_DBGF, S_NBR, S_NL, S_NN, S_DIAG, S_FLAGS, S_PIPE, S_INST, S_INSTF,
-    S_COTX, S_COTV, S_AFB, S_AGX, S_AGV, S_ART, S_MU, S_MUP, S_CCNT, S_CPART, S_COUNT
+    S_COTX, S_COTV, S_AFB, S_AGX, S_AGV, S_ART, S_MU, S_MUP, S_SUMS, S_CCNT, S_CPART, S_COUNT
 };
 
 struct Layout {
@@ -112,6 +112,7 @@ Layout plan_layout(const sph_ctx* c) {
     sz[S_ART] = c->adjoint ? 4 * nn : 16;
     sz[S_MU] = 64;
     sz[S_MUP] = c->adjoint ? 8 * nn : 16;
+    sz[S_SUMS] = c->adjoint ? 32 * nn : 16;
     sz[S_CCNT] = c->cell_sort ? 4 * (size_t)(c->sc.cs_cells > 0 ? c->sc.cs_cells : 1) : 16;
     sz[S_CPART] = c->cell_sort ? 4 * (size_t)sph::cell_scan_tiles(c->sc.cs_cells > 0 ? c->sc.cs_cells : 1) : 16;
     Layout L;
@@ -476,6 +477,7 @@ int sph_bind_workspace(sph_ctx* c, void* dev, size_t bytes, void* stream) {
     b.adj_rt = (float*)(base + L.off[S_ART]);
     b.mu_acc = (float*)(base + L.off[S_MU]);
     b.mu_part = (double*)(base + L.off[S_MUP]);
+    b.sums = c->adjoint ? (float4*)(base + L.off[S_SUMS]) : nullptr;
     b.cell_cnt = (uint32_t*)(base + L.off[S_CCNT]);
     b.cell_part = (uint32_t*)(base + L.off[S_CPART]);
     c->ws = dev;
@@ -792,7 +794,7 @@ int sph_adjoint_step_from(sph_ctx* c, const void* snapshot, size_t bytes, void* 
         return fail(SPH_E_STATE, "a staged step is in progress (run it to SPH_STAGE_FORCE first)");
     if ((rc = check_snapshot(c, snapshot, bytes))) return rc;
     hipStream_t s = (hipStream_t)stream;
-    if ((rc = hip_check(sph::launch_adjoint_pullback(c, sph::snapshot_view(c, const_cast<void*>(snapshot)), s),
+    if ((rc = hip_check(sph::launch_adjoint_pullback(c, sph::snapshot_view(c, const_cast<void*>(snapshot)), s, true),
                         "adjoint")))
         return rc;
     c->last_stage = SPH_STAGE_FORCE;

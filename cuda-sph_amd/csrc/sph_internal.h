@@ -126,6 +126,8 @@ struct Buffers {
     float* adj_rt;
     float* mu_acc;
     double* mu_part;       // dL/dmu per particle id, summed over the reversed steps
+    float4* sums;          // adjoint contexts: the force pass's pressure and viscosity sums
+                           // per cell-ordered particle, [2][N] (null otherwise)
 };
 
 }  // namespace sph
@@ -188,9 +190,10 @@ int radix_num_tiles(int64_t n);
 hipError_t launch_adjoint_set(sph_ctx* c, const float* gx, const float* gv, hipStream_t s);
 hipError_t launch_adjoint_get(sph_ctx* c, float* gx, float* gv, float* gmu, hipStream_t s);
 // the pullback of the step whose sorted state, densities, lists and cell table b holds
-hipError_t launch_adjoint_pullback(sph_ctx* c, const Buffers& b, hipStream_t s);
+// from_sums: take the force pass's sums from b.sums instead of walking the lists again
+hipError_t launch_adjoint_pullback(sph_ctx* c, const Buffers& b, hipStream_t s, bool from_sums = false);
 int64_t adjoint_snapshot_bytes(const sph_ctx* c);
-// c's buffers with the seven a snapshot holds pointed at their places in snap
+// c's buffers with the eight a snapshot holds pointed at their places in snap
 Buffers snapshot_view(const sph_ctx* c, void* snap);
-hipError_t launch_adjoint_save(sph_ctx* c, void* snap, hipStream_t s);  // copy c's seven buffers into snap
+hipError_t launch_adjoint_save(sph_ctx* c, void* snap, hipStream_t s);  // copy c's eight buffers into snap
 }  // namespace sph

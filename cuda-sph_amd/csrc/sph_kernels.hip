@@ -322,7 +322,8 @@ __global__ __launch_bounds__(256) void naive_force_kernel(
     const uint32_t* __restrict__ nn, float4* __restrict__ posOut, float4* __restrict__ velOut,
     uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
     uint64_t* __restrict__ rng, const PipeTable* __restrict__ pipe, float4* __restrict__ dbgP,
-    float4* __restrict__ dbgV, float4* __restrict__ dbgF, uint32_t* __restrict__ flags) {
+    float4* __restrict__ dbgV, float4* __restrict__ dbgF, uint32_t* __restrict__ flags,
+    float4* __restrict__ sums) {
     const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int i = (int)(gt / L);
     if (i >= G.n) return;
@@ -330,6 +331,10 @@ __global__ __launch_bounds__(256) void naive_force_kernel(
     const float4 p = pos[i], v = vel[i];
     const PairSums S = naive_pair_sums_lanes<L>(C, i, gt & (L - 1), G.n, p, v, pos, vel, cs, nl, nn);
     if (L > 1 && (gt & (L - 1)) != 0) return;
+    if (sums) {  // reverse mode keeps the sums for the pullback of this step
+        sums[i] = make_float4(S.ax, S.ay, S.az, 0.0f);
+        sums[G.n + i] = make_float4(S.bx, S.by, S.bz, 0.0f);
+    }
     const float ps = 0.5f * C.mass * C.grad_c;   // naive_kernels.py:40-52: m * (pi+pj)/(2 rho_j) * grad W
     const float vs = C.mass * C.visc * C.lap_c;  // naive_kernels.py:71-77: m * mu * (vj-vi)/rho_j * lap W
     finish_particle<MODE, true>(C, i, p, v, pidA[i], rhoA[i], Vec3{ps * S.ax, ps * S.ay, ps * S.az},
@@ -731,7 +736,7 @@ void launch_force_lanes(sph_ctx* c, hipStream_t s, float4* dP, float4* dV, float
     const int bs = c->force_block;
     naive_force_kernel<MODE, L><<<blocks_for(c->n * L, bs), bs, 0, s>>>(
         c->sc, c->b.posA, c->b.velA, c->b.pidA, c->b.rhoA, c->b.cs, c->b.nl, c->b.nn, c->b.posB, c->b.velB,
-        c->b.pidB, c->b.keyI, c->b.valI, c->b.rng, c->b.pipe, dP, dV, dF, c->b.flags);
+        c->b.pidB, c->b.keyI, c->b.valI, c->b.rng, c->b.pipe, dP, dV, dF, c->b.flags, c->b.sums);
 }
 
 template <int MODE>

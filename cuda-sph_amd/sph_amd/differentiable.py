@@ -7,7 +7,7 @@ FP32 by libsph_amd.  The backward pass is hand-written (libsph_amd's sph_adjoint
 the rollout keeps the state every `checkpoint_every` steps, and for each segment, last to
 first, recomputes the segment's states from its checkpoint and pulls the cotangent back
 through the steps one at a time.  Memory: (T / k + k) states of 24 bytes per particle.
-When T step snapshots fit in KEEP_ALL_BYTES of device memory (about 240 bytes per particle
+When T step snapshots fit in KEEP_ALL_BYTES of device memory (about 270 bytes per particle
 each), the rollout keeps them all and the backward pass runs no forward step at all.
 
     sim = DifferentiableSPH(params, constants, steps=200, checkpoint_every=20)
@@ -49,7 +49,7 @@ class DifferentiableSPH:
     constants are kernel arguments).  snapshots: the backward's recompute pass keeps each
     step's cell-ordered state, densities, lists and cell table (sph_step_save), so that
     reversing the step does not sort and recompute it again; it costs checkpoint_every
-    snapshots of device memory (about 240 bytes per particle each) and gives the same
+    snapshots of device memory (about 270 bytes per particle each) and gives the same
     gradients bit for bit.  keep_all: the forward rollout itself keeps all `steps` snapshots
     (steps times that memory), so that the backward of the latest forward call reverses
     every step from them without recomputing any (an older tape falls back to its

@@ -243,7 +243,8 @@ int sph_adjoint_step(sph_ctx* ctx, void* stream);
 int sph_adjoint_read(sph_ctx* ctx, float* dev_gx, float* dev_gv, float* dev_gmu, void* stream);
 
 /* Step snapshots (no reference counterpart): what sph_adjoint_step recomputes before its
- * pullback -- the step's cell-ordered state, densities, neighbour lists and cell table --
+ * pullback -- the step's cell-ordered state, densities, neighbour lists, cell table and
+ * force sums --
  * saved right after the forward step that produced them, so that a checkpointed backward
  * can reverse each step without sorting and recomputing it a second time.
  *   sph_adjoint_snapshot_bytes  bytes of one snapshot of this context (0 if it has no

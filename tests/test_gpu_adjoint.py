@@ -247,7 +247,7 @@ def test_snapshots_give_the_recomputed_gradients(gpu, lanes):
                                   graphs=graphs, **k) for k in kinds]
         assert [s.keep_all for s in sims] == [False, False, True]
         nbytes = sims[1].ctx.adjoint_snapshot_bytes()
-        assert nbytes % 256 == 0 and nbytes >= 236 * 3000
+        assert nbytes % 256 == 0 and nbytes >= 268 * 3000
         ref = None
         for mu in (3.5, 5.0):
             outs = [s.forward(dev(x, gpu), dev(v, gpu), mu) for s in sims]
@@ -267,9 +267,10 @@ def test_snapshots_give_the_recomputed_gradients(gpu, lanes):
 
 
 def _snapshot_regions(n, cells, cap=96):
-    """(offset, bytes) of the seven parts of a step snapshot: posA, velA, pid, rho, list
-    counts, lists (16-bit slots in pairs, [cap / 2][n][2]), cell table (cells + 1 entries)."""
-    sizes = [16 * n, 16 * n, 4 * n, 4 * n, 4 * n, 2 * cap * n, 4 * (cells + 1)]
+    """(offset, bytes) of the eight parts of a step snapshot: posA, velA, pid, rho, list
+    counts, lists (16-bit slots in pairs, [cap / 2][n][2]), cell table (cells + 1 entries),
+    the force pass's pressure and viscosity sums ([2][n] float4)."""
+    sizes = [16 * n, 16 * n, 4 * n, 4 * n, 4 * n, 2 * cap * n, 4 * (cells + 1), 32 * n]
     out, at = [], 0
     for b in sizes:
         out.append((at, b))
