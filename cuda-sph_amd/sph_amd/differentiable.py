@@ -7,8 +7,9 @@ FP32 by libsph_amd.  The backward pass is hand-written (libsph_amd's sph_adjoint
 the rollout keeps the state every `checkpoint_every` steps, and for each segment, last to
 first, recomputes the segment's states from its checkpoint and pulls the cotangent back
 through the steps one at a time.  Memory: (T / k + k) states of 24 bytes per particle.
-When T step snapshots fit in KEEP_ALL_BYTES of device memory (about 270 bytes per particle
-each), the rollout keeps them all and the backward pass runs no forward step at all.
+When T step snapshots (about 270 bytes per particle each) fit in KEEP_ALL_FRACTION of the
+device memory free when the simulator is made, the rollout keeps them all and the backward
+pass runs no forward step at all (1M particles over 200 steps: 57 GB).
 
     sim = DifferentiableSPH(params, constants, steps=200, checkpoint_every=20)
     xT, vT = rollout(sim, x0, v0, mu)          # x0, v0 (N, 3) float32 cuda; mu a tensor
@@ -28,8 +29,9 @@ from .constants import SPHConstants
 from .data_classes import SimulationParameters
 
 
-# Device memory the rollout may spend on keeping every step's snapshot (keep_all=None).
-KEEP_ALL_BYTES = 16 << 30
+# Share of the free device memory the rollout may spend on keeping every step's snapshot
+# (keep_all=None).
+KEEP_ALL_FRACTION = 0.5
 
 
 class Tape:
@@ -53,7 +55,8 @@ class DifferentiableSPH:
     gradients bit for bit.  keep_all: the forward rollout itself keeps all `steps` snapshots
     (steps times that memory), so that the backward of the latest forward call reverses
     every step from them without recomputing any (an older tape falls back to its
-    checkpoints); None keeps them when they fit in KEEP_ALL_BYTES."""
+    checkpoints); None keeps them when they fit in KEEP_ALL_FRACTION of the free device
+    memory.  The choice changes speed and memory only: the gradients are the same bits."""
 
     def __init__(self, params: SimulationParameters, constants: SPHConstants, steps: int, checkpoint_every: int = 20,
                  device="cuda:0", graphs: bool = True, launch: Optional[Tuple[int, int]] = None, lanes: int = 1,
@@ -82,7 +85,8 @@ class DifferentiableSPH:
         if nbytes <= 0:
             raise RuntimeError("libsph_amd gives no step snapshots for this context")
         if keep_all is None:
-            keep_all = self.snapshots and self.steps * nbytes <= KEEP_ALL_BYTES
+            free, _ = torch.cuda.mem_get_info(self.device)
+            keep_all = self.snapshots and self.steps * nbytes <= KEEP_ALL_FRACTION * free
         self.keep_all = bool(keep_all)
         if self.snapshots:
             self._snap = torch.empty((self.every, nbytes), dtype=torch.uint8, device=self.device)
