@@ -136,7 +136,8 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_seed_kernel(
     const uint32_t* __restrict__ nn, float4* __restrict__ posOut, float4* __restrict__ velOut,
     uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
     uint32_t* __restrict__ flags, const float4* __restrict__ cotx, const float4* __restrict__ cotv,
-    float4* __restrict__ fb, float4* __restrict__ gxs, float4* __restrict__ gvs, double* __restrict__ mu_part) {
+    float4* __restrict__ fb, float4* __restrict__ gxs, float4* __restrict__ gvs, double* __restrict__ mu_part,
+    float4* __restrict__ sums) {
     // L lanes per particle, as the step's force pass uses, so the recomputed sums are its own
     const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int i = (int)(gt / L);
@@ -144,6 +145,9 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_seed_kernel(
     const float4 p = pos[i], v = vel[i];
     const PairSums S = naive_pair_sums_lanes<L>(C, i, gt & (L - 1), C.n, p, v, pos, vel, cs, nl, nn);
     if (L > 1 && (gt & (L - 1)) != 0) return;
+    // kept as the force pass keeps them, so that a snapshot saved after this step is whole
+    sums[i] = make_float4(S.ax, S.ay, S.az, 0.0f);
+    sums[C.n + i] = make_float4(S.bx, S.by, S.bz, 0.0f);
     seed_particle(C, i, p, v, S, pidA, rhoA, posOut, velOut, pidOut, keyI, valI, flags, cotx, cotv, fb, gxs, gvs,
                   mu_part);
 }
@@ -345,7 +349,7 @@ void launch_pullback_lanes(sph_ctx* c, const Buffers& b, hipStream_t s, bool fro
     else
         adjoint_seed_kernel<L><<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.velA, b.pidA, b.rhoA, b.cs, b.nl, b.nn, b.posB,
                                                        b.velB, b.pidB, b.keyI, b.valI, b.flags, b.cotx, b.cotv,
-                                                       b.adj_fb, b.adj_gx, b.adj_gv, b.mu_part);
+                                                       b.adj_fb, b.adj_gx, b.adj_gv, b.mu_part, b.sums);
     adjoint_pair_kernel<L><<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.velA, b.pidA, b.cs, b.nl, b.nn, b.adj_fb,
                                                        b.adj_gx, b.adj_gv, b.adj_rt, b.cotv);
     adjoint_density_kernel<L><<<nb, kAdjThreads, 0, s>>>(c->sc, b.posA, b.pidA, b.cs, b.nl, b.nn, b.adj_rt, b.adj_gx,

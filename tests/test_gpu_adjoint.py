@@ -316,6 +316,17 @@ def test_step_save_writes_what_adjoint_save_copies(gpu, lanes):
              .transpose(0, 2, 1).reshape(96, a.n) for s in (s0, s1)]
     used = np.arange(96)[:, None] < np.minimum(nn, 96)[None, :]
     assert used.sum() > 0 and np.array_equal(lists[0][used], lists[1][used])
+    # a snapshot saved after the recompute path's step (sph_adjoint_step) is the same too
+    c = DifferentiableSPH(w.params, w.constants, steps=1, device=gpu, lanes=lanes, graphs=False).ctx
+    c.upload_f32(*[t.contiguous() for t in xs.t()], *[t.contiguous() for t in vs.t()])
+    c.adjoint_seed(torch.zeros_like(xs), None)
+    c.adjoint_step()
+    snap_c = torch.zeros(nbytes, dtype=torch.uint8, device=gpu)
+    c.adjoint_save(snap_c)
+    sc = snap_c.cpu().numpy()
+    for k, (off, size) in enumerate(regions):
+        if k != 5:
+            assert np.array_equal(s0[off:off + size], sc[off:off + size]), k
     # the pullback from either snapshot
     g = torch.sin(xs)
     for ctx, snap in ((a, snaps[0]), (b, snaps[1])):
