@@ -33,6 +33,28 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
+def declared_arity():
+    """Number of parameters of each function include/sph_amd.h declares."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    out = {}
+    for name, args in re.findall(r"^\s*(?:int|size_t|int64_t)\s+(sph_\w+)\s*\(([^)]*)\)\s*;", text, flags=re.M):
+        args = args.strip()
+        out[name] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def test_ctypes_signatures_match_the_header():
+    """Every binding passes as many arguments as the C declaration takes (a stale ctypes
+    signature would shift pointers and sizes)."""
+    if not os.path.exists(LIB_PATH):
+        pytest.skip("libsph_amd.so not built")
+    lib = N.load_library()
+    arity = declared_arity()
+    assert set(arity) == set(declared())
+    for name, count in arity.items():
+        assert len(getattr(lib, name).argtypes) == count, name
+
+
 def _params(**over):
     kw = dict(n=1000, external_force=(0, -9.81, 0), dt=1e-3, space_size=(1, 1, 1), voxel_size=(0.0625,) * 3,
               mode=N.SPH_MODE_BOX, formulation=N.SPH_FORM_NAIVE)
