@@ -279,7 +279,7 @@ def bench_frames(n: int = 65536, frames: int = 50):
         for state in itertools.islice(gen, frames):
             saver.save_next_state(state)
         t_sync = time.perf_counter() - t0
-        strat = NaiveSPHStrategy(params, constants=w.constants)
+        strat = NaiveSPHStrategy(params, constants=w.constants, check_finite=False)
         t0 = time.perf_counter()
         run_to_disk(strat, w.state, params, "async", root=root, frames=frames)
         t_async = time.perf_counter() - t0
@@ -303,7 +303,7 @@ def bench_small(warmup: int = 3, tune: bool = True):
     w = workloads.box_4k()
     n = w.params.particle_count
     params, launch, lanes, tuning = tune_workload(w, enabled=tune)
-    strat = NaiveSPHStrategy(params, constants=w.constants, launch=launch, lanes=lanes)
+    strat = NaiveSPHStrategy(params, constants=w.constants, launch=launch, lanes=lanes, check_finite=False)
     pos = torch.from_numpy(np.ascontiguousarray(w.state.position)).cuda()
     vel = torch.from_numpy(np.ascontiguousarray(w.state.velocity)).cuda()
     strat.ctx.upload(pos, vel)
@@ -334,7 +334,8 @@ def bench_pipe(steps: int = 1000, warmup: int = 5, tune: bool = True, graph: boo
     w = workloads.pipe_flow()
     n = w.params.particle_count
     params, launch, lanes, tuning = tune_workload(w, "PIPE", enabled=tune)
-    strat = NaiveSPHStrategy(params, constants=w.constants, mode="PIPE", launch=launch, lanes=lanes)
+    strat = NaiveSPHStrategy(params, constants=w.constants, mode="PIPE", launch=launch, lanes=lanes,
+                             check_finite=False)
     pos = torch.from_numpy(np.ascontiguousarray(w.state.position)).cuda()
     vel = torch.from_numpy(np.ascontiguousarray(w.state.velocity)).cuda()
     run = timed_steps(strat, steps, graph, pos, vel)
@@ -371,7 +372,7 @@ def bench_voxel_form(steps: int = 500, warmup: int = 5):
     w = workloads.dam_break_1m()
     w.params = dataclasses.replace(w.params, voxel_size=np.asarray([2.0 ** -6] * 3))
     n = w.params.particle_count
-    strat = VoxelSPHStrategy(w.params, constants=w.constants)
+    strat = VoxelSPHStrategy(w.params, constants=w.constants, check_finite=False)
     pos = torch.from_numpy(np.ascontiguousarray(w.state.position)).cuda()
     vel = torch.from_numpy(np.ascontiguousarray(w.state.velocity)).cuda()
     strat.ctx.upload(pos, vel)
@@ -467,7 +468,8 @@ def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
         best = None
         for sc, la, ln in dict.fromkeys(configs):
             gv = None if sc is None else workloads.voxel_for(w.constants.h) * sc
-            strat = NaiveSPHStrategy(w.params, constants=w.constants, grid_voxel=gv, launch=la, lanes=ln)
+            strat = NaiveSPHStrategy(w.params, constants=w.constants, grid_voxel=gv, launch=la, lanes=ln,
+                                             check_finite=False)
             for graph in (False, True):
                 run = timed_steps(strat, steps, graph, pos, vel)
                 strat.ctx.upload(pos, vel)
@@ -528,18 +530,36 @@ def bench_adjoint(steps: int = 200, every: int = 20, tune: bool = True):
         return loss, grads
 
     a, b, c, d = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    run()  # warm-up
-    torch.cuda.synchronize()
-    a.record()
-    loss, (gx0, gv0, gmu) = run()
-    torch.cuda.synchronize()
-    t_f, t_b = a.elapsed_time(b) / 1e3, c.elapsed_time(d) / 1e3
-    log(f"adjoint: forward {t_f * 1e3:.1f} ms, backward {t_b * 1e3:.1f} ms for {steps} steps of {n} particles")
+
+    def timed():
+        run()  # warm-up: direct launches (a new viscosity)
+        run()  # warm-up: records the HIP graphs of this viscosity
+        torch.cuda.synchronize()
+        a.record()
+        out = run()
+        torch.cuda.synchronize()
+        return out, a.elapsed_time(b) / 1e3, c.elapsed_time(d) / 1e3
+
+    loss, (gx0, gv0, gmu) = timed()[0]
+    _, t_f, t_b = timed()
+    log(f"adjoint: forward {t_f * 1e3:.1f} ms, backward {t_b * 1e3:.1f} ms for {steps} steps of {n} particles"
+        f" (every step's snapshot kept: {sim.keep_all})")
+    # the configuration as written: checkpoints every `every` steps, each segment recomputed
+    # from its checkpoint in the backward pass
+    keep_all = sim.keep_all
+    sim.set_keep_all(False)
+    (_, (_, gv0c, gmuc)), t_fc, t_bc = timed()
+    sim.set_keep_all(keep_all)
+    log(f"adjoint, checkpoints every {every}: forward {t_fc * 1e3:.1f} ms, backward {t_bc * 1e3:.1f} ms")
+    same = bool(torch.equal(gv0c, gv0) and torch.equal(gmuc, gmu))
     return {"workload": "adjoint_16k", "particles": n, "steps": steps, "checkpoint_every": every,
-            "keeps_every_step": sim.keep_all, "snapshot_mb": round(sim.ctx.adjoint_snapshot_bytes() / 1e6, 2),
+            "keeps_every_step": keep_all, "snapshot_mb": round(sim.ctx.adjoint_snapshot_bytes() / 1e6, 2),
             "value": round(n * steps / (t_f + t_b), 1), "unit": "particle-updates/s",
             "ms_forward": round(t_f * 1e3, 3), "ms_backward": round(t_b * 1e3, 3),
             "ms_per_adjoint_step": round(t_b * 1e3 / steps, 4),
+            "checkpointed_value": round(n * steps / (t_fc + t_bc), 1),
+            "checkpointed_ms_forward": round(t_fc * 1e3, 3), "checkpointed_ms_backward": round(t_bc * 1e3, 3),
+            "checkpointed_grads_bitwise_equal": same,
             "loss": float(loss), "grad_mu": float(gmu.item()), "grad_v0_norm": float(gv0.double().norm()),
             "finite": bool(torch.isfinite(gv0).all() and torch.isfinite(gmu).all()), "tuning": tuning}
 
@@ -568,7 +588,8 @@ def main() -> int:
     w.params, launch, lanes, tuning = tune_workload(w, w.mode, enabled=tune)
     if args.block:
         launch = tuple(args.block)
-    strat = NaiveSPHStrategy(w.params, constants=w.constants, mode=w.mode, device=dev, launch=launch, lanes=lanes)
+    strat = NaiveSPHStrategy(w.params, constants=w.constants, mode=w.mode, device=dev, launch=launch, lanes=lanes,
+                             check_finite=False)
     pos = torch.from_numpy(np.ascontiguousarray(w.state.position)).to(dev)
     vel = torch.from_numpy(np.ascontiguousarray(w.state.velocity)).to(dev)
     graph = args.graph == "on"

@@ -166,6 +166,7 @@ int plan_instance(const sph_params_t& P, const sph_constants_t& K, sph::StepCons
         S.ext[d] = (float)P.external_force[d];
     }
     if (cells > (int64_t(1) << 28)) return fail(SPH_E_INVALID, "voxel grid of %lld cells is too large", (long long)cells);
+    S.voxel_skip = S.dims[0] == S.tdims[0] && S.dims[1] == S.tdims[1] && S.dims[2] == S.tdims[2] ? 1 : 0;
     S.n_cells = (int)cells;
     S.cs_cells = (int)cells;
     S.box_lo_target = 1e-3f;
@@ -233,6 +234,7 @@ int plan(sph_ctx* c) {
     if (rc) return rc;
     c->n = c->prm.particle_count;
     c->keep_terms = (c->prm.flags & SPH_FLAG_KEEP_TERMS) ? 1 : 0;
+    c->check_finite = (c->prm.flags & SPH_FLAG_CHECK_FINITE) ? 1 : 0;
     c->adjoint = (c->prm.flags & SPH_FLAG_ADJOINT) ? 1 : 0;
     if (c->adjoint && (c->prm.formulation != SPH_FORM_NAIVE || c->prm.mode != SPH_MODE_BOX))
         return fail(SPH_E_INVALID, "SPH_FLAG_ADJOINT needs the NAIVE formulation in BOX mode");
@@ -326,6 +328,7 @@ int plan_batch(sph_ctx* c, const sph_params_t* params, const sph_constants_t* co
     c->n = n;
     if (int rc = batch_thread_layout(c, std::vector<int>(count, 1))) return rc;
     c->keep_terms = (params[0].flags & SPH_FLAG_KEEP_TERMS) ? 1 : 0;
+    c->check_finite = (params[0].flags & SPH_FLAG_CHECK_FINITE) ? 1 : 0;
     memset(&c->pipe_host, 0, sizeof(c->pipe_host));
     plan_sort_and_layout(c);
     return SPH_OK;
@@ -356,6 +359,26 @@ int run_step_stages(sph_ctx* c, hipStream_t s, bool fused) {
     c->last_stage = SPH_STAGE_FORCE;
     return SPH_OK;
 }
+
+// Reads and clears the count of non-finite positions hashed since the last read (waits for
+// the stream).  Returns SPH_E_NONFINITE if it was not zero.
+int read_nonfinite(sph_ctx* c, hipStream_t s, int64_t* count) {
+    uint32_t host = 0;
+    int rc;
+    if ((rc = hip_check(hipMemcpyAsync(&host, c->b.flags + sph::kFlagNonFinite, sizeof(host), hipMemcpyDeviceToHost, s),
+                        "read non-finite count")))
+        return rc;
+    if ((rc = hip_check(sph::launch_fill_u32(c->b.flags + sph::kFlagNonFinite, 0u, 1, s), "clear non-finite count")))
+        return rc;
+    if ((rc = hip_check(hipStreamSynchronize(s), "check finite"))) return rc;
+    if (count) *count = host;
+    if (host != 0)
+        return fail(SPH_E_NONFINITE, "%u particle positions were NaN or infinite when hashed (since the last check)",
+                    host);
+    return SPH_OK;
+}
+
+void assign_buffers(sph_ctx* c, void* dev);
 
 int ready(const sph_ctx* c, bool need_state) {
     if (!c) return fail(SPH_E_INVALID, "null context");
@@ -438,6 +461,38 @@ int sph_bind_workspace(sph_ctx* c, void* dev, size_t bytes, void* stream) {
     if (bytes < c->ws_bytes) return fail(SPH_E_INVALID, "workspace of %zu bytes, need %zu", bytes, c->ws_bytes);
     if (((uintptr_t)dev & 255) != 0) return fail(SPH_E_INVALID, "workspace must be 256-byte aligned");
     hipStream_t s = (hipStream_t)stream;
+    assign_buffers(c, dev);
+    sph::Buffers& b = c->b;
+    int rc;
+    if ((rc = hip_check(sph::launch_fill_u32(b.flags, 0u, 16, s), "clear flags"))) return rc;
+    if (c->cell_sort && (rc = hip_check(sph::launch_fill_u32(b.cell_cnt, 0u, c->sc.cs_cells, s), "clear cell counts")))
+        return rc;
+    if ((rc = hip_check(hipMemcpyAsync(b.pipe, &c->pipe_host, sizeof(sph::PipeTable), hipMemcpyHostToDevice, s),
+                        "upload pipe table")))
+        return rc;
+    if (c->prm.mode == SPH_MODE_PIPE && c->n > 0)
+        if ((rc = hip_check(hipMemcpyAsync(b.rng, c->rng_host, 16 * (size_t)c->n, hipMemcpyHostToDevice, s),
+                            "upload rng streams")))
+            return rc;
+    if (!c->inst.empty()) {
+        c->sc.inst_table = b.inst_table;
+        c->sc.inst_first = b.inst_first;
+        if ((rc = upload_instances(c, s))) return rc;
+    }
+    if ((rc = hip_check(hipStreamSynchronize(s), "bind"))) return rc;
+    c->bound = true;
+    c->have_state = false;
+    c->adj_seeded = false;
+    return SPH_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Points the context's buffers at their slices of the device memory at dev (256-byte
+// aligned, at least c->ws_bytes).
+void assign_buffers(sph_ctx* c, void* dev) {
     Layout L = plan_layout(c);
     char* base = (char*)dev;
     sph::Buffers& b = c->b;
@@ -483,28 +538,11 @@ int sph_bind_workspace(sph_ctx* c, void* dev, size_t bytes, void* stream) {
     c->ws = dev;
     c->sorted_keys = b.keyI;
     c->sorted_vals = b.valI;
-    int rc;
-    if ((rc = hip_check(sph::launch_fill_u32(b.flags, 0u, 16, s), "clear flags"))) return rc;
-    if (c->cell_sort && (rc = hip_check(sph::launch_fill_u32(b.cell_cnt, 0u, c->sc.cs_cells, s), "clear cell counts")))
-        return rc;
-    if ((rc = hip_check(hipMemcpyAsync(b.pipe, &c->pipe_host, sizeof(sph::PipeTable), hipMemcpyHostToDevice, s),
-                        "upload pipe table")))
-        return rc;
-    if (c->prm.mode == SPH_MODE_PIPE && c->n > 0)
-        if ((rc = hip_check(hipMemcpyAsync(b.rng, c->rng_host, 16 * (size_t)c->n, hipMemcpyHostToDevice, s),
-                            "upload rng streams")))
-            return rc;
-    if (!c->inst.empty()) {
-        c->sc.inst_table = b.inst_table;
-        c->sc.inst_first = b.inst_first;
-        if ((rc = upload_instances(c, s))) return rc;
-    }
-    if ((rc = hip_check(hipStreamSynchronize(s), "bind"))) return rc;
-    c->bound = true;
-    c->have_state = false;
-    c->adj_seeded = false;
-    return SPH_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int sph_upload_state(sph_ctx* c, const double* pos, const double* vel, void* stream) {
     int rc = ready(c, false);
@@ -539,7 +577,21 @@ int sph_step(sph_ctx* c, int32_t nsteps, void* stream) {
         if ((rc = run_step_stages(c, (hipStream_t)stream, true))) return rc;
         c->a_in_snapshot = false;
     }
+    if (c->check_finite && nsteps > 0) {
+        // a stream being captured into a graph cannot be waited on: the check is left to the
+        // next call outside the capture (the count keeps accumulating)
+        hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+        if ((rc = hip_check(hipStreamIsCapturing((hipStream_t)stream, &st), "capture status"))) return rc;
+        if (st == hipStreamCaptureStatusNone) return read_nonfinite(c, (hipStream_t)stream, nullptr);
+    }
     return SPH_OK;
+}
+
+int sph_check_finite(sph_ctx* c, int64_t* count, void* stream) {
+    if (count) *count = 0;
+    int rc = ready(c, false);
+    if (rc) return rc;
+    return read_nonfinite(c, (hipStream_t)stream, count);
 }
 
 int sph_run_stage(sph_ctx* c, int32_t stage, void* stream) {
@@ -808,5 +860,6 @@ int sph_adjoint_read(sph_ctx* c, float* gx, float* gv, float* gmu, void* stream)
     if (!c->adj_seeded) return fail(SPH_E_STATE, "no cotangent (call sph_adjoint_seed)");
     return hip_check(sph::launch_adjoint_get(c, gx, gv, gmu, (hipStream_t)stream), "adjoint read");
 }
+
 
 }  // extern "C"

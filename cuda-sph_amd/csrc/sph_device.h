@@ -111,7 +111,18 @@ __device__ __forceinline__ void hash_particle(const StepConsts& C, float x, floa
         keyI[pid] = sort_key(x, y, z, C);
         valI[pid] = slot;
     }
-    if (!finite3(x, y, z)) atomicAdd(flags, 1u);
+    if (!finite3(x, y, z)) atomicAdd(flags + kFlagNonFinite, 1u);
+}
+
+// VOXEL formulation: whether a finite position lies outside the voxel grid, i.e. its key
+// (truncated voxel index per axis, voxel_kernels.py:9-17) leaves [0, dims) on some axis and
+// so wraps onto the key of another voxel or off the table.  Positions in (-voxel, 0) truncate
+// into voxel 0 and count as inside (the density pass treats voxel 0 as open below).
+// Non-finite positions count as inside: no pair test ever accepts them.
+__device__ __forceinline__ bool outside_voxel_grid(float x, float y, float z, const StepConsts& C) {
+    if (!finite3(x, y, z)) return false;
+    const int32_t vx = ref_coord(x, 0, C), vy = ref_coord(y, 1, C), vz = ref_coord(z, 2, C);
+    return vx < 0 || vx >= C.dims[0] || vy < 0 || vy >= C.dims[1] || vz < 0 || vz >= C.dims[2];
 }
 
 // ---------------------------------------------------------------------------------------

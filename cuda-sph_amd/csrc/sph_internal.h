@@ -43,6 +43,12 @@ constexpr int kListCap = SPH_LIST_CAP;  // NAIVE neighbour-list slots per partic
 constexpr int kLdsListCap = SPH_LDS_LIST_CAP;  // of which staged in LDS by the density pass
 constexpr int kXcds = 8;            // MI355X: 8 accelerator complex dies, one L2 each
 
+// Words of the flags buffer.
+constexpr int kFlagNonFinite = 0;   // non-finite positions hashed since the last sph_check_finite
+constexpr int kFlagOutOfGrid = 1;   // VOXEL: a finite position of this step's sorted state lies
+                                    // outside the voxel grid (set by the gather into cell order,
+                                    // cleared by the first pass of the sort)
+
 // NAIVE neighbour-list entry: the neighbour's cell-ordered index minus the particle's own, as
 // int16 (neighbours in space are near each other in cell order).  A particle with a
 // neighbour farther away than that in cell order is marked for a rescan instead.
@@ -75,6 +81,12 @@ struct StepConsts {
     float box_lo_target;     // 1e-3                  (base_kernels.py:91)
     float box_hi_target[3];  // space - 1e-3          (base_kernels.py:94)
     int pipe_rows;
+    // VOXEL formulation: 1 when the lookup dims equal the key dims on every axis, so that
+    // cell-table entry ci holds the particles of the voxel that ci names; only then may the
+    // density pass skip a voxel whose slab lies beyond h (and only while no particle of the
+    // sorted state lies outside the voxel grid, which the sort's gather reports in
+    // flags[kFlagOutOfGrid])
+    int voxel_skip;
     // batched form: this instance's first cell key and first particle (0 for a plain step);
     // the kernel argument holds the totals, the device table one entry per instance
     int cell_off;
@@ -111,7 +123,7 @@ struct Buffers {
     ListEntry* nl;         // NAIVE neighbour lists, column-major [kListCap][N] (index offsets, self included)
     uint32_t* nn;          // NAIVE neighbour count per cell-ordered particle (may exceed kListCap)
     double* diag_partials;
-    uint32_t* flags;       // [0] count of non-finite positions seen by the hash
+    uint32_t* flags;       // kFlagNonFinite, kFlagOutOfGrid (16 words)
     PipeTable* pipe;
     StepConsts* inst_table;
     int* inst_first;
@@ -148,6 +160,7 @@ struct sph_ctx {
     int density_block, force_block;
     int lanes;               // NAIVE formulation: lanes of a wavefront per particle in density / force (1..16)
     int keep_terms;
+    int check_finite;        // SPH_FLAG_CHECK_FINITE: sph_step reads the non-finite count
     int adjoint;             // SPH_FLAG_ADJOINT: reverse-mode buffers are part of the workspace
     int cell_sort;           // NAIVE formulation: counting sort by cell instead of the radix sort
     bool adj_seeded;

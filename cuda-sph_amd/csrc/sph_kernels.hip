@@ -92,16 +92,22 @@ __global__ __launch_bounds__(256) void cell_table_kernel(StepConsts C, const uin
     }
 }
 
-__global__ __launch_bounds__(256) void reorder_kernel(int n, const uint32_t* __restrict__ svals,
+// Gather of the state into cell order (the staged step's own pass; the fused step gathers in
+// the sort's last pass).  VOXEL: reports a position outside the voxel grid in
+// flags[kFlagOutOfGrid], which the sort cleared.
+__global__ __launch_bounds__(256) void reorder_kernel(StepConsts C, const uint32_t* __restrict__ svals,
                                                        const float4* __restrict__ posB, const float4* __restrict__ velB,
                                                        const uint32_t* __restrict__ pidB, float4* __restrict__ posA,
-                                                       float4* __restrict__ velA, uint32_t* __restrict__ pidA) {
+                                                       float4* __restrict__ velA, uint32_t* __restrict__ pidA,
+                                                       uint32_t* __restrict__ flags) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
+    if (j >= C.n) return;
     const uint32_t s = svals[j];
-    posA[j] = posB[s];
+    const float4 p = posB[s];
+    posA[j] = p;
     velA[j] = velB[s];
     pidA[j] = pidB[s];
+    if (C.formulation == SPH_FORM_VOXEL && outside_voxel_grid(p.x, p.y, p.z, C)) flags[kFlagOutOfGrid] = 1u;
 }
 
 // Writes pressure and 1/rho into the .w lanes of posA / velA (4-byte stores; the .w lanes
@@ -399,9 +405,14 @@ __device__ __forceinline__ void voxel_slab2(float p, int v, float s, float e2[3]
 
 __global__ __launch_bounds__(256) void voxel_density_kernel(StepConsts C, float4* pos, float4* vel,
                                                              const uint32_t* __restrict__ cs,
-                                                             float* __restrict__ rhoA, int32_t* __restrict__ nbr) {
+                                                             float* __restrict__ rhoA, int32_t* __restrict__ nbr,
+                                                             const uint32_t* __restrict__ flags) {
     const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= C.n) return;
+    // a voxel beyond h may be skipped only if every particle it lists lies in its slab: the
+    // lookup dims name the voxels the keys do and no position of the sorted state lies
+    // outside the grid (a wrapped key lists such a particle under another voxel)
+    const bool skip_ok = C.voxel_skip && __builtin_amdgcn_readfirstlane(flags[kFlagOutOfGrid]) == 0u;
     const size_t n = (size_t)C.n;
     const float4 p = pos[i];
     const int vx = ref_coord(p.x, 0, C), vy = ref_coord(p.y, 1, C), vz = ref_coord(p.z, 2, C);
@@ -418,7 +429,7 @@ __global__ __launch_bounds__(256) void voxel_density_kernel(StepConsts C, float4
             for (int dz = -1; dz < 2 && cnt < kMaxNeighbours; ++dz) {
                 const int nx = vx + dx, ny = vy + dy, nz = vz + dz;
                 if (nx < 0 || nx >= C.tdims[0] || ny < 0 || ny >= C.tdims[1] || nz < 0 || nz >= C.tdims[2]) continue;
-                if (ex[dx + 1] + ey[dy + 1] + ez[dz + 1] > hs2) continue;
+                if (skip_ok && ex[dx + 1] + ey[dy + 1] + ez[dz + 1] > hs2) continue;
                 const int ci = nx + ny * C.tdims[0] + nz * C.tdims[0] * C.tdims[1];
                 uint2 se;  // cs[ci] and cs[ci + 1] in one load
                 __builtin_memcpy(&se, cs + ci, sizeof(se));
@@ -693,8 +704,8 @@ hipError_t launch_cell_table(sph_ctx* c, hipStream_t s) {
 
 hipError_t launch_reorder(sph_ctx* c, hipStream_t s) {
     if (c->n > 0)
-        reorder_kernel<<<blocks_for(c->n, 256), 256, 0, s>>>((int)c->n, c->sorted_vals, c->b.posB, c->b.velB,
-                                                               c->b.pidB, c->b.posA, c->b.velA, c->b.pidA);
+        reorder_kernel<<<blocks_for(c->n, 256), 256, 0, s>>>(c->sc, c->sorted_vals, c->b.posB, c->b.velB,
+                                                               c->b.pidB, c->b.posA, c->b.velA, c->b.pidA, c->b.flags);
     return hipGetLastError();
 }
 
@@ -710,7 +721,7 @@ hipError_t launch_density(sph_ctx* c, hipStream_t s) {
     const int bs = c->density_block;
     if (c->sc.formulation == SPH_FORM_VOXEL) {
         voxel_density_kernel<<<blocks_for(c->n, bs), bs, 0, s>>>(c->sc, c->b.posA, c->b.velA, c->b.cs, c->b.rhoA,
-                                                                   c->b.nbr);
+                                                                   c->b.nbr, c->b.flags);
         return hipGetLastError();
     }
     if (!c->inst.empty()) {

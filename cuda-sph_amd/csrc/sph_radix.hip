@@ -16,7 +16,7 @@
 //             digit order and writes each digit run contiguously.  On the last pass it can
 //             also gather the particle state into cell order (posB/velB/pidB -> A), so
 //             the separate reorder pass disappears.
-#include "sph_internal.h"
+#include "sph_device.h"
 
 namespace sph {
 namespace {
@@ -53,8 +53,12 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds_wa
 }
 
 __global__ __launch_bounds__(ST) void radix_upsweep(const uint32_t* __restrict__ keys, int n, int shift,
-                                                     uint32_t mask, uint32_t* __restrict__ hist, int ntiles) {
+                                                     uint32_t mask, uint32_t* __restrict__ hist, int ntiles,
+                                                     uint32_t* __restrict__ clear_flag) {
     __shared__ uint32_t h[WAVES][RADIX];
+    // the first pass clears the out-of-grid flag that this step's gather sets (the previous
+    // step's density pass, its only reader, has finished)
+    if (clear_flag && blockIdx.x == 0 && threadIdx.x == 0) *clear_flag = 0u;
     for (int i = threadIdx.x; i < WAVES * RADIX; i += ST) (&h[0][0])[i] = 0;
     __syncthreads();
     const int w = threadIdx.x >> 6;
@@ -88,6 +92,9 @@ __global__ __launch_bounds__(ST) void radix_scan_digits(uint32_t* __restrict__ h
     if (threadIdx.x == 0) digit_total[blockIdx.x] = carry;
 }
 
+// GATHER (the last pass of the fused step): also gathers the state into cell order and, for
+// the VOXEL formulation (C.formulation), reports a position outside the voxel grid in
+// flags[kFlagOutOfGrid].
 template <bool GATHER>
 __global__ __launch_bounds__(ST) void radix_downsweep(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                        uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int n,
@@ -95,7 +102,8 @@ __global__ __launch_bounds__(ST) void radix_downsweep(const uint32_t* __restrict
                                                        const uint32_t* __restrict__ digit_total, int ntiles,
                                                        const float4* __restrict__ posB, const float4* __restrict__ velB,
                                                        const uint32_t* __restrict__ pidB, float4* __restrict__ posA,
-                                                       float4* __restrict__ velA, uint32_t* __restrict__ pidA) {
+                                                       float4* __restrict__ velA, uint32_t* __restrict__ pidA,
+                                                       StepConsts C, uint32_t* __restrict__ flags) {
     __shared__ uint32_t s_keys[TILE];
     __shared__ uint32_t s_vals[TILE];
     __shared__ uint32_t s_whist[WAVES][RADIX];
@@ -177,9 +185,11 @@ __global__ __launch_bounds__(ST) void radix_downsweep(const uint32_t* __restrict
         kout[g] = kk;
         vout[g] = vv;
         if (GATHER) {
-            posA[g] = posB[vv];
+            const float4 p = posB[vv];
+            posA[g] = p;
             velA[g] = velB[vv];
             pidA[g] = pidB[vv];
+            if (C.formulation == SPH_FORM_VOXEL && outside_voxel_grid(p.x, p.y, p.z, C)) flags[kFlagOutOfGrid] = 1u;
         }
     }
 }
@@ -205,16 +215,17 @@ hipError_t launch_radix_sort_impl(sph_ctx* c, bool gather, hipStream_t s) {
         uint32_t* vout = c->b.valS[p & 1];
         const int shift = c->pass_shift[p], bits = c->pass_bits[p];
         const uint32_t mask = (1u << bits) - 1u;
-        radix_upsweep<<<T, ST, 0, s>>>(kin, n, shift, mask, c->b.tile_hist, T);
+        radix_upsweep<<<T, ST, 0, s>>>(kin, n, shift, mask, c->b.tile_hist, T,
+                                         p == 0 ? c->b.flags + kFlagOutOfGrid : nullptr);
         radix_scan_digits<<<RADIX, ST, 0, s>>>(c->b.tile_hist, T, c->b.digit_total);
         if (gather && p == c->passes - 1)
             radix_downsweep<true><<<T, ST, 0, s>>>(kin, vin, kout, vout, n, shift, bits, c->b.tile_hist,
                                                     c->b.digit_total, T, c->b.posB, c->b.velB, c->b.pidB, c->b.posA,
-                                                    c->b.velA, c->b.pidA);
+                                                    c->b.velA, c->b.pidA, c->sc, c->b.flags);
         else
             radix_downsweep<false><<<T, ST, 0, s>>>(kin, vin, kout, vout, n, shift, bits, c->b.tile_hist,
                                                      c->b.digit_total, T, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                     nullptr);
+                                                     nullptr, c->sc, nullptr);
         kin = kout;
         vin = vout;
     }

@@ -20,6 +20,8 @@ SPH_MODE_BOX, SPH_MODE_PIPE = 0, 1
 SPH_FORM_NAIVE, SPH_FORM_VOXEL = 0, 1
 SPH_FLAG_KEEP_TERMS = 1
 SPH_FLAG_ADJOINT = 2
+SPH_FLAG_CHECK_FINITE = 4
+SPH_E_NONFINITE = -4
 
 STAGE_SORT, STAGE_TABLE, STAGE_REORDER, STAGE_DENSITY, STAGE_FORCE = 1, 2, 3, 4, 5
 
@@ -60,6 +62,7 @@ EXPORTED = [
     "sph_run_stage", "sph_buffer_elems", "sph_read_buffer", "sph_diagnostics", "sph_grid_info", "sph_set_launch",
     "sph_create_batch", "sph_batch_info", "sph_set_constants", "sph_set_lanes", "sph_adjoint_seed", "sph_adjoint_step",
     "sph_adjoint_read", "sph_set_batch_lanes", "sph_adjoint_snapshot_bytes", "sph_adjoint_save", "sph_step_save", "sph_adjoint_step_from",
+    "sph_check_finite",
 ]
 
 _lib = None
@@ -105,6 +108,7 @@ def load_library() -> ctypes.CDLL:
         "sph_adjoint_save": ([vp, vp, c_size_t, vp], i32),
         "sph_step_save": ([vp, vp, c_size_t, vp], i32),
         "sph_adjoint_step_from": ([vp, vp, c_size_t, vp], i32),
+        "sph_check_finite": ([vp, POINTER(i64), vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -120,7 +124,13 @@ def last_error() -> str:
     return buf.value.decode(errors="replace")
 
 
+class NonFiniteStateError(RuntimeError):
+    """A particle position became NaN or infinite (SPH_E_NONFINITE)."""
+
+
 def check(rc: int, what: str) -> None:
+    if rc == SPH_E_NONFINITE:
+        raise NonFiniteStateError(f"{what} failed ({rc}): {last_error()}")
     if rc != 0:
         raise RuntimeError(f"{what} failed ({rc}): {last_error()}")
 
@@ -152,6 +162,9 @@ class NativeContext:
             carr = (SphConstantsC * count)(*consts)
             check(self.lib.sph_create_batch(parr, carr, count, ctypes.byref(h)), "sph_create_batch")
         self.handle = h
+        # bumped by every call that changes the resident state (upload, steps, stages), so a
+        # holder of a downloaded copy can tell whether the device still holds that state
+        self.generation = 0
         cnt = c_int32()
         self.lib.sph_batch_info(h, ctypes.byref(cnt), None)
         first = (c_int32 * (cnt.value + 1))()
@@ -184,19 +197,39 @@ class NativeContext:
         assert pos.dtype == torch.float64 and vel.dtype == torch.float64
         assert pos.shape == (self.n, 3) and vel.shape == (self.n, 3)
         assert pos.is_contiguous() and vel.is_contiguous()
+        self.generation += 1
         check(self.lib.sph_upload_state(self.handle, _ptr(pos), _ptr(vel), _stream(self.device)), "sph_upload_state")
 
     def upload_f32(self, x, y, z, vx, vy, vz):
         for t in (x, y, z, vx, vy, vz):
             assert t.dtype == torch.float32 and t.shape == (self.n,) and t.is_contiguous()
+        self.generation += 1
         check(self.lib.sph_upload_state_f32(self.handle, *[_ptr(t) for t in (x, y, z, vx, vy, vz)],
                                             _stream(self.device)), "sph_upload_state_f32")
 
     def step(self, nsteps: int = 1):
+        self.generation += 1
         check(self.lib.sph_step(self.handle, int(nsteps), _stream(self.device)), "sph_step")
 
     def run_stage(self, stage: int):
+        self.generation += 1
         check(self.lib.sph_run_stage(self.handle, int(stage), _stream(self.device)), "sph_run_stage")
+
+    def check_finite(self) -> int:
+        """Waits for the stream; returns the number of positions found non-finite when hashed
+        since the last check (raises NonFiniteStateError if it is not zero)."""
+        cnt = c_int64(0)
+        rc = self.lib.sph_check_finite(self.handle, ctypes.byref(cnt), _stream(self.device))
+        check(rc, "sph_check_finite")
+        return int(cnt.value)
+
+    def nonfinite_count(self) -> int:
+        """Like check_finite, but returns the count instead of raising."""
+        cnt = c_int64(0)
+        rc = self.lib.sph_check_finite(self.handle, ctypes.byref(cnt), _stream(self.device))
+        if rc not in (0, SPH_E_NONFINITE):
+            check(rc, "sph_check_finite")
+        return int(cnt.value)
 
     def download(self, force: bool = False, out=None):
         """Returns float64 device tensors pos (N,3), vel (N,3), rho (N,) [, force (N,3)];
@@ -271,6 +304,7 @@ class NativeContext:
         check(self.lib.sph_adjoint_seed(self.handle, _ptr(gx), _ptr(gv), _stream(self.device)), "sph_adjoint_seed")
 
     def adjoint_step(self):
+        self.generation += 1
         check(self.lib.sph_adjoint_step(self.handle, _stream(self.device)), "sph_adjoint_step")
 
     def adjoint_snapshot_bytes(self) -> int:
@@ -291,6 +325,7 @@ class NativeContext:
     def step_save(self, snapshot: torch.Tensor):
         """One step whose snapshot is written straight into `snapshot`."""
         nbytes = self._snapshot(snapshot)
+        self.generation += 1
         check(self.lib.sph_step_save(self.handle, _ptr(snapshot), nbytes, _stream(self.device)), "sph_step_save")
 
     def adjoint_step_from(self, snapshot: torch.Tensor):

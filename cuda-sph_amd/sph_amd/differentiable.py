@@ -99,6 +99,19 @@ class DifferentiableSPH:
         self._fwd_graphs = {}
         self._seg_graphs = {}
         self._rev_graphs = {}
+        self._graph_visc = {}  # id(cache) -> viscosity of that cache's previous call
+
+    def set_keep_all(self, keep_all: bool) -> None:
+        """Switches between keeping every step's snapshot (allocated at construction) and the
+        checkpoint-only backward; drops the recorded graphs, which hold the other scheme."""
+        keep_all = bool(keep_all)
+        if keep_all and not hasattr(self, "_all"):
+            raise ValueError("keep_all needs the per-step snapshots allocated at construction")
+        self.keep_all = keep_all
+        self._generation += 1  # no tape recorded so far is reversed from snapshots any more
+        for cache in (self._fwd_graphs, self._seg_graphs, self._rev_graphs):
+            cache.clear()
+        self._graph_visc.clear()
 
     @property
     def num_checkpoints(self) -> int:
@@ -157,11 +170,25 @@ class DifferentiableSPH:
             self.ctx.adjoint_step()
 
     def _run(self, cache: dict, key, body) -> None:
+        """Runs body, from a HIP graph when graphs are on.  A graph bakes the viscosity into
+        its kernel arguments, so graphs are kept for one viscosity only (the key's first
+        element): a pass at a viscosity not seen on the previous call of this cache runs
+        directly, the second consecutive call at one viscosity records its graph, and graphs
+        of any other viscosity are dropped.  An optimisation loop that changes mu every
+        iteration therefore never records a graph it would replay once."""
         if not self.graphs:
             body()
             return
+        visc = key[0] if isinstance(key, tuple) else key
+        last = self._graph_visc.get(id(cache))
+        self._graph_visc[id(cache)] = visc
         g = cache.get(key)
         if g is None:
+            for k in [k for k in cache if (k[0] if isinstance(k, tuple) else k) != visc]:
+                del cache[k]
+            if last != visc:
+                body()
+                return
             g = torch.cuda.CUDAGraph()
             torch.cuda.synchronize(self.device)
             with torch.cuda.graph(g):

@@ -11,7 +11,11 @@ Differences in behaviour at the boundary:
     module globals, and the collision mode ('BOX' / 'PIPE') is a constructor argument
     instead of config.SIM_MODE;
   - arrays of the returned state are read-only.  Passing the returned state back skips
-    the host-to-device copy; any other state object is uploaded.
+    the host-to-device copy while the device still holds it (no other call has changed
+    the resident state since); any other state object is uploaded;
+  - a step that leaves a particle position NaN or infinite raises NonFiniteStateError
+    (a RuntimeError) instead of returning that state (check_finite=False restores the
+    reference's silent behaviour).
 """
 from __future__ import annotations
 
@@ -41,11 +45,13 @@ class SPHStrategy:
 
     def __init__(self, params: SimulationParameters, constants: Optional[SPHConstants] = None, mode: str = "BOX",
                  device="cuda:0", keep_terms: bool = False, seed: int = RNG_SEED, grid_voxel: Optional[float] = None,
-                 launch: Optional[Tuple[int, int]] = None, lanes: int = 1):
+                 launch: Optional[Tuple[int, int]] = None, lanes: int = 1, check_finite: bool = True):
         """grid_voxel: NAIVE formulation only, the edge of the search grid's cells in place
         of params.voxel_size (the NAIVE sums do not depend on it; see autotune.py).
         launch: (density, force) threads per workgroup.  lanes: NAIVE only, lanes of a
-        wavefront per particle in the density and force passes."""
+        wavefront per particle in the density and force passes.  check_finite: every step
+        call waits for its steps and raises NonFiniteStateError if a position turned
+        NaN or infinite."""
         if mode not in _MODES:
             raise Exception(f"Wrong simulation mode! ({mode})")
         if grid_voxel is not None and self.formulation != N.SPH_FORM_NAIVE:
@@ -60,16 +66,21 @@ class SPHStrategy:
         self.result_force = np.zeros((n, 3), dtype=np.float64)
         table = params.pipe.to_numpy() if mode == "PIPE" else None
         voxel = params.voxel_size if grid_voxel is None else [float(grid_voxel)] * 3
+        flags = (N.SPH_FLAG_KEEP_TERMS if keep_terms else 0) | (N.SPH_FLAG_CHECK_FINITE if check_finite else 0)
         self._params_c = N.make_params(n, params.external_force, self.dt, params.space_size, voxel,
-                                       _MODES[mode], self.formulation, table,
-                                       N.SPH_FLAG_KEEP_TERMS if keep_terms else 0, seed)
+                                       _MODES[mode], self.formulation, table, flags, seed)
         self.keep_terms = keep_terms
         self.ctx = N.NativeContext(self._params_c, N.make_consts(self.constants), device)
         self.device = self.ctx.device
+        self._launch = (256, 256)
+        self._lanes = 1
+        self._returned_generation = -1  # ctx.generation when new_state was downloaded
         if launch:
             self.ctx.set_launch(*launch)
+            self._launch = (int(launch[0]), int(launch[1]))
         if lanes != 1:
             self.ctx.set_lanes(lanes)
+            self._lanes = int(lanes)
         logger.info("SPH strategy on %s: %s", self.device, self.ctx.grid_info())
 
     # -- reference entry point -----------------------------------------------------------
@@ -91,11 +102,15 @@ class SPHStrategy:
     def capture(self, nsteps: int) -> torch.cuda.CUDAGraph:
         """A HIP graph of nsteps steps of the resident state (recorded, not run).  The step
         records no memsets and no host work, so each replay advances the resident state by
-        nsteps steps exactly as ctx.step(nsteps) does."""
+        nsteps steps exactly as ctx.step(nsteps) does.  Replays change the resident state
+        behind this object's back, so the last returned state is re-uploaded when it is
+        passed back; the finite check of a captured step is left to the next step call
+        outside the graph."""
         g = torch.cuda.CUDAGraph()
         torch.cuda.synchronize(self.device)
         with torch.cuda.graph(g):
             self.ctx.step(int(nsteps))
+        self.new_state = None
         return g
 
     # -- stages --------------------------------------------------------------------------
@@ -103,7 +118,7 @@ class SPHStrategy:
         """Host-to-device copy of the state (abstract_sph_strategy.py:64-74), skipped when the
         state is the one this strategy returned last."""
         st = self.old_state
-        if st is not None and st is self.new_state:
+        if st is not None and st is self.new_state and self.ctx.generation == self._returned_generation:
             return
         pos = torch.from_numpy(np.ascontiguousarray(st.position, dtype=np.float64)).to(self.device)
         vel = torch.from_numpy(np.ascontiguousarray(st.velocity, dtype=np.float64)).to(self.device)
@@ -120,6 +135,7 @@ class SPHStrategy:
         if self.keep_terms:
             self.result_force = out[3].cpu().numpy()
         self.new_state = SimulationState(host[0], host[1], host[2])
+        self._returned_generation = self.ctx.generation
 
     def diagnostics(self) -> dict:
         """FP64 totals of the resident state."""
@@ -127,8 +143,21 @@ class SPHStrategy:
         return dict(mass=d[0], momentum=d[1:4], kinetic_energy=d[4], non_finite=int(d[5]), max_speed=d[6],
                     mean_density=d[7])
 
+    # thread_layout.organize (thread_layout.py:27-33) gives the reference's launch shape;
+    # sim/src/main.py:20-21 logs it as grid_size / block_size
     @property
-    def grid_size(self):
+    def block_size(self) -> int:
+        """Threads per workgroup of the force pass (the step's fused integration launch)."""
+        return self._launch[1]
+
+    @property
+    def grid_size(self) -> int:
+        """Workgroups of the force pass: ceil(N * lanes / block_size)."""
+        return -(-int(self.ctx.n) * self._lanes // self.block_size)
+
+    @property
+    def voxel_dims(self):
+        """Voxels per axis of the grid the step sorts by."""
         return self.ctx.grid_info()["dims"]
 
 

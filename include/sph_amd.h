@@ -15,7 +15,9 @@
  *   - Pointers named dev_* are device pointers (for example torch tensor data_ptr()).
  *     Pointers without the prefix are host pointers.
  *   - stream is a hipStream_t passed as void*; NULL means the default stream.  All work
- *     is enqueued on that stream; no call synchronises except sph_create/sph_destroy.
+ *     is enqueued on that stream; the calls that wait for it are sph_bind_workspace,
+ *     sph_set_batch_lanes, sph_check_finite and sph_step on a context created with
+ *     SPH_FLAG_CHECK_FINITE.
  *   - The library never allocates device memory: the caller allocates the workspace
  *     (sph_workspace_bytes) and keeps it alive while the context is used.
  *   - A context is not re-entrant: calls on one context must not overlap.
@@ -77,6 +79,12 @@ typedef struct {
 /* Allocate the adjoint buffers (sph_adjoint_*): 64 bytes per particle more.  Plain
  * NAIVE-formulation contexts in BOX mode only. */
 #define SPH_FLAG_ADJOINT 2
+/* sph_step waits for its steps and fails with SPH_E_NONFINITE when a position became NaN
+ * or infinite (it calls sph_check_finite; skipped while the stream is being captured into
+ * a graph).  The reference has no such check: its state silently turns non-finite, e.g.
+ * the VOXEL formulation's density leaves the particle itself out (voxel_kernels.py:127-130),
+ * so an isolated particle has rho = 0 and base_kernels.py:47-52 divides by it. */
+#define SPH_FLAG_CHECK_FINITE 4
 
 typedef struct sph_ctx sph_ctx;
 
@@ -128,6 +136,13 @@ int sph_upload_state_f32(sph_ctx* ctx, const float* dev_x, const float* dev_y, c
  * density, pressure and viscosity terms, integration (base_kernels.py:30-53) and
  * collision (base_kernels.py:56-98).  Returns SPH_E_STATE before the first upload. */
 int sph_step(sph_ctx* ctx, int32_t nsteps, void* stream);
+
+/* Waits for the stream and reads the number of particle positions found NaN or infinite
+ * when hashed since the last read (every step hashes every particle once, as does an upload),
+ * then clears it.  *count (may be NULL) receives the number; returns SPH_E_NONFINITE if it is
+ * not zero.  No reference counterpart: the reference returns a non-finite state without
+ * notice (abstract_sph_strategy.py:76-84). */
+int sph_check_finite(sph_ctx* ctx, int64_t* count, void* stream);
 
 /* Replaces __finalize_computation (abstract_sph_strategy.py:76-84): writes the state in
  * particle order as float64 (N, 3) / (N,), i.e. the new SimulationState's position,

@@ -17,17 +17,22 @@
  * Pinning (what checks each function, besides the code itself):
  *   geometry helpers        known answers of sim/tests/test_collisions.py:90-182
  *   pipe table              known answers of common/tests/test_pipe.py:12-34
- *   solve_collision         post-condition of sim/tests/test_collisions.py:184-209 plus
- *                           closed-form cylinder reflection cases (tests/test_oracle_pipe.py);
- *                           exact intermediate values are "parity unpinned" (no golden
- *                           vectors exist and the reference cannot run here)
+ *   solve_collision         post-condition of sim/tests/test_collisions.py:184-209, closed-form
+ *                           cylinder reflection, and on cones (tests/test_oracle_pins.py): the
+ *                           backtrack d / (sin a + 0.001) with d from plane geometry, |v'| = |v|,
+ *                           v' + v along the wall line, re-advance by the backtracked distance
  *   xoroshiro128+/splitmix  published SplitMix64 vectors and a GF(2) matrix-power check
  *                           of the jump polynomial (tests/test_oracle_rng.py)
  *   kernels W, grad W, lap W  normalisation integral, finite differences of the spiky and
  *                           viscosity kernels (Mueller et al. 2003) (tests/test_oracle_physics.py)
  *   naive rho / P / V       independent NumPy brute force, pair identities, invariances
  *   voxel keys / sort       closed form (trunc, NumPy stable argsort)
- *   voxel formulation       brute-force cap-32 enumeration in NumPy on small inputs
+ *   voxel formulation       a hand-worked three-particle chain (rho without self, pressure with
+ *                           no mass, clipped viscosity), pair antisymmetry (sum P = sum V = 0
+ *                           with complete lists), mass scaling, an isolated particle going
+ *                           non-finite, and the 32-cap order on a hand-built layout whose list
+ *                           is written out literally (tests/test_oracle_pins.py); a second
+ *                           restatement of the enumeration in plain Python on random inputs
  *   rollout adjoint         directional central differences of this file's own forward map
  *                           (tests/test_oracle_adjoint.py), wall bounces replayed
  */

@@ -154,7 +154,7 @@ def test_density_and_force_stages(gpu):
     _, _, _, sP2, _ = naive_terms(pos, vel, c, rho=rho_ref)
     eP2 = np.linalg.norm(P - P_ref2, axis=1) / np.maximum(sP2, 1e-30)
     print(f"pressure err vs FP64 densities {eP2.max():.3e}")
-    assert eP2.max() <= 1e-4
+    assert eP2.max() <= TOL
 
 
 @pytest.mark.parametrize("lanes", [1, 4])
@@ -230,7 +230,7 @@ def test_full_step_matches_oracle(gpu):
     dfor = np.linalg.norm(strat.result_force - rf, axis=1) / np.maximum(sP + sV, 1e-30)
     print(f"step: rho {drho.max():.2e} pos {dpos.max():.2e} vel(norm) {dvel.max():.2e} force(norm) {dfor.max():.2e}")
     assert drho.max() <= TOL and dpos.max() <= TOL
-    assert dvel.max() <= 1e-4 and dfor.max() <= 1e-4
+    assert dvel.max() <= TOL and dfor.max() <= TOL
 
 
 def test_multi_step_trajectory(gpu):
@@ -246,7 +246,7 @@ def test_multi_step_trajectory(gpu):
     drho = np.abs(out.density - rrho) / rrho
     print(f"20 steps: pos {dpos.max():.2e} rho {drho.max():.2e}")
     assert dpos.max() <= TOL
-    assert drho.max() <= 1e-4
+    assert drho.max() <= TOL
 
 
 def test_box_collisions_all_walls(gpu):
@@ -340,9 +340,32 @@ def test_voxel_formulation(gpu):
     assert np.max(np.abs(rho - rho_ref)[ok] / np.abs(rho_ref[ok])) <= TOL
     P = read(strat, N.BUF_PRESSURE).reshape(-1, 3)
     V = read(strat, N.BUF_VISCOSITY).reshape(-1, 3)
-    scaleP = np.maximum(np.abs(P_ref).max(axis=1), 1e-30)
-    print("voxel P err", np.max(np.linalg.norm(P - P_ref, axis=1)[ok] / scaleP[ok]))
-    assert np.allclose(V[ok], V_ref[ok], rtol=1e-4, atol=1e-6)
+    # errors against the per-particle sum of the pair terms' magnitudes (the sums cancel)
+    sP, sV = _voxel_term_scales(pos, vel, rho_ref, nb, nn, c)
+    eP = np.linalg.norm(P - P_ref, axis=1) / np.maximum(sP, 1e-30)
+    clipped = V_ref >= 1.0
+    eV = np.linalg.norm(np.where(clipped, 0.0, V - V_ref), axis=1) / np.maximum(sV, 1e-30)
+    print(f"voxel P err {eP[ok].max():.2e}, V err {eV[ok].max():.2e} (normalised)")
+    assert eP[ok].max() <= TOL and eV[ok].max() <= TOL
+    assert np.all(V[ok][clipped[ok]] == 1.0)
+
+
+def _voxel_term_scales(pos, vel, rho, nb, nn, c):
+    """Per particle, sum_j |(p_i/rho_i^2 + p_j/rho_j^2) grad W| and
+    sum_j |v_j - v_i| / rho_j lap W m mu / rho_i over its VOXEL list (voxel_kernels.py:135-211)."""
+    n = len(pos)
+    p = c.k * (rho - c.rho0)
+    sP, sV = np.zeros(n), np.zeros(n)
+    gc, lc = 45.0 / (math.pi * c.h ** 6), 45.0 / (math.pi * c.h ** 6)
+    for i in range(n):
+        js = nb[i, :nn[i]]
+        js = js[js != i]
+        if len(js) == 0:
+            continue
+        r = np.linalg.norm(pos[i] - pos[js], axis=1)
+        sP[i] = np.sum(np.abs(p[i] / rho[i] ** 2 + p[js] / rho[js] ** 2) * gc * (c.h - r) ** 2)
+        sV[i] = np.sum(np.linalg.norm(vel[js] - vel[i], axis=1) / rho[js] * lc * (c.h - r)) * c.mass * c.visc / rho[i]
+    return sP, sV
 
 
 def test_pipe_mode_and_rng(gpu):
@@ -367,7 +390,7 @@ def test_pipe_mode_and_rng(gpu):
     assert moved
     err = np.abs(out.position - rp).max(axis=1)
     print("pipe max abs pos err", err.max(), "re-injected", int(np.sum(rr.reshape(-1, 2)[:, 0] != rng0.reshape(-1, 2)[:, 0])))
-    assert err.max() <= 1e-4
+    assert err.max() <= TOL
 
 
 def read_rng_after_upload(strat, pos, vel):
@@ -429,8 +452,11 @@ def test_lanes_per_particle_match_oracle(gpu, lanes, dense):
     out = strat.compute_next_state(SimulationState(pos, vel, np.zeros(len(pos))))
     rp, rv, rr, rf, _ = oracle_step(w, pos, vel)
     assert np.max(np.abs(out.density - rr) / rr) <= TOL
-    scale = np.max(np.abs(rf), axis=1, keepdims=True) + 1e-30
-    assert np.max(np.abs(strat.result_force - rf) / scale) <= 1e-4
+    # against the per-particle sum of the pair terms' magnitudes (the sums cancel)
+    _, _, _, sP, sV = naive_terms(pos, vel, w.constants)
+    ef = np.max(np.linalg.norm(strat.result_force - rf, axis=1) / np.maximum(sP + sV, 1e-30))
+    print(f"lanes {lanes} dense {dense}: force err {ef:.2e}")
+    assert ef <= TOL
     assert np.max(np.abs(out.position - rp) / np.maximum(np.abs(rp), 1.0)) <= TOL
 
 
