@@ -860,6 +860,97 @@ int sph_adjoint_read(sph_ctx* c, float* gx, float* gv, float* gmu, void* stream)
     if (!c->adj_seeded) return fail(SPH_E_STATE, "no cotangent (call sph_adjoint_seed)");
     return hip_check(sph::launch_adjoint_get(c, gx, gv, gmu, (hipStream_t)stream), "adjoint read");
 }
+}  // extern "C"
 
+// ---------------------------------------------------------------------------------------
+// The step without any device allocation (sph_step_noalloc): the context's workspace is a
+// file-scope device array, one per GPU, looked up once per device with
+// hipGetSymbolAddress.  Host code never passes the array's name to a kernel or a copy (on
+// the host the name stands in for the symbol, it is not a device address); only the
+// looked-up address is used.
+// ---------------------------------------------------------------------------------------
+#include <mutex>
+
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;  // 64 MiB per GPU
+constexpr int kMaxDevices = 64;
+
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of this GPU's scratch array (looked up on the first call per device,
+// with that device current).
+int scratch_address(void** out) {
+    int dev = -1;
+    int rc;
+    if ((rc = hip_check(hipGetDevice(&dev), "current device"))) return rc;
+    if (dev < 0 || dev >= kMaxDevices) return fail(SPH_E_INVALID, "device %d beyond the %d supported", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if ((rc = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "scratch address"))) return rc;
+        if (!p || ((uintptr_t)p & 255u)) return fail(SPH_E_HIP, "scratch address %p is not 256-byte aligned", p);
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SPH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t sph_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes; }
+
+int sph_step_noalloc(const sph_params_t* params, const sph_constants_t* consts, const double* dev_pos_in,
+                     const double* dev_vel_in, double* dev_pos_out, double* dev_vel_out, double* dev_rho_out,
+                     int32_t nsteps, void* stream) {
+    if (!params || !consts) return fail(SPH_E_INVALID, "null argument");
+    if (params->mode != SPH_MODE_BOX)
+        return fail(SPH_E_INVALID, "sph_step_noalloc runs BOX mode only (PIPE mode keeps RNG streams across calls)");
+    if (params->flags & (SPH_FLAG_ADJOINT | SPH_FLAG_KEEP_TERMS))
+        return fail(SPH_E_INVALID, "sph_step_noalloc takes neither SPH_FLAG_ADJOINT nor SPH_FLAG_KEEP_TERMS");
+    if (nsteps < 0) return fail(SPH_E_INVALID, "nsteps < 0");
+    if (params->particle_count > 0 && (!dev_pos_in || !dev_vel_in || !dev_pos_out || !dev_vel_out))
+        return fail(SPH_E_INVALID, "null state pointer");
+    // a plain context planned on the stack: a BOX-mode plan allocates no host memory either
+    sph_ctx c{};
+    c.prm = *params;
+    c.prm.pipe = nullptr;
+    c.prm.pipe_rows = 0;
+    c.cst = *consts;
+    int rc = plan(&c);
+    if (rc) return rc;
+    if (c.ws_bytes > kNoAllocScratchBytes)
+        return fail(SPH_E_INVALID,
+                    "%lld particles on %d cells need %zu bytes of scratch; sph_step_noalloc has %zu (use sph_step "
+                    "with a caller-owned workspace)",
+                    (long long)c.n, c.sc.cs_cells, c.ws_bytes, kNoAllocScratchBytes);
+    void* base = nullptr;
+    if ((rc = scratch_address(&base))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    assign_buffers(&c, base);
+    // the scratch keeps the previous call's contents: clear what the step needs at zero
+    if ((rc = hip_check(sph::launch_fill_u32(c.b.flags, 0u, 16, s), "clear flags"))) return rc;
+    if (c.cell_sort && (rc = hip_check(sph::launch_fill_u32(c.b.cell_cnt, 0u, c.sc.cs_cells, s), "clear cell counts")))
+        return rc;
+    c.bound = true;
+    if ((rc = hip_check(sph::launch_upload_f64(&c, dev_pos_in, dev_vel_in, s), "upload"))) return rc;
+    c.have_state = true;
+    c.last_stage = 0;
+    for (int32_t k = 0; k < nsteps; ++k)
+        if ((rc = run_step_stages(&c, s, true))) return rc;
+    if ((rc = hip_check(sph::launch_download_f64(&c, dev_pos_out, dev_vel_out, dev_rho_out, nullptr, s), "download")))
+        return rc;
+    if (c.check_finite && nsteps > 0) {
+        hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+        if ((rc = hip_check(hipStreamIsCapturing(s, &st), "capture status"))) return rc;
+        if (st == hipStreamCaptureStatusNone) return read_nonfinite(&c, s, nullptr);
+    }
+    return SPH_OK;
+}
 
 }  // extern "C"

@@ -62,7 +62,7 @@ EXPORTED = [
     "sph_run_stage", "sph_buffer_elems", "sph_read_buffer", "sph_diagnostics", "sph_grid_info", "sph_set_launch",
     "sph_create_batch", "sph_batch_info", "sph_set_constants", "sph_set_lanes", "sph_adjoint_seed", "sph_adjoint_step",
     "sph_adjoint_read", "sph_set_batch_lanes", "sph_adjoint_snapshot_bytes", "sph_adjoint_save", "sph_step_save", "sph_adjoint_step_from",
-    "sph_check_finite",
+    "sph_check_finite", "sph_noalloc_scratch_bytes", "sph_step_noalloc",
 ]
 
 _lib = None
@@ -109,6 +109,8 @@ def load_library() -> ctypes.CDLL:
         "sph_step_save": ([vp, vp, c_size_t, vp], i32),
         "sph_adjoint_step_from": ([vp, vp, c_size_t, vp], i32),
         "sph_check_finite": ([vp, POINTER(i64), vp], i32),
+        "sph_noalloc_scratch_bytes": ([], c_size_t),
+        "sph_step_noalloc": ([POINTER(SphParamsC), POINTER(SphConstantsC), vp, vp, vp, vp, vp, i32, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -282,6 +282,27 @@ int sph_adjoint_save(sph_ctx* ctx, void* dev_snapshot, size_t bytes, void* strea
 int sph_step_save(sph_ctx* ctx, void* dev_snapshot, size_t bytes, void* stream);
 int sph_adjoint_step_from(sph_ctx* ctx, const void* dev_snapshot, size_t bytes, void* stream);
 
+/* ---------------------------------------------------------------------------------------
+ * The step with no device allocation.  No counterpart in the reference: this is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  sph_step_noalloc plans a plain context on the stack,
+ * takes its workspace from a static device array of sph_noalloc_scratch_bytes() bytes
+ * (64 MiB) that the library holds on every GPU it runs on until the process ends, and runs
+ * sph_upload_state, sph_step(nsteps) and sph_download_state on it: dev_pos_in / dev_vel_in
+ * and dev_pos_out / dev_vel_out are float64 (N, 3) in particle order, dev_rho_out float64
+ * (N,) or NULL.  Results are bitwise those of that sequence on a context of its own.  BOX
+ * mode only, without SPH_FLAG_ADJOINT or SPH_FLAG_KEEP_TERMS (SPH_FLAG_CHECK_FINITE is
+ * honoured); a problem whose workspace (sph_workspace_bytes of the same parameters) exceeds
+ * the array fails with SPH_E_INVALID.  Nothing persists between calls: every call on a GPU
+ * shares that GPU's one array, so calls that overlap on one GPU (other streams, other
+ * threads) race and must be serialised by the caller.  It calls no hipMalloc, hipFree or
+ * other allocating routine.
+ * --------------------------------------------------------------------------------------- */
+size_t sph_noalloc_scratch_bytes(void);
+int sph_step_noalloc(const sph_params_t* params, const sph_constants_t* consts, const double* dev_pos_in,
+                     const double* dev_vel_in, double* dev_pos_out, double* dev_vel_out, double* dev_rho_out,
+                     int32_t nsteps, void* stream);
+
 /* Message of the last error on this thread (always NUL-terminated). */
 int sph_last_error(char* buf, size_t len);
 
