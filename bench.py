@@ -386,9 +386,34 @@ def bench_voxel_form(steps: int = 500, warmup: int = 5):
     ms = a.elapsed_time(b)
     d = strat.diagnostics()
     log(f"voxel formulation: {steps} steps of {n} particles in {ms:.1f} ms")
-    return {"workload": "dam_break_1m", "formulation": "VOXEL", "particles": n, "steps": steps,
-            "voxel": [float(x) for x in w.params.voxel_size], "value": round(n * steps / (ms / 1e3), 1),
-            "unit": "particle-updates/s", "ms_per_step": round(ms / steps, 5), "non_finite": d["non_finite"]}
+    out = {"workload": "dam_break_1m", "formulation": "VOXEL", "particles": n, "steps": steps,
+           "voxel": [float(x) for x in w.params.voxel_size], "value": round(n * steps / (ms / 1e3), 1),
+           "unit": "particle-updates/s", "ms_per_step": round(ms / steps, 5), "non_finite": d["non_finite"],
+           "state_went_non_finite": d["non_finite"] > 0}
+    if d["non_finite"]:
+        # the reference's VOXEL density leaves the particle itself out (voxel_kernels.py:127-130):
+        # a particle that ends with no neighbour has rho = 0 and the integration divides by it
+        out["note"] = (f"{d['non_finite']} particles ended NaN or infinite (an isolated particle has rho = 0 in this "
+                       "formulation); the rate is that of a run whose state went non-finite")
+    return out
+
+
+def launches_per_step(ctx) -> int:
+    """Kernel launches of one step of a context: the nodes of a HIP graph recorded from
+    ctx.step(1) (recorded, never run)."""
+    import ctypes
+
+    import torch
+
+    g = torch.cuda.CUDAGraph(keep_graph=True)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        ctx.step(1)
+    hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime torch has loaded (same soname)
+    count = ctypes.c_size_t(0)
+    rc = hip.hipGraphGetNodes(ctypes.c_void_p(g.raw_cuda_graph()), None, ctypes.byref(count))
+    del g
+    return int(count.value) if rc == 0 else -1
 
 
 def timed_steps(strat, steps: int, graph: bool, pos, vel):
@@ -439,6 +464,8 @@ def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
         log(f"batch tuned: {tuning}")
     batch = BatchedSPH(inst, cell_scale=scale, launch=launch, lanes=lanes)
     batch.upload(states)
+    launches = launches_per_step(batch.ctx)
+    log(f"batched step: {launches} kernel launches for {count} instances")
     graph = batch.capture(steps)  # recorded, not run
     batch.upload(states)
     batch.step(warmup)
@@ -456,6 +483,7 @@ def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
     # as one HIP graph; the fastest counted
     t_seq = 0.0
     per_instance = []
+    plain_launches = None
     for b, w in enumerate(bw.instances):
         configs = [(None, None, 1)]
         if tune:
@@ -470,6 +498,9 @@ def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
             gv = None if sc is None else workloads.voxel_for(w.constants.h) * sc
             strat = NaiveSPHStrategy(w.params, constants=w.constants, grid_voxel=gv, launch=la, lanes=ln,
                                              check_finite=False)
+            if plain_launches is None:
+                strat.ctx.upload(pos, vel)
+                plain_launches = launches_per_step(strat.ctx)
             for graph in (False, True):
                 run = timed_steps(strat, steps, graph, pos, vel)
                 strat.ctx.upload(pos, vel)
@@ -494,6 +525,7 @@ def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
             "ms_per_step": round(t_batch * 1e3 / steps, 4), "graph": True,
             "sequential_plain_value": round(total * steps / t_seq, 1),
             "speedup_vs_sequential": round(t_seq / t_batch, 3), "tuning": tuning,
+            "launches_per_step": launches, "sequential_launches_per_step": count * plain_launches,
             "slowest_instance_alone_ms_per_step": round(slowest[0][0], 4)}
 
 
@@ -723,5 +755,32 @@ def main() -> int:
     return 0
 
 
+def reap_children() -> None:
+    """Ends and waits for any child process this run left (none is started on purpose), so
+    that nothing outlives the bench."""
+    try:
+        import psutil
+    except ImportError:
+        return
+    kids = psutil.Process().children(recursive=True)
+    for k in kids:
+        try:
+            k.terminate()
+        except psutil.Error:
+            pass
+    gone, alive = psutil.wait_procs(kids, timeout=5)
+    for k in alive:
+        try:
+            k.kill()
+        except psutil.Error:
+            pass
+    if kids:
+        log(f"reaped {len(kids)} child process(es)")
+
+
 if __name__ == "__main__":
-    sys.exit(main())
+    try:
+        rc = main()
+    finally:
+        reap_children()
+    sys.exit(rc)

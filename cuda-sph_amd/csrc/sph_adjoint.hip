@@ -26,7 +26,8 @@
 //   adjoint_seed_kernel     the force pass itself (same code, so the step's output is
 //                           written bitwise as sph_step writes it) plus the collision and
 //                           integration pullback; each particle's dL/dmu term is added to
-//                           its FP64 running sum (indexed by particle id)
+//                           its running sum, an FP32 pair (sum, rounding error) kept exact
+//                           to about 2^-48 by compensated addition (indexed by particle id)
 //   adjoint_pair_kernel     pressure / viscosity pullback: gv final, rt, gx partial
 //   adjoint_density_kernel  density pullback: gx final
 // and when the gradient is read, adjoint_mu_total_kernel sums the running sums in a fixed
@@ -38,6 +39,24 @@ namespace sph {
 namespace {
 
 constexpr int kAdjThreads = 256;
+
+// Compensated FP32 sums (Knuth's two-sum): a value is the unevaluated sum hi + lo of an FP32
+// pair, so a long running sum keeps about twice FP32's precision without FP64 arithmetic.
+// Nothing here may be reassociated (the build uses no fast-math flags).
+__device__ __forceinline__ float2 two_sum(float a, float b) {
+    const float s = a + b;
+    const float bb = s - a;
+    const float err = (a - (s - bb)) + (b - bb);
+    return make_float2(s, err);
+}
+__device__ __forceinline__ float2 two_float_add(float2 acc, float x) {
+    const float2 t = two_sum(acc.x, x);
+    return two_sum(t.x, acc.y + t.y);  // renormalised
+}
+__device__ __forceinline__ float2 two_float_sum(float2 a, float2 b) {
+    const float2 t = two_sum(a.x, b.x);
+    return two_sum(t.x, a.y + b.y + t.y);  // renormalised
+}
 
 // Neighbours of cell-ordered particle i (the particle itself included), from its list or,
 // for a particle with more neighbours than list slots, by rescanning its candidates.
@@ -100,7 +119,7 @@ __device__ __forceinline__ void seed_particle(
     const float* __restrict__ rhoA, float4* __restrict__ posOut, float4* __restrict__ velOut,
     uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
     uint32_t* __restrict__ flags, const float4* __restrict__ cotx, const float4* __restrict__ cotv,
-    float4* __restrict__ fb, float4* __restrict__ gxs, float4* __restrict__ gvs, double* __restrict__ mu_part) {
+    float4* __restrict__ fb, float4* __restrict__ gxs, float4* __restrict__ gvs, float2* __restrict__ mu_part) {
     const float ps = 0.5f * C.mass * C.grad_c;
     const float vs = C.mass * C.visc * C.lap_c;
     const Vec3 P{ps * S.ax, ps * S.ay, ps * S.az}, V{vs * S.bx, vs * S.by, vs * S.bz};
@@ -124,7 +143,7 @@ __device__ __forceinline__ void seed_particle(
     // dV/dmu = m Lc sum_j (h - r) iota_j (v_j - v_i)
     const float mu_term = C.mass * C.lap_c * (fbx * S.bx + fby * S.by + fbz * S.bz);
     fb[i] = make_float4(fbx, fby, fbz, rb);
-    mu_part[pid] += (double)mu_term;  // one writer per particle id and step
+    mu_part[pid] = two_float_add(mu_part[pid], mu_term);  // one writer per particle id and step
     gxs[i] = make_float4(gxp[0], gxp[1], gxp[2], 0.0f);
     gvs[i] = make_float4(gvp[0], gvp[1], gvp[2], 0.0f);
 }
@@ -136,7 +155,7 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_seed_kernel(
     const uint32_t* __restrict__ nn, float4* __restrict__ posOut, float4* __restrict__ velOut,
     uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
     uint32_t* __restrict__ flags, const float4* __restrict__ cotx, const float4* __restrict__ cotv,
-    float4* __restrict__ fb, float4* __restrict__ gxs, float4* __restrict__ gvs, double* __restrict__ mu_part,
+    float4* __restrict__ fb, float4* __restrict__ gxs, float4* __restrict__ gvs, float2* __restrict__ mu_part,
     float4* __restrict__ sums) {
     // L lanes per particle, as the step's force pass uses, so the recomputed sums are its own
     const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -160,7 +179,7 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_seed_sums_kernel(
     float4* __restrict__ velOut, uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI,
     uint32_t* __restrict__ valI, uint32_t* __restrict__ flags, const float4* __restrict__ cotx,
     const float4* __restrict__ cotv, float4* __restrict__ fb, float4* __restrict__ gxs, float4* __restrict__ gvs,
-    double* __restrict__ mu_part) {
+    float2* __restrict__ mu_part) {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= C.n) return;
     const float4 a = sums[i], b = sums[C.n + i];
@@ -276,32 +295,32 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_density_kernel(
     cotx[pidA[k]] = make_float4(gx.x + s * ax, gx.y + s * ay, gx.z + s * az, 0.0f);
 }
 
-// dL/dmu: the per-particle sums over the reversed steps (mu_part, FP64), summed by one
-// workgroup in a fixed order (each lane a strided range in order, then a fixed tree) into
-// acc[0] when the gradient is read.
-__global__ __launch_bounds__(kAdjThreads) void adjoint_mu_total_kernel(int n, const double* __restrict__ mu_part,
+// dL/dmu: the per-particle sums over the reversed steps (mu_part, compensated FP32 pairs),
+// summed by one workgroup in a fixed order (each lane a strided range in order, then a fixed
+// tree), every addition compensated, into acc[0] when the gradient is read.
+__global__ __launch_bounds__(kAdjThreads) void adjoint_mu_total_kernel(int n, const float2* __restrict__ mu_part,
                                                                         float* __restrict__ acc) {
-    __shared__ double part[kAdjThreads];
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += kAdjThreads) s += mu_part[i];
+    __shared__ float2 part[kAdjThreads];
+    float2 s = make_float2(0.0f, 0.0f);
+    for (int i = threadIdx.x; i < n; i += kAdjThreads) s = two_float_sum(s, mu_part[i]);
     part[threadIdx.x] = s;
     __syncthreads();
     for (int w = kAdjThreads / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        if ((int)threadIdx.x < w) part[threadIdx.x] = two_float_sum(part[threadIdx.x], part[threadIdx.x + w]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) acc[0] = (float)part[0];
+    if (threadIdx.x == 0) acc[0] = part[0].x + part[0].y;
 }
 
 __global__ __launch_bounds__(kAdjThreads) void adjoint_set_kernel(int n, const float* __restrict__ gx,
                                                                    const float* __restrict__ gv,
                                                                    float4* __restrict__ cotx,
                                                                    float4* __restrict__ cotv, float* __restrict__ acc,
-                                                                   double* __restrict__ mu_part) {
+                                                                   float2* __restrict__ mu_part) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) acc[0] = 0.0f;
     if (i >= n) return;
-    mu_part[i] = 0.0;
+    mu_part[i] = make_float2(0.0f, 0.0f);
     const int64_t o = 3 * (int64_t)i;
     cotx[i] = gx ? make_float4(gx[o], gx[o + 1], gx[o + 2], 0.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
     cotv[i] = gv ? make_float4(gv[o], gv[o + 1], gv[o + 2], 0.0f) : make_float4(0.f, 0.f, 0.f, 0.f);

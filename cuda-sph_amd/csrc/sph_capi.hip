@@ -531,7 +531,7 @@ void assign_buffers(sph_ctx* c, void* dev) {
     b.adj_gv = (float4*)(base + L.off[S_AGV]);
     b.adj_rt = (float*)(base + L.off[S_ART]);
     b.mu_acc = (float*)(base + L.off[S_MU]);
-    b.mu_part = (double*)(base + L.off[S_MUP]);
+    b.mu_part = (float2*)(base + L.off[S_MUP]);
     b.sums = c->adjoint ? (float4*)(base + L.off[S_SUMS]) : nullptr;
     b.cell_cnt = (uint32_t*)(base + L.off[S_CCNT]);
     b.cell_part = (uint32_t*)(base + L.off[S_CPART]);

@@ -137,7 +137,7 @@ struct Buffers {
     float4 *adj_fb, *adj_gx, *adj_gv;
     float* adj_rt;
     float* mu_acc;
-    double* mu_part;       // dL/dmu per particle id, summed over the reversed steps
+    float2* mu_part;       // dL/dmu per particle id, summed over the reversed steps (compensated FP32 pair)
     float4* sums;          // adjoint contexts: the force pass's pressure and viscosity sums
                            // per cell-ordered particle, [2][N] (null otherwise)
 };

@@ -13,6 +13,8 @@
 // reproduces voxel_kernels.py including its 32-neighbour cap and enumeration order.
 #include <math.h>
 
+#include <type_traits>
+
 #include "sph_device.h"
 
 namespace sph {
@@ -122,8 +124,8 @@ __device__ __forceinline__ void store_density(const StepConsts& C, int i, float 
 
 // Density (naive_kernels.py:7-21) and the particle's neighbour list: every k with
 // r^2 <= h^2 in candidate order, the particle itself included (the force pass gives the
-// self pair a zero contribution).  (1 - r^2/h^2)^3 is accumulated without a branch and is
-// zero outside h.
+// self pair a zero contribution).  (1 - r^2/h^2)^3 is accumulated for exactly the listed
+// pairs (naive_kernels.py:18 skips a pair with norm_squared > h^2).
 constexpr int kDensityThreads = 256;
 #ifndef SPH_DENSITY_UNROLL
 #define SPH_DENSITY_UNROLL 2
@@ -171,14 +173,20 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
     // first and one past the last candidate: every list entry fits an int16 offset when the
     // candidates do (checked once per particle, not per hit)
     uint32_t klo = 0xffffffffu, khi = 0u;
-    auto visit = [&](float4 q, uint32_t k) {
+    // one candidate: a hit (r^2 <= h^2) adds (1 - r^2/h^2)^3 to the sum and is appended to the
+    // list; CHECKED: the list may be full (staged slots, then HBM slots, then only counted)
+    auto visit = [&](float4 q, uint32_t k, auto checked) {
         const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
-        const float t = fmaxf(1.0f - r2 * C.inv_h2, 0.0f);
-        acc += t * t * t;
         if (r2 <= C.h2) {
-            if (cnt < (uint32_t)kStageCap) s_list[cnt][col] = stage_value(k, i);
-            else if (cnt < (uint32_t)kListCap) nl[list_at(G.n, i, cnt)] = list_encode(k, i);
+            const float t = 1.0f - r2 * C.inv_h2;
+            acc += t * t * t;
+            if constexpr (decltype(checked)::value) {
+                if (cnt < (uint32_t)kStageCap) s_list[cnt][col] = stage_value(k, i);
+                else if (cnt < (uint32_t)kListCap) nl[list_at(G.n, i, cnt)] = list_encode(k, i);
+            } else {
+                s_list[cnt][col] = stage_value(k, i);
+            }
             ++cnt;
         }
     };
@@ -187,16 +195,22 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
         klo = min(klo, k0);
         khi = max(khi, k1);
         if constexpr (L == 1) {
-            uint32_t off = k0 * 16u;
-            const uint32_t end = k1 * 16u;
-            for (; off + 16u * (kDensityUnroll - 1) < end; off += 16u * kDensityUnroll) {
-                float4 q[kDensityUnroll];
+            auto walk = [&](auto checked) {
+                uint32_t off = k0 * 16u;
+                const uint32_t end = k1 * 16u;
+                for (; off + 16u * (kDensityUnroll - 1) < end; off += 16u * kDensityUnroll) {
+                    float4 q[kDensityUnroll];
 #pragma unroll
-                for (int u = 0; u < kDensityUnroll; ++u) q[u] = ld_f4(rpos, off + 16u * u);
+                    for (int u = 0; u < kDensityUnroll; ++u) q[u] = ld_f4(rpos, off + 16u * u);
 #pragma unroll
-                for (int u = 0; u < kDensityUnroll; ++u) visit(q[u], (off >> 4) + (uint32_t)u);
-            }
-            for (; off < end; off += 16u) visit(ld_f4(rpos, off), off >> 4);
+                    for (int u = 0; u < kDensityUnroll; ++u) visit(q[u], (off >> 4) + (uint32_t)u, checked);
+                }
+                for (; off < end; off += 16u) visit(ld_f4(rpos, off), off >> 4, checked);
+            };
+            // while every active lane's list has room in LDS for all of the row's candidates, the
+            // row's hits need no capacity checks (a wave-uniform choice)
+            if (__ballot(cnt + (k1 - k0) > (uint32_t)kStageCap) == 0) walk(std::false_type{});
+            else walk(std::true_type{});
         } else {
             for (uint32_t base = k0; base < k1; base += L) {
                 const uint32_t k = base + l;
@@ -204,9 +218,11 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
                 const float4 q = ld_f4(rpos, (valid ? k : k0) * 16u);
                 const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
                 const float r2 = valid ? dx * dx + dy * dy + dz * dz : INFINITY;
-                const float t = fmaxf(1.0f - r2 * C.inv_h2, 0.0f);
-                acc += t * t * t;
                 const bool hit = r2 <= C.h2;
+                if (hit) {
+                    const float t = 1.0f - r2 * C.inv_h2;
+                    acc += t * t * t;
+                }
                 constexpr uint64_t group = L == 64 ? ~0ull : (1ull << L) - 1ull;
                 const uint64_t gm = (__ballot(hit) >> gbase) & group;
                 if (hit) {
