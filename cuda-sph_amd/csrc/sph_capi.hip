@@ -334,11 +334,12 @@ int plan_batch(sph_ctx* c, const sph_params_t* params, const sph_constants_t* co
     return SPH_OK;
 }
 
-// Sort (with the gather into cell order), cell table and density of a step.
-int run_density_stages(sph_ctx* c, hipStream_t s, bool fused) {
+// Sort (with the gather into cell order), cell table and density of a step.  counted: the
+// previous step's force pass, in the same call, counted the keys (see run_steps).
+int run_density_stages(sph_ctx* c, hipStream_t s, bool fused, bool counted = false) {
     int rc;
     if (c->cell_sort) {  // the counting sort builds the cell table itself
-        if ((rc = hip_check(sph::launch_cell_sort(c, fused, s), "cell sort"))) return rc;
+        if ((rc = hip_check(sph::launch_cell_sort(c, fused, s, counted), "cell sort"))) return rc;
         if (!fused && (rc = hip_check(sph::launch_reorder(c, s), "reorder"))) return rc;
         return hip_check(sph::launch_density(c, s), "density");
     }
@@ -352,11 +353,30 @@ int run_density_stages(sph_ctx* c, hipStream_t s, bool fused) {
     return hip_check(sph::launch_density(c, s), "density");
 }
 
-int run_step_stages(sph_ctx* c, hipStream_t s, bool fused) {
+int run_step_stages(sph_ctx* c, hipStream_t s, bool fused, bool counted = false, bool count = false) {
     int rc;
-    if ((rc = run_density_stages(c, s, fused))) return rc;
-    if ((rc = hip_check(sph::launch_force(c, s), "force"))) return rc;
+    if ((rc = run_density_stages(c, s, fused, counted))) return rc;
+    if ((rc = hip_check(sph::launch_force(c, s, count), "force"))) return rc;
     c->last_stage = SPH_STAGE_FORCE;
+    return SPH_OK;
+}
+
+// nsteps fused steps.  Where the force pass can count the keys it writes (count_in_force),
+// every step but the last counts them for the next step's sort, which then starts at the
+// scan; the first step of a call counts in its sort, so between calls the per-cell counts
+// stay zero and every other path (upload, staged steps, snapshots, the pullbacks) keeps
+// its own sort.
+int run_steps(sph_ctx* c, hipStream_t s, int32_t nsteps) {
+    const bool fuse = sph::count_in_force(c);
+    for (int32_t k = 0; k < nsteps; ++k) {
+        if (int rc = run_step_stages(c, s, true, fuse && k > 0, fuse && k + 1 < nsteps)) {
+            // counts left by a force pass whose sort did not run would be counted twice by
+            // the next sort (and scatter past the end): clear them
+            if (fuse && k > 0) (void)sph::launch_fill_u32(c->b.cell_cnt, 0u, c->sc.cs_cells, s);
+            return rc;
+        }
+        c->a_in_snapshot = false;
+    }
     return SPH_OK;
 }
 
@@ -573,10 +593,7 @@ int sph_step(sph_ctx* c, int32_t nsteps, void* stream) {
     if (nsteps < 0) return fail(SPH_E_INVALID, "nsteps < 0");
     if (c->last_stage != 0 && c->last_stage != SPH_STAGE_FORCE)
         return fail(SPH_E_STATE, "a staged step is in progress (run it to SPH_STAGE_FORCE first)");
-    for (int32_t k = 0; k < nsteps; ++k) {
-        if ((rc = run_step_stages(c, (hipStream_t)stream, true))) return rc;
-        c->a_in_snapshot = false;
-    }
+    if ((rc = run_steps(c, (hipStream_t)stream, nsteps))) return rc;
     if (c->check_finite && nsteps > 0) {
         // a stream being captured into a graph cannot be waited on: the check is left to the
         // next call outside the capture (the count keeps accumulating)
@@ -941,8 +958,7 @@ int sph_step_noalloc(const sph_params_t* params, const sph_constants_t* consts, 
     if ((rc = hip_check(sph::launch_upload_f64(&c, dev_pos_in, dev_vel_in, s), "upload"))) return rc;
     c.have_state = true;
     c.last_stage = 0;
-    for (int32_t k = 0; k < nsteps; ++k)
-        if ((rc = run_step_stages(&c, s, true))) return rc;
+    if ((rc = run_steps(&c, s, nsteps))) return rc;
     if ((rc = hip_check(sph::launch_download_f64(&c, dev_pos_out, dev_vel_out, dev_rho_out, nullptr, s), "download")))
         return rc;
     if (c.check_finite && nsteps > 0) {

@@ -7,7 +7,9 @@
 // order, see hash_particle), so neighbouring threads mostly hit the same or neighbouring
 // cells.  Per step:
 //   count    one thread per slot: rank = atomicAdd(&cnt[key], 1) (an arbitrary order among
-//            the particles of a cell)
+//            the particles of a cell).  Between the steps of one multi-step call the previous
+//            step's force pass has done this already as it wrote the keys (rank_in_cell), and
+//            the sort starts at the scan
 //   scan     exclusive prefix sum of cnt over all cells: tile sums (16-byte loads), then each
 //            tile's offset is the sum of the tile sums before it (added up by the tile's own
 //            workgroup when there are few tiles, else by a one-workgroup scan first), then
@@ -34,42 +36,13 @@ constexpr int CTILE = CT * CI;   // cells per scan tile
 constexpr int CWAVES = CT / 64;
 constexpr int kMergedTiles = 2048;  // up to this many tiles, each tile's workgroup sums the counts before it
 
-// Consecutive particle ids are often in the same cell, so a wavefront's lanes come in runs of
-// equal keys: the first lane of each run adds the run's length with one atomic and the run's
-// lanes take consecutive ranks from the returned base.
-struct Run {
-    bool head;  // first lane of its run
-    int first;  // first lane of this lane's run
-    int next;   // first lane of the next run, or one past the last valid lane
-};
-
-// Runs of equal values among the valid lanes (a prefix of the wavefront).
-__device__ __forceinline__ Run lane_run(uint32_t x, bool valid) {
-    const int lane = threadIdx.x & 63;
-    const uint32_t prev = __shfl_up(x, 1);
-    Run r;
-    r.head = valid && (lane == 0 || prev != x);
-    const uint64_t heads = __ballot(r.head);
-    const uint64_t valids = __ballot(valid);
-    const uint64_t upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
-    const uint64_t below = heads & upto;
-    r.first = 63 - __clzll((long long)below);
-    const uint64_t above = heads & ~upto;
-    r.next = above ? __ffsll((unsigned long long)above) - 1 : 64 - __clzll((long long)valids);
-    return r;
-}
-
+// One thread per slot; the ranks of rank_in_cell.
 __global__ __launch_bounds__(CT) void cell_count_kernel(int n, const uint32_t* __restrict__ keyI,
                                                          uint32_t* __restrict__ cnt, uint32_t* __restrict__ rank) {
     const int i = xcd_tile(blockIdx.x, gridDim.x) * CT + threadIdx.x;
-    const int lane = threadIdx.x & 63;
     const bool valid = i < n;
-    const uint32_t key = valid ? keyI[i] : 0xffffffffu;
-    const Run r = lane_run(key, valid);
-    uint32_t base = 0;
-    if (r.head) base = atomicAdd(&cnt[key], (uint32_t)(r.next - lane));
-    base = __shfl(base, r.first);
-    if (valid) rank[i] = base + (uint32_t)(lane - r.first);
+    const uint32_t r = rank_in_cell(valid ? keyI[i] : 0xffffffffu, valid, cnt);
+    if (valid) rank[i] = r;
 }
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
@@ -302,17 +275,21 @@ inline int cblocks(int64_t n) { return (int)((n + CT - 1) / CT); }
 
 }  // namespace
 
+bool cell_sort_global(const sph_ctx* c) { return !(c->sc.cs_cells <= kLdsSortCells && c->n <= kLdsSortMax); }
+
 int64_t cell_scan_tiles(int64_t cells) { return (cells + CTILE - 1) / CTILE; }
 
-// Sorted keys and slots go to keyS[1] / valS[1]; the cell table to cs.
-hipError_t launch_cell_sort(sph_ctx* c, bool gather, hipStream_t s) {
+// Sorted keys and slots go to keyS[1] / valS[1]; the cell table to cs.  counted: the
+// per-cell counts and the ranks (in keyS[0]) of the current keys are there already
+// (launch_force with count; only on the global path, see count_in_force).
+hipError_t launch_cell_sort(sph_ctx* c, bool gather, hipStream_t s, bool counted) {
     const int n = (int)c->n;
     const int m = c->sc.cs_cells;
     const Buffers& b = c->b;
     uint32_t* rank = b.keyS[0];
     uint32_t* tmp = b.valS[0];
     const int tiles = (int)cell_scan_tiles(m);
-    if (m <= kLdsSortCells && n <= kLdsSortMax) {
+    if (!cell_sort_global(c)) {
         cell_sort_lds_kernel<<<1, LT, 0, s>>>(n, m, b.keyI, rank, b.cs, tmp);
         if (n > 0) {
             if (gather)
@@ -326,7 +303,7 @@ hipError_t launch_cell_sort(sph_ctx* c, bool gather, hipStream_t s) {
         c->sorted_vals = b.valS[1];
         return hipGetLastError();
     }
-    if (n > 0) cell_count_kernel<<<cblocks(n), CT, 0, s>>>(n, b.keyI, b.cell_cnt, rank);
+    if (n > 0 && !counted) cell_count_kernel<<<cblocks(n), CT, 0, s>>>(n, b.keyI, b.cell_cnt, rank);
     cell_scan_reduce<<<tiles, CT, 0, s>>>(b.cell_cnt, m, b.cell_part);
     if (tiles <= kMergedTiles) {
         cell_scan_down<true><<<tiles, CT, 0, s>>>(b.cell_cnt, m, b.cell_part, b.cs, (uint32_t)n);

@@ -101,17 +101,43 @@ __device__ __forceinline__ bool finite3(float x, float y, float z) {
 // cell order (coalesced stores; the sort's atomics and scatters touch neighbouring cells).
 // Otherwise (the radix sort, which must see particle-id order to be stable by id): key and
 // slot indexed by particle id.
+// Returns the key.
 template <bool SLOT_KEYS>
-__device__ __forceinline__ void hash_particle(const StepConsts& C, float x, float y, float z, uint32_t pid,
-                                              uint32_t slot, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
-                                              uint32_t* __restrict__ flags) {
+__device__ __forceinline__ uint32_t hash_particle(const StepConsts& C, float x, float y, float z, uint32_t pid,
+                                                  uint32_t slot, uint32_t* __restrict__ keyI,
+                                                  uint32_t* __restrict__ valI, uint32_t* __restrict__ flags) {
+    const uint32_t key = sort_key(x, y, z, C);
     if constexpr (SLOT_KEYS) {
-        keyI[slot] = sort_key(x, y, z, C);
+        keyI[slot] = key;
     } else {
-        keyI[pid] = sort_key(x, y, z, C);
+        keyI[pid] = key;
         valI[pid] = slot;
     }
     if (!finite3(x, y, z)) atomicAdd(flags + kFlagNonFinite, 1u);
+    return key;
+}
+
+// Rank of a key among the equal keys counted into cnt (the counting sort's per-cell
+// counts): rank = atomicAdd(&cnt[key], 1), in an arbitrary order among the particles of a
+// cell.  Consecutive slots are often in the same cell, so a wavefront's lanes come in runs
+// of equal keys: the first lane of each run adds the run's length with one atomic and the
+// run's lanes take consecutive ranks from the returned base.  The lanes that hold a key
+// (valid) must be a prefix of the wavefront, and every active lane must call this together.
+__device__ __forceinline__ uint32_t rank_in_cell(uint32_t key, bool valid, uint32_t* __restrict__ cnt) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t prev = __shfl_up(key, 1);
+    const bool head = valid && (lane == 0 || prev != key);
+    const uint64_t heads = __ballot(head);
+    const uint64_t valids = __ballot(valid);
+    const uint64_t upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    const int first = 63 - __clzll((long long)(heads & upto));  // first lane of this lane's run
+    const uint64_t above = heads & ~upto;
+    // first lane of the next run, or one past the last valid lane
+    const int next = above ? __ffsll((unsigned long long)above) - 1 : 64 - __clzll((long long)valids);
+    uint32_t base = 0;
+    if (head) base = atomicAdd(&cnt[key], (uint32_t)(next - lane));
+    base = __shfl(base, first);
+    return base + (uint32_t)(lane - first);
 }
 
 // VOXEL formulation: whether a finite position lies outside the voxel grid, i.e. its key
@@ -388,9 +414,10 @@ __device__ __forceinline__ bool box_bounced(const StepConsts& C, float x, int d)
 }
 
 // Symplectic Euler (base_kernels.py:30-53), collision, output in the same sorted slot of
-// posB/velB, hash of the new position for the next step (SLOT_KEYS: see hash_particle).
+// posB/velB, hash of the new position for the next step (SLOT_KEYS: see hash_particle);
+// returns the key.
 template <int MODE, bool SLOT_KEYS>
-__device__ __forceinline__ void finish_particle(const StepConsts& C, int i, float4 p, float4 v, uint32_t pid,
+__device__ __forceinline__ uint32_t finish_particle(const StepConsts& C, int i, float4 p, float4 v, uint32_t pid,
                                                 float rho, Vec3 P, Vec3 V, float4* __restrict__ posOut,
                                                 float4* __restrict__ velOut, uint32_t* __restrict__ pidOut,
                                                 uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
@@ -415,7 +442,7 @@ __device__ __forceinline__ void finish_particle(const StepConsts& C, int i, floa
         dbgV[i] = make_float4(V.x, V.y, V.z, 0.0f);
         dbgF[i] = make_float4(fx, fy, fz, 0.0f);
     }
-    hash_particle<SLOT_KEYS>(C, pp.x, pp.y, pp.z, pid, (uint32_t)i, keyI, valI, flags);
+    return hash_particle<SLOT_KEYS>(C, pp.x, pp.y, pp.z, pid, (uint32_t)i, keyI, valI, flags);
 }
 
 // Neighbour-list entries (ListEntry): the neighbour's index relative to the particle's own.

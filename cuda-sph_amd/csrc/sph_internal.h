@@ -187,12 +187,19 @@ hipError_t launch_upload_f64(sph_ctx* c, const double* pos, const double* vel, h
 hipError_t launch_upload_f32(sph_ctx* c, const float* x, const float* y, const float* z, const float* vx,
                              const float* vy, const float* vz, hipStream_t s);
 hipError_t launch_radix_sort_impl(sph_ctx* c, bool gather, hipStream_t s);
-hipError_t launch_cell_sort(sph_ctx* c, bool gather, hipStream_t s);  // sph_cellsort.hip; also builds cs
+// sph_cellsort.hip; also builds cs.  counted: the previous force pass counted the keys
+hipError_t launch_cell_sort(sph_ctx* c, bool gather, hipStream_t s, bool counted = false);
+// whether the counting sort runs on the global path (per-cell counts in cell_cnt), not in LDS
+bool cell_sort_global(const sph_ctx* c);
+// whether the force pass can count the keys it writes for the next step's counting sort:
+// a plain NAIVE step on the global sort path with one lane per particle
+bool count_in_force(const sph_ctx* c);
 int64_t cell_scan_tiles(int64_t cells);
 hipError_t launch_cell_table(sph_ctx* c, hipStream_t s);
 hipError_t launch_reorder(sph_ctx* c, hipStream_t s);
 hipError_t launch_density(sph_ctx* c, hipStream_t s);
-hipError_t launch_force(sph_ctx* c, hipStream_t s);
+// count: also rank the new keys into cell_cnt / keyS[0] for the next sort (count_in_force)
+hipError_t launch_force(sph_ctx* c, hipStream_t s, bool count = false);
 hipError_t launch_download_f64(sph_ctx* c, double* pos, double* vel, double* rho, double* force, hipStream_t s);
 hipError_t launch_download_f32(sph_ctx* c, float* x, float* y, float* z, float* vx, float* vy, float* vz,
                                float* rho, hipStream_t s);

@@ -337,7 +337,7 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_batch_kernel(St
 // their gathers in flight); a particle with more neighbours than list slots rescans its
 // candidates instead.  With L lanes per particle the group splits the list (or the rescan)
 // and lane 0 finishes the particle.  MODE selects the wall collision at compile time.
-template <int MODE, int L = 1>
+template <int MODE, int L = 1, bool COUNT = false>
 __global__ __launch_bounds__(256) void naive_force_kernel(
     StepConsts G, const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ pidA,
     const float* __restrict__ rhoA, const uint32_t* __restrict__ cs, const ListEntry* __restrict__ nl,
@@ -345,7 +345,8 @@ __global__ __launch_bounds__(256) void naive_force_kernel(
     uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
     uint64_t* __restrict__ rng, const PipeTable* __restrict__ pipe, float4* __restrict__ dbgP,
     float4* __restrict__ dbgV, float4* __restrict__ dbgF, uint32_t* __restrict__ flags,
-    float4* __restrict__ sums) {
+    float4* __restrict__ sums, uint32_t* __restrict__ cnt, uint32_t* __restrict__ rank) {
+    static_assert(!COUNT || L == 1, "the force pass counts keys with one lane per particle");
     const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int i = (int)(gt / L);
     if (i >= G.n) return;
@@ -359,9 +360,12 @@ __global__ __launch_bounds__(256) void naive_force_kernel(
     }
     const float ps = 0.5f * C.mass * C.grad_c;   // naive_kernels.py:40-52: m * (pi+pj)/(2 rho_j) * grad W
     const float vs = C.mass * C.visc * C.lap_c;  // naive_kernels.py:71-77: m * mu * (vj-vi)/rho_j * lap W
-    finish_particle<MODE, true>(C, i, p, v, pidA[i], rhoA[i], Vec3{ps * S.ax, ps * S.ay, ps * S.az},
-                          Vec3{vs * S.bx, vs * S.by, vs * S.bz}, posOut, velOut, pidOut, keyI, valI, rng, pipe, dbgP,
-                          dbgV, dbgF, flags);
+    const uint32_t key = finish_particle<MODE, true>(C, i, p, v, pidA[i], rhoA[i], Vec3{ps * S.ax, ps * S.ay, ps * S.az},
+                                                     Vec3{vs * S.bx, vs * S.by, vs * S.bz}, posOut, velOut, pidOut,
+                                                     keyI, valI, rng, pipe, dbgP, dbgV, dbgF, flags);
+    // the next sort's count (cell_count_kernel's work): the wavefront's particles past n left
+    // at the top, so the lanes here are a prefix of it
+    if constexpr (COUNT) rank[i] = rank_in_cell(key, true, cnt);
 }
 
 // Batched form of the force pass (box walls): the thread map of naive_density_batch_kernel.
@@ -758,16 +762,17 @@ hipError_t launch_density(sph_ctx* c, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <int MODE, int L>
+template <int MODE, int L, bool COUNT = false>
 void launch_force_lanes(sph_ctx* c, hipStream_t s, float4* dP, float4* dV, float4* dF) {
     const int bs = c->force_block;
-    naive_force_kernel<MODE, L><<<blocks_for(c->n * L, bs), bs, 0, s>>>(
+    naive_force_kernel<MODE, L, COUNT><<<blocks_for(c->n * L, bs), bs, 0, s>>>(
         c->sc, c->b.posA, c->b.velA, c->b.pidA, c->b.rhoA, c->b.cs, c->b.nl, c->b.nn, c->b.posB, c->b.velB,
-        c->b.pidB, c->b.keyI, c->b.valI, c->b.rng, c->b.pipe, dP, dV, dF, c->b.flags, c->b.sums);
+        c->b.pidB, c->b.keyI, c->b.valI, c->b.rng, c->b.pipe, dP, dV, dF, c->b.flags, c->b.sums,
+        COUNT ? c->b.cell_cnt : nullptr, COUNT ? c->b.keyS[0] : nullptr);
 }
 
 template <int MODE>
-void launch_force_mode(sph_ctx* c, hipStream_t s) {
+void launch_force_mode(sph_ctx* c, hipStream_t s, bool count) {
     const int bs = c->force_block;
     float4* dP = c->keep_terms ? c->b.dbgP : nullptr;
     float4* dV = c->keep_terms ? c->b.dbgV : nullptr;
@@ -784,6 +789,10 @@ void launch_force_mode(sph_ctx* c, hipStream_t s) {
             c->b.pidB, c->b.keyI, c->b.valI, dP, dV, dF, c->b.flags);
         return;
     }
+    if (count) {  // count_in_force: one lane per particle
+        launch_force_lanes<MODE, 1, true>(c, s, dP, dV, dF);
+        return;
+    }
     switch (c->lanes) {
         case 2: launch_force_lanes<MODE, 2>(c, s, dP, dV, dF); break;
         case 4: launch_force_lanes<MODE, 4>(c, s, dP, dV, dF); break;
@@ -795,10 +804,16 @@ void launch_force_mode(sph_ctx* c, hipStream_t s) {
     }
 }
 
-hipError_t launch_force(sph_ctx* c, hipStream_t s) {
+bool count_in_force(const sph_ctx* c) {
+    return c->cell_sort && c->inst.empty() && c->lanes == 1 && c->sc.formulation == SPH_FORM_NAIVE &&
+           cell_sort_global(c);
+}
+
+hipError_t launch_force(sph_ctx* c, hipStream_t s, bool count) {
     if (c->n == 0) return hipGetLastError();
-    if (c->sc.mode == SPH_MODE_PIPE) launch_force_mode<SPH_MODE_PIPE>(c, s);
-    else launch_force_mode<SPH_MODE_BOX>(c, s);
+    if (count && !count_in_force(c)) return hipErrorInvalidValue;
+    if (c->sc.mode == SPH_MODE_PIPE) launch_force_mode<SPH_MODE_PIPE>(c, s, count);
+    else launch_force_mode<SPH_MODE_BOX>(c, s, count);
     return hipGetLastError();
 }
 

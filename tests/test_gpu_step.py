@@ -411,11 +411,14 @@ def test_diagnostics_fp64(gpu):
     assert d["non_finite"] == 0
 
 
+@pytest.mark.parametrize("n", [3000, 20000])
 @pytest.mark.parametrize("mode", ["BOX", "PIPE"])
-def test_graph_replays_match_eager(gpu, mode):
+def test_graph_replays_match_eager(gpu, mode, n):
     """A captured run of steps, replayed several times with eager work and synchronisation
-    in between, advances the state exactly as eager steps do (PIPE: RNG streams included)."""
-    w = box_uniform(3000, seed=23) if mode == "BOX" else pipe_flow(n=3000, seed=24)
+    in between, advances the state exactly as eager steps do (PIPE: RNG streams included).
+    3,000 particles sort in one workgroup; 20,000 take the multi-workgroup counting sort,
+    whose counts the captured force passes make for the next step's sort."""
+    w = box_uniform(n, seed=23) if mode == "BOX" else pipe_flow(n=n, seed=24)
     pos, vel = f32(w.state.position), f32(w.state.velocity)
     eager = NaiveSPHStrategy(w.params, constants=w.constants, mode=mode, device=gpu)
     graph = NaiveSPHStrategy(w.params, constants=w.constants, mode=mode, device=gpu)
@@ -510,10 +513,11 @@ def _same_after(make, upload_state, steps):
 @pytest.mark.parametrize("mode,lanes", [("BOX", 1), ("BOX", 4), ("PIPE", 1)])
 def test_steps_in_one_call_equal_single_calls(gpu, mode, lanes):
     """One sph_step(k) call and k single-step calls give bitwise the same state, keys,
-    permutation and cell table (the per-cell counts are cleared between sorts whichever way
-    the steps are launched).  20,000 particles on 32,768 cells or more (the counting sort's
-    multi-workgroup path, which keeps counts in device memory), particles outside the grid
-    (BOX) and re-injection (PIPE)."""
+    permutation and cell table.  20,000 particles on 32,768 cells or more (the counting
+    sort's multi-workgroup path, which keeps counts in device memory): with one lane per
+    particle the k-step call's force passes count the keys for the next sort and the sorts
+    after the first skip their count; single calls count in every sort; with 4 lanes every
+    sort counts.  Particles outside the grid (BOX) and re-injection (PIPE)."""
     if mode == "BOX":
         w = box_uniform(20000, seed=31)
         pos = outside_points(f32(w.state.position), w.params.space_size)
