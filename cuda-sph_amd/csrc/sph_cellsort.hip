@@ -18,8 +18,8 @@
 //   scatter  one thread per slot: tmp[cs[key] + rank] = slot
 //   order    one thread per position: the particle there moves to cs[key] + (number of
 //            particles of its cell with a smaller id), which is id order within the cell; the
-//            sorted key and slot are written there and, in the fused step, the particle's
-//            state is gathered
+//            sorted key is written there and either the slot (read by the separate reorder
+//            pass of a staged step) or, in the fused step, the particle's gathered state
 // The within-cell pass costs one comparison per particle of the cell, less than the density
 // pass spends on the same cell.  The final order does not depend on the atomic order.
 // The per-slot passes (count, scatter, order) map workgroups to slot ranges XCD by XCD as
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(CT) void cell_order_kernel(int n, const uint32_t* _
     for (uint32_t t = a; t < b; ++t) r += pidB[tmp[t]] < id ? 1u : 0u;
     const uint32_t f = a + r;
     skey[f] = k;
-    sval[f] = slot;
+    if (!GATHER) sval[f] = slot;
     if (GATHER) {
         posA[f] = posB[slot];
         velA[f] = velB[slot];
