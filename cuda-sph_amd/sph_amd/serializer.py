@@ -1,8 +1,8 @@
 """Frame output in the reference's on-disk format, and an asynchronous writer for it.
 
 Format (common/serializer/saver.py:8-46, loader.py:9-64), one folder per run:
-  params.json          dataclasses.asdict(SimulationParameters) with arrays as lists,
-                       sorted keys, indent 4
+  params.json          every SimulationParameters field (the pipe as {"segments": [...]})
+                       with arrays as lists, sorted keys, indent 4
   position_<k>.npy     float64 (N, 3) state after frame k (k = 0, 1, ...)
   velocity_<k>.npy     float64 (N, 3)
   density_<k>.npy      float64 (N,)
@@ -33,77 +33,90 @@ from .data_classes import Pipe, Segment, SimulationParameters, SimulationState
 
 PARAMS_FILENAME = "params.json"  # config.py:43
 
+# One .npy file per quantity and frame: the fields of SimulationState, in order
+FRAME_QUANTITIES = tuple(f.name for f in fields(SimulationState))
 
-def _plain(o):
-    """JSON form of values json cannot encode itself: numpy scalars and arrays (a numpy
-    integer field would make the reference's json.dump fail), and objects by __dict__
-    (saver.py:45)."""
-    if isinstance(o, np.generic):
-        return o.item()
-    if isinstance(o, np.ndarray):
-        return o.tolist()
-    return o.__dict__
+
+def frame_file(folder: str, quantity: str, frame: int) -> str:
+    """<folder>/<quantity>_<frame>.npy, the name both classes below use for one quantity of
+    one frame (saver.py:28-36, loader.py:52-64)."""
+    return os.path.join(folder, f"{quantity}_{frame}.npy")
+
+
+def _json_form(value):
+    """A parameter value as params.json holds it: a dataclass as the object of its fields
+    (the pipe as {"segments": [...]}), arrays and tuples as lists, numpy scalars as plain
+    numbers (json cannot encode a numpy integer)."""
+    if dataclasses.is_dataclass(value):
+        return {f.name: _json_form(getattr(value, f.name)) for f in fields(value)}
+    if isinstance(value, np.ndarray):
+        return value.tolist()
+    if isinstance(value, (list, tuple)):
+        return [_json_form(v) for v in value]
+    if isinstance(value, np.generic):
+        return value.item()
+    return value
+
+
+def params_json(params: SimulationParameters) -> str:
+    """The text of params.json: every parameter in _json_form, keys sorted, indent 4."""
+    return json.dumps(_json_form(params), sort_keys=True, indent=4)
+
+
+def _default(f: dataclasses.Field):
+    return f.default_factory() if f.default is dataclasses.MISSING else f.default
+
+
+def params_from_json(obj: dict) -> SimulationParameters:
+    """The inverse of params_json: array parameters (those whose default is an array) back
+    to arrays, the pipe back to Pipe and Segment objects, the rest as read."""
+    values = {}
+    for f in fields(SimulationParameters):
+        v = obj[f.name]
+        if f.name == "pipe":
+            v = Pipe([Segment(start_point=tuple(s["start_point"]), start_radius=s["start_radius"],
+                              end_radius=s["end_radius"], length=s["length"]) for s in v["segments"]])
+        elif isinstance(_default(f), np.ndarray):
+            v = np.asarray(v)
+        values[f.name] = v
+    return SimulationParameters(**values)
 
 
 class Saver:
-    """common/serializer/saver.py:8-46: writes params.json at construction and one set of
-    .npy files per save_next_state call."""
+    """common/serializer/saver.py:8-46: the run's folder under `root` (created if missing),
+    params.json written at construction, then one frame per save_next_state call."""
 
     def __init__(self, out_dirname: str, params: SimulationParameters, root: Optional[str] = None) -> None:
-        self.out_folder_path = os.path.join(root if root is not None else os.getcwd(), out_dirname)
-        if not os.path.exists(self.out_folder_path):
+        self.out_folder_path = os.path.join(os.getcwd() if root is None else root, out_dirname)
+        if not os.path.isdir(self.out_folder_path):
             os.mkdir(self.out_folder_path)
-        self._save_parameters(params)
+        with open(os.path.join(self.out_folder_path, PARAMS_FILENAME), "w") as fh:
+            fh.write(params_json(params))
         self.current_epoch = 0
 
     def save_next_state(self, state: SimulationState) -> None:
-        """Saves the state as frame current_epoch and advances the counter (saver.py:28-36)."""
-        for name, value in vars(state).items():
-            np.save(os.path.join(self.out_folder_path, name + "_" + str(self.current_epoch)), value)
+        """Writes the state as frame current_epoch, then advances the frame counter."""
+        for q in FRAME_QUANTITIES:
+            np.save(frame_file(self.out_folder_path, q, self.current_epoch), getattr(state, q))
         self.current_epoch += 1
-
-    def _save_parameters(self, params: SimulationParameters) -> None:
-        d = dataclasses.asdict(params)
-        for key, value in d.items():
-            if type(value) is np.ndarray:
-                d[key] = value.tolist()
-        with open(os.path.join(self.out_folder_path, PARAMS_FILENAME), "w") as f:
-            json.dump(d, f, default=_plain, sort_keys=True, indent=4)
 
 
 class Loader:
-    """common/serializer/loader.py:9-64."""
+    """common/serializer/loader.py:9-64: reads a folder Saver wrote; a missing folder is an
+    error at construction (with the reference's message)."""
 
     def __init__(self, out_dirname: str, root: Optional[str] = None) -> None:
-        self.out_folder_path = os.path.join(root if root is not None else os.getcwd(), out_dirname)
+        self.out_folder_path = os.path.join(os.getcwd() if root is None else root, out_dirname)
         if not os.path.exists(self.out_folder_path):
             raise Exception(f"Directory ({self.out_folder_path}) does not exists! Could not load simulation!")
-        with open(os.path.join(self.out_folder_path, PARAMS_FILENAME)) as f:
-            self._json = json.load(f)
+        with open(os.path.join(self.out_folder_path, PARAMS_FILENAME)) as fh:
+            self._params = json.load(fh)
 
     def load_simulation_parameters(self) -> SimulationParameters:
-        d = {}
-        for f in fields(SimulationParameters):
-            if f.name == "pipe":
-                d["pipe"] = self._load_pipe()
-            elif f.type == "np.ndarray":
-                d[f.name] = np.asarray(self._json[f.name])
-            else:
-                d[f.name] = self._json[f.name]
-        return SimulationParameters(**d)
-
-    def _load_pipe(self) -> Pipe:
-        segs = []
-        for s in self._json["pipe"]["segments"]:
-            segs.append(Segment(start_point=tuple(s["start_point"]), start_radius=s["start_radius"],
-                                end_radius=s["end_radius"], length=s["length"]))
-        return Pipe(segs)
+        return params_from_json(self._params)
 
     def load_simulation_state(self, epoch: int) -> SimulationState:
-        d = {}
-        for f in fields(SimulationState):
-            d[f.name] = np.load(os.path.join(self.out_folder_path, f.name + "_" + str(epoch)) + ".npy")
-        return SimulationState(**d)
+        return SimulationState(**{q: np.load(frame_file(self.out_folder_path, q, epoch)) for q in FRAME_QUANTITIES})
 
 
 class AsyncFrameWriter:
@@ -168,9 +181,8 @@ class AsyncFrameWriter:
             s, epoch = item
             try:
                 self._copied[s].synchronize()
-                path = self.saver.out_folder_path
-                for name, t in zip(("position", "velocity", "density"), self._host[s]):
-                    np.save(os.path.join(path, f"{name}_{epoch}"), t.numpy())
+                for q, t in zip(FRAME_QUANTITIES, self._host[s]):
+                    np.save(frame_file(self.saver.out_folder_path, q, epoch), t.numpy())
             except BaseException as e:  # reported by the next capture() or close()
                 self._error = e
             finally:
