@@ -418,12 +418,12 @@ __device__ __forceinline__ bool box_bounced(const StepConsts& C, float x, int d)
 // returns the key.
 template <int MODE, bool SLOT_KEYS>
 __device__ __forceinline__ uint32_t finish_particle(const StepConsts& C, int i, float4 p, float4 v, uint32_t pid,
-                                                float rho, Vec3 P, Vec3 V, float4* __restrict__ posOut,
-                                                float4* __restrict__ velOut, uint32_t* __restrict__ pidOut,
-                                                uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
-                                                uint64_t* __restrict__ rng, const PipeTable* __restrict__ pipe,
-                                                float4* __restrict__ dbgP, float4* __restrict__ dbgV,
-                                                float4* __restrict__ dbgF, uint32_t* __restrict__ flags) {
+                                                    float rho, Vec3 P, Vec3 V, float4* __restrict__ posOut,
+                                                    float4* __restrict__ velOut, uint32_t* __restrict__ pidOut,
+                                                    uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
+                                                    uint64_t* __restrict__ rng, const PipeTable* __restrict__ pipe,
+                                                    float4* __restrict__ dbgP, float4* __restrict__ dbgV,
+                                                    float4* __restrict__ dbgF, uint32_t* __restrict__ flags) {
     const Integrated I = integrate_particle(C, p, v, rho, P, V);
     const float fx = I.f.x, fy = I.f.y, fz = I.f.z;
     Vec3 vv = I.v, pp = I.x;

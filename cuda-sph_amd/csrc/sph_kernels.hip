@@ -360,9 +360,9 @@ __global__ __launch_bounds__(256) void naive_force_kernel(
     }
     const float ps = 0.5f * C.mass * C.grad_c;   // naive_kernels.py:40-52: m * (pi+pj)/(2 rho_j) * grad W
     const float vs = C.mass * C.visc * C.lap_c;  // naive_kernels.py:71-77: m * mu * (vj-vi)/rho_j * lap W
-    const uint32_t key = finish_particle<MODE, true>(C, i, p, v, pidA[i], rhoA[i], Vec3{ps * S.ax, ps * S.ay, ps * S.az},
-                                                     Vec3{vs * S.bx, vs * S.by, vs * S.bz}, posOut, velOut, pidOut,
-                                                     keyI, valI, rng, pipe, dbgP, dbgV, dbgF, flags);
+    const Vec3 P{ps * S.ax, ps * S.ay, ps * S.az}, V{vs * S.bx, vs * S.by, vs * S.bz};
+    const uint32_t key = finish_particle<MODE, true>(C, i, p, v, pidA[i], rhoA[i], P, V, posOut, velOut, pidOut, keyI,
+                                                     valI, rng, pipe, dbgP, dbgV, dbgF, flags);
     // the next sort's count (cell_count_kernel's work): the wavefront's particles past n left
     // at the top, so the lanes here are a prefix of it
     if constexpr (COUNT) rank[i] = rank_in_cell(key, true, cnt);
