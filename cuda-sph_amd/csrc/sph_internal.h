@@ -192,7 +192,7 @@ hipError_t launch_cell_sort(sph_ctx* c, bool gather, hipStream_t s, bool counted
 // whether the counting sort runs on the global path (per-cell counts in cell_cnt), not in LDS
 bool cell_sort_global(const sph_ctx* c);
 // whether the force pass can count the keys it writes for the next step's counting sort:
-// a plain NAIVE step on the global sort path with one lane per particle
+// a plain (not batched) NAIVE step on the global sort path
 bool count_in_force(const sph_ctx* c);
 int64_t cell_scan_tiles(int64_t cells);
 hipError_t launch_cell_table(sph_ctx* c, hipStream_t s);

@@ -346,7 +346,6 @@ __global__ __launch_bounds__(256) void naive_force_kernel(
     uint64_t* __restrict__ rng, const PipeTable* __restrict__ pipe, float4* __restrict__ dbgP,
     float4* __restrict__ dbgV, float4* __restrict__ dbgF, uint32_t* __restrict__ flags,
     float4* __restrict__ sums, uint32_t* __restrict__ cnt, uint32_t* __restrict__ rank) {
-    static_assert(!COUNT || L == 1, "the force pass counts keys with one lane per particle");
     const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int i = (int)(gt / L);
     if (i >= G.n) return;
@@ -363,9 +362,11 @@ __global__ __launch_bounds__(256) void naive_force_kernel(
     const Vec3 P{ps * S.ax, ps * S.ay, ps * S.az}, V{vs * S.bx, vs * S.by, vs * S.bz};
     const uint32_t key = finish_particle<MODE, true>(C, i, p, v, pidA[i], rhoA[i], P, V, posOut, velOut, pidOut, keyI,
                                                      valI, rng, pipe, dbgP, dbgV, dbgF, flags);
-    // the next sort's count (cell_count_kernel's work): the wavefront's particles past n left
-    // at the top, so the lanes here are a prefix of it
-    if constexpr (COUNT) rank[i] = rank_in_cell(key, true, cnt);
+    // the next sort's count (cell_count_kernel's work).  One lane per particle: the
+    // wavefront's particles past n left at the top, so the lanes here are a prefix of it and
+    // runs of equal keys share an atomic; with L lanes only the groups' first lanes are
+    // here, at most 64 / L of them, and each counts its own particle.
+    if constexpr (COUNT) rank[i] = L == 1 ? rank_in_cell(key, true, cnt) : atomicAdd(&cnt[key], 1u);
 }
 
 // Batched form of the force pass (box walls): the thread map of naive_density_batch_kernel.
@@ -789,8 +790,16 @@ void launch_force_mode(sph_ctx* c, hipStream_t s, bool count) {
             c->b.pidB, c->b.keyI, c->b.valI, dP, dV, dF, c->b.flags);
         return;
     }
-    if (count) {  // count_in_force: one lane per particle
-        launch_force_lanes<MODE, 1, true>(c, s, dP, dV, dF);
+    if (count) {  // count_in_force
+        switch (c->lanes) {
+            case 2: launch_force_lanes<MODE, 2, true>(c, s, dP, dV, dF); break;
+            case 4: launch_force_lanes<MODE, 4, true>(c, s, dP, dV, dF); break;
+            case 8: launch_force_lanes<MODE, 8, true>(c, s, dP, dV, dF); break;
+            case 16: launch_force_lanes<MODE, 16, true>(c, s, dP, dV, dF); break;
+            case 32: launch_force_lanes<MODE, 32, true>(c, s, dP, dV, dF); break;
+            case 64: launch_force_lanes<MODE, 64, true>(c, s, dP, dV, dF); break;
+            default: launch_force_lanes<MODE, 1, true>(c, s, dP, dV, dF); break;
+        }
         return;
     }
     switch (c->lanes) {
@@ -805,8 +814,7 @@ void launch_force_mode(sph_ctx* c, hipStream_t s, bool count) {
 }
 
 bool count_in_force(const sph_ctx* c) {
-    return c->cell_sort && c->inst.empty() && c->lanes == 1 && c->sc.formulation == SPH_FORM_NAIVE &&
-           cell_sort_global(c);
+    return c->cell_sort && c->inst.empty() && c->sc.formulation == SPH_FORM_NAIVE && cell_sort_global(c);
 }
 
 hipError_t launch_force(sph_ctx* c, hipStream_t s, bool count) {
