@@ -141,12 +141,28 @@ constexpr int kDensityUnroll = SPH_DENSITY_UNROLL;
 typedef ListEntry DensityList[kDensityThreads];
 constexpr int kStageCap = 2 * kLdsListCap;  // the same LDS bytes hold twice the slots
 __device__ __forceinline__ ListEntry stage_value(uint32_t k, int i) { return list_encode(k, i); }
-__device__ __forceinline__ ListEntry staged_entry(ListEntry v, int) { return v; }
+// Slot t of LDS list column col: slots 2w and 2w + 1 are the low and high halves of the
+// 32-bit word (w, col), so no two lanes of a store share a word (nor its bank), and the
+// copy-out reads each list word, two slots already in the list's format, with one load.
+__device__ __forceinline__ ListEntry& stage_at(DensityList* s, uint32_t t, uint32_t col) {
+    return (&s[0][0])[((t >> 1) * kDensityThreads + col) * 2 + (t & 1)];
+}
+// slots t and t + 1 (t even) as one list word; the high half is zero past `staged` slots
+__device__ __forceinline__ uint32_t staged_word(DensityList* s, uint32_t t, uint32_t col, int, uint32_t staged) {
+    uint32_t w;
+    __builtin_memcpy(&w, &stage_at(s, t, col), sizeof(w));
+    return t + 1 < staged ? w : (w & 0xffffu);
+}
 #else
 typedef uint32_t DensityList[kDensityThreads];
 constexpr int kStageCap = kLdsListCap;
 __device__ __forceinline__ uint32_t stage_value(uint32_t k, int) { return k; }
-__device__ __forceinline__ ListEntry staged_entry(uint32_t k, int i) { return list_encode(k, i); }
+__device__ __forceinline__ uint32_t& stage_at(DensityList* s, uint32_t t, uint32_t col) { return s[t][col]; }
+__device__ __forceinline__ uint32_t staged_word(DensityList* s, uint32_t t, uint32_t col, int i, uint32_t staged) {
+    const uint32_t lo = list_encode(s[t][col], i);
+    const uint32_t hi = t + 1 < staged ? list_encode(s[t + 1][col], i) : 0u;
+    return lo | (hi << 16);
+}
 #endif
 
 // Density of cell-ordered particle i with L lanes per particle.  L = 1: the lane walks every
@@ -182,10 +198,10 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
             const float t = 1.0f - r2 * C.inv_h2;
             acc += t * t * t;
             if constexpr (decltype(checked)::value) {
-                if (cnt < (uint32_t)kStageCap) s_list[cnt][col] = stage_value(k, i);
+                if (cnt < (uint32_t)kStageCap) stage_at(s_list, cnt, col) = stage_value(k, i);
                 else if (cnt < (uint32_t)kListCap) nl[list_at(G.n, i, cnt)] = list_encode(k, i);
             } else {
-                s_list[cnt][col] = stage_value(k, i);
+                stage_at(s_list, cnt, col) = stage_value(k, i);
             }
             ++cnt;
         }
@@ -227,7 +243,7 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
                 const uint64_t gm = (__ballot(hit) >> gbase) & group;
                 if (hit) {
                     const uint32_t slot = cnt + (uint32_t)__popcll(gm & ((1ull << l) - 1ull));
-                    if (slot < (uint32_t)kStageCap) s_list[slot][col] = stage_value(k, i);
+                    if (slot < (uint32_t)kStageCap) stage_at(s_list, slot, col) = stage_value(k, i);
                     else if (slot < (uint32_t)kListCap) nl[list_at(G.n, i, slot)] = list_encode(k, i);
                 }
                 cnt += (uint32_t)__popcll(gm);
@@ -249,9 +265,7 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
     // the staged slots as 32-bit words (two slots each), the group's lanes taking turns
     uint32_t* words = reinterpret_cast<uint32_t*>(nl);
     for (uint32_t t = 2 * l; t < staged; t += 2 * L) {
-        const uint32_t lo = staged_entry(s_list[t][col], i);
-        const uint32_t hi = t + 1 < staged ? staged_entry(s_list[t + 1][col], i) : 0u;
-        words[(size_t)(t >> 1) * stride + i] = lo | (hi << 16);
+        words[(size_t)(t >> 1) * stride + i] = staged_word(s_list, t, col, i, staged);
     }
     if (l == 0) {
         // a particle with a far neighbour is rescanned by the force pass, as one past the cap
@@ -269,7 +283,7 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_kernel(StepCons
                                                                          ListEntry* __restrict__ nl,
                                                                          uint32_t* __restrict__ nn) {
     static_assert(L >= 1 && L <= 64 && (L & (L - 1)) == 0, "lanes per particle: 1, 2, 4, ..., 64");
-    __shared__ DensityList s_list[kStageCap];
+    __shared__ __align__(16) DensityList s_list[kStageCap];
     const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int i = (int)(gt / L);
     if (i >= G.n) return;
@@ -312,7 +326,7 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_batch_kernel(St
                                                                                float* __restrict__ rhoA,
                                                                                ListEntry* __restrict__ nl,
                                                                                uint32_t* __restrict__ nn) {
-    __shared__ DensityList s_list[kStageCap];
+    __shared__ __align__(16) DensityList s_list[kStageCap];
     const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const BatchLane t = batch_lane(G, gt);
     if (t.i < 0) return;
