@@ -22,6 +22,7 @@ SPH_FLAG_KEEP_TERMS = 1
 SPH_FLAG_ADJOINT = 2
 SPH_FLAG_CHECK_FINITE = 4
 SPH_E_NONFINITE = -4
+SPH_ABI_VERSION = 1  # include/sph_amd.h; the loaded library must report the same
 
 STAGE_SORT, STAGE_TABLE, STAGE_REORDER, STAGE_DENSITY, STAGE_FORCE = 1, 2, 3, 4, 5
 
@@ -116,6 +117,10 @@ def load_library() -> ctypes.CDLL:
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res
+    version = lib.sph_abi_version()
+    if version != SPH_ABI_VERSION:
+        raise RuntimeError(f"{LIB_PATH} reports ABI version {version}, this binding expects {SPH_ABI_VERSION}: "
+                           "rebuild it with __graft_entry__.build()")
     _lib = lib
     return lib
 

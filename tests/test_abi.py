@@ -68,6 +68,15 @@ def _consts(h=0.0457):
     return N.make_consts(SPHConstants(mass=0.02, rho0=1000, h=h, visc=3.5, k=3, damp=0.5))
 
 
+def test_abi_version_agrees_with_header():
+    import os
+    import re
+
+    hdr = open(os.path.join(os.path.dirname(__file__), "..", "include", "sph_amd.h")).read()
+    assert int(re.search(r"#define SPH_ABI_VERSION (\d+)", hdr).group(1)) == N.SPH_ABI_VERSION
+    assert N.load_library().sph_abi_version() == N.SPH_ABI_VERSION  # load_library checks it too
+
+
 def test_create_plans_grid_and_sort():
     lib = N.load_library()
     assert lib.sph_abi_version() == 1
