@@ -680,7 +680,7 @@ __global__ __launch_bounds__(kDiagThreads) void diag_partial_kernel(int n, const
     for (int k = 0; k < kDiagVals; ++k) red[k][threadIdx.x] = acc[k];
     __syncthreads();
     for (int s = kDiagThreads / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
+        if ((int)threadIdx.x < s)
             for (int k = 0; k < kDiagVals; ++k)
                 red[k][threadIdx.x] = k == 5 ? fmax(red[k][threadIdx.x], red[k][threadIdx.x + s])
                                              : red[k][threadIdx.x] + red[k][threadIdx.x + s];
