@@ -438,12 +438,27 @@ __device__ __forceinline__ void voxel_slab2(float p, int v, float s, float e2[3]
     }
 }
 
+// The list is staged in LDS as 16-bit offsets from the particle's own index, two slots per
+// 32-bit word of the lane's column (slot t of column col: half t & 1 of word (t / 2, col)),
+// and copied out slot by slot at the end, so every list store is coalesced across the
+// wavefront (a store per hit, straight to the column-major list, wrote each 64-byte segment
+// many times over: 985 MB per launch on the 1M dam break for 134 MB of lists).  An entry
+// farther than an int16 offset is written directly and its slot marked kVoxelFar.
+constexpr uint16_t kVoxelFar = 0x8000u;
+constexpr size_t kVoxelStageBytes = (size_t)kMaxNeighbours * sizeof(uint16_t);  // per thread
+
+__device__ __forceinline__ uint16_t& voxel_stage_at(uint16_t* s, uint32_t t, uint32_t col) {
+    return s[((t >> 1) * blockDim.x + col) * 2 + (t & 1)];
+}
+
 __global__ __launch_bounds__(256) void voxel_density_kernel(StepConsts C, float4* pos, float4* vel,
                                                              const uint32_t* __restrict__ cs,
                                                              float* __restrict__ rhoA, int32_t* __restrict__ nbr,
                                                              const uint32_t* __restrict__ flags) {
+    extern __shared__ __align__(16) uint16_t s_vlist[];  // kVoxelStageBytes per thread of the block
     const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= C.n) return;
+    const uint32_t col = threadIdx.x;
     // a voxel beyond h may be skipped only if every particle it lists lies in its slab: the
     // lookup dims name the voxels the keys do and no position of the sorted state lies
     // outside the grid (a wrapped key lists such a particle under another voxel)
@@ -484,7 +499,11 @@ __global__ __launch_bounds__(256) void voxel_density_kernel(StepConsts C, float4
                         const float ddx = p.x - q[u].x, ddy = p.y - q[u].y, ddz = p.z - q[u].z;
                         const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
                         if (r2 <= C.r2_accept) {  // sqrtf(r2) <= h, exactly
-                            nbr[(size_t)(1 + cnt) * n + i] = (int32_t)e;
+                            const int off = (int)e - i;
+                            uint16_t staged = kVoxelFar;
+                            if (off >= -32767 && off <= 32767) staged = (uint16_t)(int16_t)off;
+                            else nbr[(size_t)(1 + cnt) * n + i] = (int32_t)e;
+                            voxel_stage_at(s_vlist, (uint32_t)cnt, col) = staged;
                             ++cnt;
                             if (e != (uint32_t)i) {  // voxel_kernels.py:127-130: rho skips the particle itself
                                 const float t = 1.0f - r2 * C.inv_h2;
@@ -495,6 +514,13 @@ __global__ __launch_bounds__(256) void voxel_density_kernel(StepConsts C, float4
                 }
             }
     nbr[i] = cnt;
+    for (int t = 0; t < cnt; t += 2) {
+        uint32_t w;
+        __builtin_memcpy(&w, &voxel_stage_at(s_vlist, (uint32_t)t, col), sizeof(w));
+        const uint16_t lo = (uint16_t)(w & 0xffffu), hi = (uint16_t)(w >> 16);
+        if (lo != kVoxelFar) nbr[(size_t)(1 + t) * n + i] = i + (int)(int16_t)lo;
+        if (t + 1 < cnt && hi != kVoxelFar) nbr[(size_t)(2 + t) * n + i] = i + (int)(int16_t)hi;
+    }
     store_density(C, i, acc, pos, vel, rhoA);
 }
 
@@ -755,7 +781,8 @@ hipError_t launch_density(sph_ctx* c, hipStream_t s) {
     if (c->n == 0) return hipGetLastError();
     const int bs = c->density_block;
     if (c->sc.formulation == SPH_FORM_VOXEL) {
-        voxel_density_kernel<<<blocks_for(c->n, bs), bs, 0, s>>>(c->sc, c->b.posA, c->b.velA, c->b.cs, c->b.rhoA,
+        voxel_density_kernel<<<blocks_for(c->n, bs), bs, bs * kVoxelStageBytes, s>>>(c->sc, c->b.posA, c->b.velA,
+                                                                                   c->b.cs, c->b.rhoA,
                                                                    c->b.nbr, c->b.flags);
         return hipGetLastError();
     }
