@@ -132,38 +132,30 @@ constexpr int kDensityThreads = 256;
 #endif
 constexpr int kDensityUnroll = SPH_DENSITY_UNROLL;
 
-// LDS staging of the first list slots: the entries themselves (16-bit offsets) when
-// SPH_LIST_STAGE16, else full indices.
-#ifndef SPH_LIST_STAGE16
-#define SPH_LIST_STAGE16 1
-#endif
-#if SPH_LIST_STAGE16
-typedef ListEntry DensityList[kDensityThreads];
-constexpr int kStageCap = 2 * kLdsListCap;  // the same LDS bytes hold twice the slots
-__device__ __forceinline__ ListEntry stage_value(uint32_t k, int i) { return list_encode(k, i); }
-// Slot t of LDS list column col: slots 2w and 2w + 1 are the low and high halves of the
-// 32-bit word (w, col), so no two lanes of a store share a word (nor its bank), and the
-// copy-out reads each list word, two slots already in the list's format, with one load.
-__device__ __forceinline__ ListEntry& stage_at(DensityList* s, uint32_t t, uint32_t col) {
-    return (&s[0][0])[((t >> 1) * kDensityThreads + col) * 2 + (t & 1)];
-}
-// slots t and t + 1 (t even) as one list word; the high half is zero past `staged` slots
-__device__ __forceinline__ uint32_t staged_word(DensityList* s, uint32_t t, uint32_t col, int, uint32_t staged) {
-    uint32_t w;
-    __builtin_memcpy(&w, &stage_at(s, t, col), sizeof(w));
-    return t + 1 < staged ? w : (w & 0xffffu);
-}
-#else
-typedef uint32_t DensityList[kDensityThreads];
-constexpr int kStageCap = kLdsListCap;
-__device__ __forceinline__ uint32_t stage_value(uint32_t k, int) { return k; }
-__device__ __forceinline__ uint32_t& stage_at(DensityList* s, uint32_t t, uint32_t col) { return s[t][col]; }
-__device__ __forceinline__ uint32_t staged_word(DensityList* s, uint32_t t, uint32_t col, int i, uint32_t staged) {
-    const uint32_t lo = list_encode(s[t][col], i);
-    const uint32_t hi = t + 1 < staged ? list_encode(s[t + 1][col], i) : 0u;
-    return lo | (hi << 16);
-}
-#endif
+// LDS staging of the list slots, as 16-bit entries (the list's own format).  Each wavefront
+// owns a band of kBandSlots entries; with L lanes per particle the band holds 64 / L
+// particles with 32 L slots each, so a particle with 4 or more lanes stages every one of its
+// kListCap slots (with fewer lanes, slots past 32 L go straight to HBM: one scattered 16-bit
+// store per hit).  Slots 2w and 2w + 1 of particle p of a band are the halves of 32-bit word
+// w (64 / L) + p, so no two lanes of a store share a word (nor its bank), and the copy-out
+// reads each list word with one load.
+constexpr int kBandSlots = 64 * 2 * kLdsListCap;  // 2,048 entries (4 KiB) per wavefront
+typedef ListEntry DensityList[kBandSlots];        // one band
+template <int L>
+constexpr uint32_t stage_cap() { return (uint32_t)(2 * kLdsListCap * L < kListCap ? 2 * kLdsListCap * L : kListCap); }
+
+template <int L>
+struct StageBand {
+    ListEntry* base;  // the wavefront's band
+    uint32_t p;       // the particle's place in it, 0 .. 64 / L - 1
+    __device__ __forceinline__ ListEntry& at(uint32_t t) const { return base[((t >> 1) * (64 / L) + p) * 2 + (t & 1)]; }
+    // slots t and t + 1 (t even) as one list word; the high half is zero past `staged` slots
+    __device__ __forceinline__ uint32_t word(uint32_t t, uint32_t staged) const {
+        uint32_t w;
+        __builtin_memcpy(&w, &at(t), sizeof(w));
+        return t + 1 < staged ? w : (w & 0xffffu);
+    }
+};
 
 // Density of cell-ordered particle i with L lanes per particle.  L = 1: the lane walks every
 // candidate row, two candidates in flight through 32-bit buffer offsets.  L = 2..16: a group
@@ -172,14 +164,15 @@ __device__ __forceinline__ uint32_t staged_word(DensityList* s, uint32_t t, uint
 // many candidates is spread over L lanes; hits are appended in candidate order (a group
 // ballot ranks them), so the neighbour list is the one a single lane builds, and the density
 // is summed per lane, then across the group.  G holds the totals (particle count, cell
-// table length), C the particle's own instance; col is the particle's LDS list column
+// table length), C the particle's own instance; band is the particle's place in its
+// wavefront's LDS staging band
 // (unique in the workgroup), gbase its group's first lane in the wavefront.
 template <int L>
 __device__ __forceinline__ void density_particle(const StepConsts& G, const StepConsts& C, int i, uint32_t l,
-                                                 uint32_t col, uint32_t gbase, float4* pos, float4* vel,
+                                                 StageBand<L> band, uint32_t gbase, float4* pos, float4* vel,
                                                  const uint32_t* __restrict__ cs, float* __restrict__ rhoA,
-                                                 ListEntry* __restrict__ nl, uint32_t* __restrict__ nn,
-                                                 DensityList* s_list) {
+                                                 ListEntry* __restrict__ nl, uint32_t* __restrict__ nn) {
+    constexpr uint32_t kCap = stage_cap<L>();
     const __amdgpu_buffer_rsrc_t rpos = make_rsrc(pos, (uint32_t)G.n * 16u);
     const __amdgpu_buffer_rsrc_t rcs = make_rsrc(cs, ((uint32_t)G.cs_cells + 1u) * 4u);
     const float4 p = pos[i];
@@ -198,10 +191,10 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
             const float t = 1.0f - r2 * C.inv_h2;
             acc += t * t * t;
             if constexpr (decltype(checked)::value) {
-                if (cnt < (uint32_t)kStageCap) stage_at(s_list, cnt, col) = stage_value(k, i);
+                if (cnt < kCap) band.at(cnt) = list_encode(k, i);
                 else if (cnt < (uint32_t)kListCap) nl[list_at(G.n, i, cnt)] = list_encode(k, i);
             } else {
-                stage_at(s_list, cnt, col) = stage_value(k, i);
+                band.at(cnt) = list_encode(k, i);
             }
             ++cnt;
         }
@@ -225,7 +218,7 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
             };
             // while every active lane's list has room in LDS for all of the row's candidates, the
             // row's hits need no capacity checks (a wave-uniform choice)
-            if (__ballot(cnt + (k1 - k0) > (uint32_t)kStageCap) == 0) walk(std::false_type{});
+            if (__ballot(cnt + (k1 - k0) > kCap) == 0) walk(std::false_type{});
             else walk(std::true_type{});
         } else {
             for (uint32_t base = k0; base < k1; base += L) {
@@ -243,7 +236,7 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
                 const uint64_t gm = (__ballot(hit) >> gbase) & group;
                 if (hit) {
                     const uint32_t slot = cnt + (uint32_t)__popcll(gm & ((1ull << l) - 1ull));
-                    if (slot < (uint32_t)kStageCap) stage_at(s_list, slot, col) = stage_value(k, i);
+                    if (slot < kCap) band.at(slot) = list_encode(k, i);
                     else if (slot < (uint32_t)kListCap) nl[list_at(G.n, i, slot)] = list_encode(k, i);
                 }
                 cnt += (uint32_t)__popcll(gm);
@@ -261,11 +254,11 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
     for (int o = 1; o < L; o <<= 1) acc += __shfl_xor(acc, o);
     // the lanes of a group walk the same rows, so they agree on this
     const bool wide = klo < khi && (klo + 32768u < (uint32_t)i || khi > (uint32_t)i + 32768u);
-    const uint32_t staged = cnt < (uint32_t)kStageCap ? cnt : (uint32_t)kStageCap;
+    const uint32_t staged = cnt < kCap ? cnt : kCap;
     // the staged slots as 32-bit words (two slots each), the group's lanes taking turns
     uint32_t* words = reinterpret_cast<uint32_t*>(nl);
     for (uint32_t t = 2 * l; t < staged; t += 2 * L) {
-        words[(size_t)(t >> 1) * stride + i] = staged_word(s_list, t, col, i, staged);
+        words[(size_t)(t >> 1) * stride + i] = band.word(t, staged);
     }
     if (l == 0) {
         // a particle with a far neighbour is rescanned by the force pass, as one past the cap
@@ -283,12 +276,13 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_kernel(StepCons
                                                                          ListEntry* __restrict__ nl,
                                                                          uint32_t* __restrict__ nn) {
     static_assert(L >= 1 && L <= 64 && (L & (L - 1)) == 0, "lanes per particle: 1, 2, 4, ..., 64");
-    __shared__ __align__(16) DensityList s_list[kStageCap];
+    __shared__ __align__(16) DensityList s_stage[kDensityThreads / 64];
     const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int i = (int)(gt / L);
     if (i >= G.n) return;
-    density_particle<L>(G, G, i, gt & (L - 1), threadIdx.x / L, (threadIdx.x & 63u) & ~(uint32_t)(L - 1), pos, vel,
-                        cs, rhoA, nl, nn, s_list);
+    const uint32_t lane = threadIdx.x & 63u;
+    const StageBand<L> band{s_stage[threadIdx.x >> 6], lane / L};
+    density_particle<L>(G, G, i, gt & (L - 1), band, lane & ~(uint32_t)(L - 1), pos, vel, cs, rhoA, nl, nn);
 }
 
 // Batched form: each instance has its own lanes per particle and its own thread range, which
@@ -326,23 +320,23 @@ __global__ __launch_bounds__(kDensityThreads) void naive_density_batch_kernel(St
                                                                                float* __restrict__ rhoA,
                                                                                ListEntry* __restrict__ nl,
                                                                                uint32_t* __restrict__ nn) {
-    __shared__ __align__(16) DensityList s_list[kStageCap];
+    __shared__ __align__(16) DensityList s_stage[kDensityThreads / 64];
     const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const BatchLane t = batch_lane(G, gt);
     if (t.i < 0) return;
     const StepConsts& C = G.inst_table[t.b];
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = threadIdx.x & ~63u;
     const uint32_t gbase = lane & ~(uint32_t)(t.lanes - 1);
-    const uint32_t col = wave + (lane >> C.lane_shift);
+    ListEntry* const base = s_stage[threadIdx.x >> 6];
+    const uint32_t p = lane >> C.lane_shift;  // the particle's place in the wavefront's band
     switch (t.lanes) {
-        case 1: density_particle<1>(G, C, t.i, 0u, col, gbase, pos, vel, cs, rhoA, nl, nn, s_list); break;
-        case 2: density_particle<2>(G, C, t.i, t.l, col, gbase, pos, vel, cs, rhoA, nl, nn, s_list); break;
-        case 4: density_particle<4>(G, C, t.i, t.l, col, gbase, pos, vel, cs, rhoA, nl, nn, s_list); break;
-        case 8: density_particle<8>(G, C, t.i, t.l, col, gbase, pos, vel, cs, rhoA, nl, nn, s_list); break;
-        case 16: density_particle<16>(G, C, t.i, t.l, col, gbase, pos, vel, cs, rhoA, nl, nn, s_list); break;
-        case 32: density_particle<32>(G, C, t.i, t.l, col, gbase, pos, vel, cs, rhoA, nl, nn, s_list); break;
-        default: density_particle<64>(G, C, t.i, t.l, col, gbase, pos, vel, cs, rhoA, nl, nn, s_list); break;
+        case 1: density_particle<1>(G, C, t.i, 0u, StageBand<1>{base, p}, gbase, pos, vel, cs, rhoA, nl, nn); break;
+        case 2: density_particle<2>(G, C, t.i, t.l, StageBand<2>{base, p}, gbase, pos, vel, cs, rhoA, nl, nn); break;
+        case 4: density_particle<4>(G, C, t.i, t.l, StageBand<4>{base, p}, gbase, pos, vel, cs, rhoA, nl, nn); break;
+        case 8: density_particle<8>(G, C, t.i, t.l, StageBand<8>{base, p}, gbase, pos, vel, cs, rhoA, nl, nn); break;
+        case 16: density_particle<16>(G, C, t.i, t.l, StageBand<16>{base, p}, gbase, pos, vel, cs, rhoA, nl, nn); break;
+        case 32: density_particle<32>(G, C, t.i, t.l, StageBand<32>{base, p}, gbase, pos, vel, cs, rhoA, nl, nn); break;
+        default: density_particle<64>(G, C, t.i, t.l, StageBand<64>{base, p}, gbase, pos, vel, cs, rhoA, nl, nn); break;
     }
 }
 
