@@ -11,8 +11,9 @@ run on.
 
 Also measured in the same run:
   tuning        the autotuner's choice of grid cell edge and block sizes (before timing)
-  stage timing  events around each stage of the same K steps run stage by stage (each
-                stage's kernels are one or a few launches, so event time = kernel time);
+  stage timing  the same K steps again in one call, with HIP events on the kernel stream
+                between the sort, density and force stages (sph_step_timed), so the
+                stages are timed with the kernels and launch sequence of the timed region;
   roofline      algorithmic bytes per launch / event time of the dominant kernel (and of
                 the force kernel, and per stage and per step in "kernels"/"step_roofline");
   repeats       the timed region repeated five times, and the median;
@@ -138,34 +139,6 @@ def make_workload(name):
 
     return {"dam_break_1m": workloads.dam_break_1m, "box_4k": workloads.box_4k,
             "pipe_512k": workloads.pipe_flow}[name]()
-
-
-def stage_times(strat, nsteps: int):
-    """Mean milliseconds of each stage over nsteps staged steps (events on the kernel stream)."""
-    import torch
-
-    from sph_amd import _native as N
-
-    names = {N.STAGE_SORT: "sort", N.STAGE_TABLE: "cell_table", N.STAGE_REORDER: "reorder",
-             N.STAGE_DENSITY: "density", N.STAGE_FORCE: "force"}
-    acc = {v: 0.0 for v in names.values()}
-    stream = torch.cuda.current_stream()
-    done = 0
-    while done < nsteps:
-        chunk = min(50, nsteps - done)
-        evs = []
-        for _ in range(chunk):
-            for st in names:
-                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                a.record(stream)
-                strat.ctx.run_stage(st)
-                b.record(stream)
-                evs.append((names[st], a, b))
-        torch.cuda.synchronize()
-        for name, a, b in evs:
-            acc[name] += a.elapsed_time(b)
-        done += chunk
-    return {k: v / max(nsteps, 1) for k, v in acc.items()}
 
 
 def neighbour_stats(w, sample=2048, seed=0):
@@ -647,17 +620,18 @@ def main() -> int:
     value = n * args.steps / (gpu_ms / 1e3)
 
     # per-kernel timing over the same trajectory: the step is deterministic, so replaying
-    # the upload, the warm-up and K staged steps revisits exactly the timed states
+    # the upload, the warm-up and the K steps (one sph_step_timed call: the timed region's
+    # kernels, with HIP events between the stages) revisits exactly the timed states
     log(f"stage timing over the same {args.steps} steps")
     strat.ctx.upload(pos, vel)
     strat.ctx.step(args.warmup)
-    st = stage_times(strat, args.steps)
+    st = strat.ctx.step_timed(args.steps)
     cand, rows, pairs = neighbour_stats(w)
     grid = strat.ctx.grid_info()
     n_cells = int(np.prod(grid["dims"]))
-    per_particle = {"sort": SORT_BYTES_PER_PASS * grid["passes"], "cell_table": 4 + 4 * n_cells / n,
-                    "reorder": REORDER_BYTES_PER_PARTICLE, "density": DENSITY_BYTES_PER_PARTICLE,
-                    "force": FORCE_BYTES_PER_PARTICLE}
+    # the sort stage: counting-sort passes, the cell table and the gather into cell order
+    per_particle = {"sort": SORT_BYTES_PER_PASS * grid["passes"] + 4 + 4 * n_cells / n + REORDER_BYTES_PER_PARTICLE,
+                    "density": DENSITY_BYTES_PER_PARTICLE, "force": FORCE_BYTES_PER_PARTICLE}
     kernels = {}
     for k, ms in st.items():
         gbs = per_particle[k] * n / (ms / 1e3) / 1e9

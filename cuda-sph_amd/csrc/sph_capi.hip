@@ -5,6 +5,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 
 #include "sph_internal.h"
@@ -209,8 +210,49 @@ int plan_instance(const sph_params_t& P, const sph_constants_t& K, sph::StepCons
     return SPH_OK;
 }
 
+#ifndef SPH_SLAB_SHIFT
+#define SPH_SLAB_SHIFT 2
+#endif
+
+// x slabs per cell of the NAIVE grid (StepConsts::slab_shift) and the layout of their
+// offsets in the cell table: four slabs (SPH_SLAB_SHIFT = 2) while the start, which needs
+// bit_length(n) bits, leaves room for three offset fields, else two.  The field width is
+// fixed at 3 bits (offsets up to 6), so whether a cell's offsets are kept depends on its
+// particles alone: a batch instance walks the same candidate ranges as the same simulation
+// run alone, given the same slab count (batches of fewer than 2^23 particles).  The bits
+// left above the fields (up to 16) keep the cell's count where it fits.  The VOXEL
+// formulation keeps the reference's table.
+void plan_slabs(sph_ctx* c) {
+    constexpr int kField = 3;
+    int shift = 0, start_bits = 32, field = 0;
+    if (c->sc.formulation == SPH_FORM_NAIVE) {
+        const int need = bit_length((uint64_t)(c->n > 0 ? c->n : 1));
+        for (int sh = SPH_SLAB_SHIFT; sh > 0; --sh) {
+            if (need + kField * ((1 << sh) - 1) <= 32) {
+                shift = sh;
+                start_bits = need;
+                field = kField;
+                break;
+            }
+        }
+    }
+    const int count_pos = start_bits + field * ((1 << shift) - 1);
+    const int count_bits = shift > 0 ? std::min(16, 32 - count_pos) : 0;
+    auto apply = [&](sph::StepConsts& S) {
+        S.slab_shift = shift;
+        S.start_bits = start_bits;
+        S.slab_bits = field;
+        S.count_pos = count_pos;
+        S.count_bits = count_bits;
+        S.start_mask = start_bits >= 32 ? ~0u : (1u << start_bits) - 1u;
+    };
+    apply(c->sc);
+    for (auto& S : c->inst) apply(S);
+}
+
 // Sort plan and workspace size from c->sc (totals) and c->n.
 void plan_sort_and_layout(sph_ctx* c) {
+    plan_slabs(c);
     const int64_t cells = c->sc.cs_cells;
     c->key_bits = c->sc.ref_keys ? 32 : bit_length((uint64_t)(cells > 1 ? cells - 1 : 0));
     c->passes = (c->key_bits + 7) / 8;
@@ -334,13 +376,19 @@ int plan_batch(sph_ctx* c, const sph_params_t* params, const sph_constants_t* co
     return SPH_OK;
 }
 
+// Records ev (timing of sph_step_timed) on s, if given.
+int mark(hipEvent_t ev, hipStream_t s) { return ev ? hip_check(hipEventRecord(ev, s), "record event") : SPH_OK; }
+
 // Sort (with the gather into cell order), cell table and density of a step.  counted: the
-// previous step's force pass, in the same call, counted the keys (see run_steps).
-int run_density_stages(sph_ctx* c, hipStream_t s, bool fused, bool counted = false) {
+// previous step's force pass, in the same call, counted the keys (see run_steps).  ev: if
+// given, events recorded before the sort (ev[0]) and before the density pass (ev[1]).
+int run_density_stages(sph_ctx* c, hipStream_t s, bool fused, bool counted = false, hipEvent_t* ev = nullptr) {
     int rc;
+    if (ev && (rc = mark(ev[0], s))) return rc;
     if (c->cell_sort) {  // the counting sort builds the cell table itself
         if ((rc = hip_check(sph::launch_cell_sort(c, fused, s, counted), "cell sort"))) return rc;
         if (!fused && (rc = hip_check(sph::launch_reorder(c, s), "reorder"))) return rc;
+        if (ev && (rc = mark(ev[1], s))) return rc;
         return hip_check(sph::launch_density(c, s), "density");
     }
     if (fused && c->passes > 0) {
@@ -350,13 +398,18 @@ int run_density_stages(sph_ctx* c, hipStream_t s, bool fused, bool counted = fal
         if ((rc = hip_check(sph::launch_reorder(c, s), "reorder"))) return rc;
     }
     if ((rc = hip_check(sph::launch_cell_table(c, s), "cell table"))) return rc;
+    if (ev && (rc = mark(ev[1], s))) return rc;
     return hip_check(sph::launch_density(c, s), "density");
 }
 
-int run_step_stages(sph_ctx* c, hipStream_t s, bool fused, bool counted = false, bool count = false) {
+// ev: if given, four events: before the sort, the density and the force pass, and after it
+int run_step_stages(sph_ctx* c, hipStream_t s, bool fused, bool counted = false, bool count = false,
+                    hipEvent_t* ev = nullptr) {
     int rc;
-    if ((rc = run_density_stages(c, s, fused, counted))) return rc;
+    if ((rc = run_density_stages(c, s, fused, counted, ev))) return rc;
+    if (ev && (rc = mark(ev[2], s))) return rc;
     if ((rc = hip_check(sph::launch_force(c, s, count), "force"))) return rc;
+    if (ev && (rc = mark(ev[3], s))) return rc;
     c->last_stage = SPH_STAGE_FORCE;
     return SPH_OK;
 }
@@ -366,10 +419,11 @@ int run_step_stages(sph_ctx* c, hipStream_t s, bool fused, bool counted = false,
 // scan; the first step of a call counts in its sort, so between calls the per-cell counts
 // stay zero and every other path (upload, staged steps, snapshots, the pullbacks) keeps
 // its own sort.
-int run_steps(sph_ctx* c, hipStream_t s, int32_t nsteps) {
+// ev: if given, four events per step (see run_step_stages).
+int run_steps(sph_ctx* c, hipStream_t s, int32_t nsteps, hipEvent_t* ev = nullptr) {
     const bool fuse = sph::count_in_force(c);
     for (int32_t k = 0; k < nsteps; ++k) {
-        if (int rc = run_step_stages(c, s, true, fuse && k > 0, fuse && k + 1 < nsteps)) {
+        if (int rc = run_step_stages(c, s, true, fuse && k > 0, fuse && k + 1 < nsteps, ev ? ev + 4 * k : nullptr)) {
             // counts left by a force pass whose sort did not run would be counted twice by
             // the next sort (and scatter past the end): clear them
             if (fuse && k > 0) (void)sph::launch_fill_u32(c->b.cell_cnt, 0u, c->sc.cs_cells, s);
@@ -601,6 +655,36 @@ int sph_step(sph_ctx* c, int32_t nsteps, void* stream) {
         if ((rc = hip_check(hipStreamIsCapturing((hipStream_t)stream, &st), "capture status"))) return rc;
         if (st == hipStreamCaptureStatusNone) return read_nonfinite(c, (hipStream_t)stream, nullptr);
     }
+    return SPH_OK;
+}
+
+int sph_step_timed(sph_ctx* c, int32_t nsteps, double* stage_ms, void* stream) {
+    int rc = ready(c, true);
+    if (rc) return rc;
+    if (!stage_ms) return fail(SPH_E_INVALID, "null stage_ms");
+    if (nsteps < 1 || nsteps > 100000) return fail(SPH_E_INVALID, "nsteps %d outside [1, 100000]", nsteps);
+    if (c->last_stage != 0 && c->last_stage != SPH_STAGE_FORCE)
+        return fail(SPH_E_STATE, "a staged step is in progress (run it to SPH_STAGE_FORCE first)");
+    hipStream_t s = (hipStream_t)stream;
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if ((rc = hip_check(hipStreamIsCapturing(s, &st), "capture status"))) return rc;
+    if (st != hipStreamCaptureStatusNone) return fail(SPH_E_STATE, "sph_step_timed waits for its events: not under capture");
+    std::vector<hipEvent_t> ev(4 * (size_t)nsteps, nullptr);
+    for (auto& e : ev)
+        if ((rc = hip_check(hipEventCreate(&e), "create event"))) break;
+    if (rc == SPH_OK) rc = run_steps(c, s, nsteps, ev.data());
+    if (rc == SPH_OK) rc = hip_check(hipEventSynchronize(ev.back()), "wait for the timed steps");
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int32_t k = 0; k < nsteps && rc == SPH_OK; ++k)
+        for (int j = 0; j < 3 && rc == SPH_OK; ++j) {
+            float ms = 0.0f;
+            rc = hip_check(hipEventElapsedTime(&ms, ev[4 * k + j], ev[4 * k + j + 1]), "stage time");
+            acc[j] += ms;
+        }
+    for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    for (int j = 0; j < 3; ++j) stage_ms[j] = acc[j] / nsteps;
     return SPH_OK;
 }
 

@@ -19,7 +19,13 @@
 //   order    one thread per position: the particle there moves to cs[key] + (number of
 //            particles of its cell with a smaller id), which is id order within the cell; the
 //            sorted key is written there and either the slot (read by the separate reorder
-//            pass of a staged step) or, in the fused step, the particle's gathered state
+//            pass of a staged step) or, in the fused step, the particle's gathered state.
+//            With cells cut into x slabs (StepConsts::slab_shift) the fused step orders a
+//            cell by (slab, id) instead, and the first particle of each cell stores the
+//            offsets of its slabs in the high bits of the cell's table entry; the scan has
+//            stored offset 0 there for the empty cells.  A staged step keeps the id order
+//            (the reference's), and its table carries no offsets for occupied cells, which
+//            the row walk then searches whole.
 // The within-cell pass costs one comparison per particle of the cell, less than the density
 // pass spends on the same cell.  The final order does not depend on the atomic order.
 // The per-slot passes (count, scatter, order) map workgroups to slot ranges XCD by XCD as
@@ -36,12 +42,32 @@ constexpr int CTILE = CT * CI;   // cells per scan tile
 constexpr int CWAVES = CT / 64;
 constexpr int kMergedTiles = 2048;  // up to this many tiles, each tile's workgroup sums the counts before it
 
+// The slab layout of the cell table (StepConsts::slab_shift .. start_mask).
+struct SlabTable {
+    int shift;
+    int start_bits;
+    int bits;
+    int count_pos;
+    int count_bits;
+    uint32_t start_mask;
+    // the fields of a cell with all its slab offsets 0 (an empty cell)
+    __host__ __device__ uint32_t empty() const {
+        uint32_t f = 0;
+        for (int j = 1; j < (1 << shift); ++j) f |= 1u << (start_bits + bits * (j - 1));
+        return f;
+    }
+};
+
+SlabTable slab_table(const StepConsts& S) {
+    return SlabTable{S.slab_shift, S.start_bits, S.slab_bits, S.count_pos, S.count_bits, S.start_mask};
+}
+
 // One thread per slot; the ranks of rank_in_cell.
 __global__ __launch_bounds__(CT) void cell_count_kernel(int n, const uint32_t* __restrict__ keyI,
                                                          uint32_t* __restrict__ cnt, uint32_t* __restrict__ rank) {
     const int i = xcd_tile(blockIdx.x, gridDim.x) * CT + threadIdx.x;
     const bool valid = i < n;
-    const uint32_t r = rank_in_cell(valid ? keyI[i] : 0xffffffffu, valid, cnt);
+    const uint32_t r = rank_in_cell(valid ? keyI[i] & kCellKeyMask : 0xffffffffu, valid, cnt);
     if (valid) rank[i] = r;
 }
 
@@ -133,7 +159,7 @@ __device__ __forceinline__ int cpad(int i) { return i + (i >> 5); }
 template <bool MERGED>
 __global__ __launch_bounds__(CT) void cell_scan_down(uint32_t* __restrict__ cnt, int m,
                                                       const uint32_t* __restrict__ part, uint32_t* __restrict__ cs,
-                                                      uint32_t n) {
+                                                      uint32_t n, uint32_t empty) {
     __shared__ uint32_t lw[CWAVES];
     __shared__ uint32_t tile[CTILE + CTILE / 32];
     const int64_t base = (int64_t)blockIdx.x * CTILE;
@@ -174,7 +200,7 @@ __global__ __launch_bounds__(CT) void cell_scan_down(uint32_t* __restrict__ cnt,
 #pragma unroll
     for (int k = 0; k < CI; ++k) {
         const uint32_t v = tile[cpad(j0 + k)];
-        tile[cpad(j0 + k)] = run;
+        tile[cpad(j0 + k)] = run | (v == 0u ? empty : 0u);
         run += v;
     }
     __syncthreads();
@@ -197,16 +223,17 @@ __global__ __launch_bounds__(CT) void cell_scan_down(uint32_t* __restrict__ cnt,
 
 __global__ __launch_bounds__(CT) void cell_scatter_kernel(int n, const uint32_t* __restrict__ keyI,
                                                            const uint32_t* __restrict__ rank,
-                                                           const uint32_t* __restrict__ cs, uint32_t* __restrict__ tmp) {
+                                                           const uint32_t* __restrict__ cs, uint32_t* __restrict__ tmp,
+                                                           uint32_t start_mask) {
     const int i = xcd_tile(blockIdx.x, gridDim.x) * CT + threadIdx.x;
-    if (i < n) tmp[cs[keyI[i]] + rank[i]] = (uint32_t)i;
+    if (i < n) tmp[(cs[keyI[i] & kCellKeyMask] & start_mask) + rank[i]] = (uint32_t)i;
 }
 
 template <bool GATHER>
-__global__ __launch_bounds__(CT) void cell_order_kernel(int n, const uint32_t* __restrict__ tmp,
-                                                         const uint32_t* __restrict__ keyI,
-                                                         const uint32_t* __restrict__ cs, uint32_t* __restrict__ skey,
-                                                         uint32_t* __restrict__ sval, const float4* __restrict__ posB,
+__global__ __launch_bounds__(CT) void cell_order_kernel(int n, SlabTable T, const uint32_t* __restrict__ tmp,
+                                                         const uint32_t* __restrict__ keyI, uint32_t* __restrict__ cs,
+                                                         uint32_t* __restrict__ skey, uint32_t* __restrict__ sval,
+                                                         const float4* __restrict__ posB,
                                                          const float4* __restrict__ velB,
                                                          const uint32_t* __restrict__ pidB, float4* __restrict__ posA,
                                                          float4* __restrict__ velA, uint32_t* __restrict__ pidA) {
@@ -214,11 +241,42 @@ __global__ __launch_bounds__(CT) void cell_order_kernel(int n, const uint32_t* _
     if (s >= n) return;
     const uint32_t slot = tmp[s];
     const uint32_t id = pidB[slot];
-    const uint32_t k = keyI[slot];
-    const uint32_t a = cs[k], b = cs[k + 1];
-    uint32_t r = 0;
-    for (uint32_t t = a; t < b; ++t) r += pidB[tmp[t]] < id ? 1u : 0u;
+    const uint32_t kf = keyI[slot];
+    const uint32_t k = kf & kCellKeyMask;
+    // the first particle of cell k + 1 may be storing its slab offsets into cs[k + 1]
+    const uint32_t a = cs[k] & T.start_mask, b = cs[k + 1] & T.start_mask;
+    // the fused step's order within a cell is by (slab, id): ids are below 2^27, so the slab
+    // bits of the key above the id order both
+    const bool slabs = GATHER && T.shift > 0;
+    const uint32_t own = slabs ? (kf & ~kCellKeyMask) | id : id;
+    uint32_t r = 0, b1 = 0, b2 = 0, b3 = 0;  // b_j: particles of the cell in slabs below j
+    for (uint32_t t = a; t < b; ++t) {
+        const uint32_t st = tmp[t];
+        if (slabs) {
+            const uint32_t kt = keyI[st];
+            const uint32_t q = kt >> kSlabKeyShift;
+            r += ((kt & ~kCellKeyMask) | pidB[st]) < own ? 1u : 0u;
+            b1 += q < 1u ? 1u : 0u;
+            b2 += q < 2u ? 1u : 0u;
+            b3 += q < 3u ? 1u : 0u;
+        } else {
+            r += pidB[st] < id ? 1u : 0u;
+        }
+    }
     const uint32_t f = a + r;
+    if (slabs && r == 0u) {
+        // offsets of slabs 1 .. S - 1, each stored + 1, none if the largest does not fit; the
+        // count + 1 where it fits
+        const bool four = T.shift == 2;
+        const uint32_t top = four ? b3 : b1;
+        uint32_t fields = 0u;
+        if (top + 1u <= (1u << T.bits) - 1u) {
+            fields = (b1 + 1u) << T.start_bits;
+            if (four) fields |= ((b2 + 1u) << (T.start_bits + T.bits)) | ((b3 + 1u) << (T.start_bits + 2 * T.bits));
+        }
+        if (T.count_bits > 0 && (b - a) + 1u <= (1u << T.count_bits) - 1u) fields |= ((b - a) + 1u) << T.count_pos;
+        if (fields) cs[k] = a | fields;
+    }
     skey[f] = k;
     if (!GATHER) sval[f] = slot;
     if (GATHER) {
@@ -242,12 +300,12 @@ constexpr int LWAVES = LT / 64;
 
 __global__ __launch_bounds__(LT) void cell_sort_lds_kernel(int n, int m, const uint32_t* __restrict__ keyI,
                                                            uint32_t* __restrict__ rank, uint32_t* __restrict__ cs,
-                                                           uint32_t* __restrict__ tmp) {
+                                                           uint32_t* __restrict__ tmp, SlabTable T) {
     __shared__ uint32_t cnt[kLdsSortCells + kLdsSortCells / 32];  // index c at cpad(c)
     __shared__ uint32_t lw[LWAVES];
     for (int c = threadIdx.x; c < m; c += LT) cnt[cpad(c)] = 0u;
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += LT) rank[i] = atomicAdd(&cnt[cpad((int)keyI[i])], 1u);
+    for (int i = threadIdx.x; i < n; i += LT) rank[i] = atomicAdd(&cnt[cpad((int)(keyI[i] & kCellKeyMask))], 1u);
     __syncthreads();
     // exclusive scan: thread t owns the cells [t per, (t + 1) per)
     const int per = (m + LT - 1) / LT;
@@ -260,15 +318,17 @@ __global__ __launch_bounds__(LT) void cell_sort_lds_kernel(int n, int m, const u
     __syncthreads();
     uint32_t run = incl - sum;
     for (int k = 0; k < w; ++k) run += lw[k];
+    const uint32_t empty = T.empty();
     for (int c = c0; c < c1; ++c) {
         const uint32_t v = cnt[cpad(c)];
-        cnt[cpad(c)] = run;
+        cnt[cpad(c)] = run | (v == 0u ? empty : 0u);
         run += v;
     }
     __syncthreads();
     for (int c = threadIdx.x; c < m; c += LT) cs[c] = cnt[cpad(c)];
     if (threadIdx.x == 0) cs[m] = (uint32_t)n;
-    for (int i = threadIdx.x; i < n; i += LT) tmp[cnt[cpad((int)keyI[i])] + rank[i]] = (uint32_t)i;
+    for (int i = threadIdx.x; i < n; i += LT)
+        tmp[(cnt[cpad((int)(keyI[i] & kCellKeyMask))] & T.start_mask) + rank[i]] = (uint32_t)i;
 }
 
 inline int cblocks(int64_t n) { return (int)((n + CT - 1) / CT); }
@@ -289,14 +349,15 @@ hipError_t launch_cell_sort(sph_ctx* c, bool gather, hipStream_t s, bool counted
     uint32_t* rank = b.keyS[0];
     uint32_t* tmp = b.valS[0];
     const int tiles = (int)cell_scan_tiles(m);
+    const SlabTable T = slab_table(c->sc);
     if (!cell_sort_global(c)) {
-        cell_sort_lds_kernel<<<1, LT, 0, s>>>(n, m, b.keyI, rank, b.cs, tmp);
+        cell_sort_lds_kernel<<<1, LT, 0, s>>>(n, m, b.keyI, rank, b.cs, tmp, T);
         if (n > 0) {
             if (gather)
-                cell_order_kernel<true><<<cblocks(n), CT, 0, s>>>(n, tmp, b.keyI, b.cs, b.keyS[1], b.valS[1], b.posB,
-                                                                   b.velB, b.pidB, b.posA, b.velA, b.pidA);
+                cell_order_kernel<true><<<cblocks(n), CT, 0, s>>>(n, T, tmp, b.keyI, b.cs, b.keyS[1], b.valS[1],
+                                                                   b.posB, b.velB, b.pidB, b.posA, b.velA, b.pidA);
             else
-                cell_order_kernel<false><<<cblocks(n), CT, 0, s>>>(n, tmp, b.keyI, b.cs, b.keyS[1], b.valS[1],
+                cell_order_kernel<false><<<cblocks(n), CT, 0, s>>>(n, T, tmp, b.keyI, b.cs, b.keyS[1], b.valS[1],
                                                                     nullptr, nullptr, b.pidB, nullptr, nullptr, nullptr);
         }
         c->sorted_keys = b.keyS[1];
@@ -306,18 +367,18 @@ hipError_t launch_cell_sort(sph_ctx* c, bool gather, hipStream_t s, bool counted
     if (n > 0 && !counted) cell_count_kernel<<<cblocks(n), CT, 0, s>>>(n, b.keyI, b.cell_cnt, rank);
     cell_scan_reduce<<<tiles, CT, 0, s>>>(b.cell_cnt, m, b.cell_part);
     if (tiles <= kMergedTiles) {
-        cell_scan_down<true><<<tiles, CT, 0, s>>>(b.cell_cnt, m, b.cell_part, b.cs, (uint32_t)n);
+        cell_scan_down<true><<<tiles, CT, 0, s>>>(b.cell_cnt, m, b.cell_part, b.cs, (uint32_t)n, T.empty());
     } else {
         cell_scan_partials<<<1, CT, 0, s>>>(b.cell_part, tiles);
-        cell_scan_down<false><<<tiles, CT, 0, s>>>(b.cell_cnt, m, b.cell_part, b.cs, (uint32_t)n);
+        cell_scan_down<false><<<tiles, CT, 0, s>>>(b.cell_cnt, m, b.cell_part, b.cs, (uint32_t)n, T.empty());
     }
     if (n > 0) {
-        cell_scatter_kernel<<<cblocks(n), CT, 0, s>>>(n, b.keyI, rank, b.cs, tmp);
+        cell_scatter_kernel<<<cblocks(n), CT, 0, s>>>(n, b.keyI, rank, b.cs, tmp, T.start_mask);
         if (gather)
-            cell_order_kernel<true><<<cblocks(n), CT, 0, s>>>(n, tmp, b.keyI, b.cs, b.keyS[1], b.valS[1], b.posB,
+            cell_order_kernel<true><<<cblocks(n), CT, 0, s>>>(n, T, tmp, b.keyI, b.cs, b.keyS[1], b.valS[1], b.posB,
                                                                b.velB, b.pidB, b.posA, b.velA, b.pidA);
         else
-            cell_order_kernel<false><<<cblocks(n), CT, 0, s>>>(n, tmp, b.keyI, b.cs, b.keyS[1], b.valS[1], nullptr,
+            cell_order_kernel<false><<<cblocks(n), CT, 0, s>>>(n, T, tmp, b.keyI, b.cs, b.keyS[1], b.valS[1], nullptr,
                                                                 nullptr, b.pidB, nullptr, nullptr, nullptr);
     }
     c->sorted_keys = b.keyS[1];

@@ -81,8 +81,25 @@ __device__ __forceinline__ uint32_t sort_key(float x, float y, float z, const St
         uint32_t key = vx + vy * (uint32_t)C.dims[0] + vz * (uint32_t)C.dims[0] * (uint32_t)C.dims[1];
         return key ^ 0x80000000u;  // signed order as unsigned
     }
-    return (uint32_t)(C.cell_off + grid_coord(x, 0, C) +
-                      C.dims[0] * (grid_coord(y, 1, C) + C.dims[1] * grid_coord(z, 2, C)));
+    const int cx = grid_coord(x, 0, C);
+    uint32_t key = (uint32_t)(C.cell_off + cx + C.dims[0] * (grid_coord(y, 1, C) + C.dims[1] * grid_coord(z, 2, C)));
+    if (C.slab_shift > 0) {
+        // the x slab within the cell: floor(S x / edge) - S cx, clamped to the cell's slabs
+        // (the edge cells' outer slabs extend to infinity, NaN -> 0); exact for power-of-two
+        // edges, and monotone in x otherwise
+        const float S = (float)(1 << C.slab_shift);
+        const float q = floorf(x * C.inv_cell[0] * S) - S * (float)cx;
+        key |= (uint32_t)fminf(fmaxf(q, 0.0f), S - 1.0f) << kSlabKeyShift;
+    }
+    return key;
+}
+
+// The start of a cell held by a cell-table entry, and the offset + 1 of slab k (1 .. S - 1)
+// from it (0 for k = 0, and 0 when not known); see StepConsts::slab_shift.
+__device__ __forceinline__ uint32_t table_start(uint32_t w, const StepConsts& C) { return w & C.start_mask; }
+__device__ __forceinline__ uint32_t slab_field(uint32_t w, uint32_t k, const StepConsts& C) {
+    return __builtin_amdgcn_ubfe(w, (uint32_t)(C.start_bits + C.slab_bits * ((int)k - 1)),
+                                 k ? (uint32_t)C.slab_bits : 0u);
 }
 
 // bucket of a sort key in the cell table: -1 below the grid, n_cells above it
@@ -161,29 +178,31 @@ __device__ __forceinline__ bool outside_voxel_grid(float x, float y, float z, co
 // ---------------------------------------------------------------------------------------
 // The rows of cells that the sphere of radius h_search around a particle reaches, in
 // increasing cell order (cz, then cy): a row is the run of cells along x that the sphere's
-// chord through the nearest point of the row's (y, z) slab covers, named by the cell-table
-// indices of its first cell and of the cell after its last (its candidates are cs[c0] ..
-// cs[c1] - 1).  The position is clamped into the grid first; an edge cell holds the
-// particles beyond it, so a particle outside the grid gets a superset of its rows.  The
-// margin of h_search over h covers the rounding of this float arithmetic (a cell it adds or
-// drops holds only particles beyond h).
+// chord through the nearest point of the row's (y, z) slab covers, taken in x slabs of the
+// cells (StepConsts::slab_shift; whole cells when the cells are not cut), so that its
+// candidates are one contiguous range of the cell order.  The position is clamped into the
+// grid first; an edge cell holds the particles beyond it, so a particle outside the grid
+// gets a superset of its rows.  The margin of h_search over h covers the rounding of this
+// float arithmetic (a slab it adds or drops holds only particles beyond h).
 struct RowGrid {
-    float gx, gy, gz, fx, hs2, mx;
+    float gx, gy, gz, fx, hs2, mx, sinv;  // fx, mx: x in slabs; sinv: slabs per unit length
     int ylo, yhi, zlo, zhi;
 
     __device__ __forceinline__ RowGrid(float px, float py, float pz, const StepConsts& C) {
         const float hs = C.h_search;
         hs2 = hs * hs;
-        mx = (float)(C.dims[0] - 1);
+        const float S = (float)(1 << C.slab_shift);
         const float my = (float)(C.dims[1] - 1), mz = (float)(C.dims[2] - 1);
-        gx = fminf(fmaxf(px, 0.0f), (mx + 1.0f) * C.cell[0]);  // NaN -> 0
+        gx = fminf(fmaxf(px, 0.0f), (float)C.dims[0] * C.cell[0]);  // NaN -> 0
         gy = fminf(fmaxf(py, 0.0f), (my + 1.0f) * C.cell[1]);
         gz = fminf(fmaxf(pz, 0.0f), (mz + 1.0f) * C.cell[2]);
         ylo = (int)__builtin_amdgcn_fmed3f((gy - hs) * C.inv_cell[1], 0.0f, my);
         yhi = (int)__builtin_amdgcn_fmed3f((gy + hs) * C.inv_cell[1], 0.0f, my);
         zlo = (int)__builtin_amdgcn_fmed3f((gz - hs) * C.inv_cell[2], 0.0f, mz);
         zhi = (int)__builtin_amdgcn_fmed3f((gz + hs) * C.inv_cell[2], 0.0f, mz);
-        fx = gx * C.inv_cell[0];
+        sinv = C.inv_cell[0] * S;
+        fx = gx * sinv;
+        mx = (float)C.dims[0] * S - 1.0f;
     }
 
     // distance from the particle to the slab [c s, (c + 1) s), zero inside it
@@ -198,29 +217,53 @@ struct RowGrid {
         return fmaf(-ez, ez, hs2);
     }
 
-    // the row at cy of the slab of cz (row0: the table index of that row's cell at x = 0,
-    // rz: rest_z(cz)); false if the sphere misses the row's slab
-    __device__ __forceinline__ bool row(int cy, int row0, float rz, const StepConsts& C, uint32_t& c0,
-                                        uint32_t& c1) const {
+    // the row at cy of the slab of cz (rz: rest_z(cz)): its first x slab q0 and one past its
+    // last, q1, counted from the row's first cell; false if the sphere misses the row's slab
+    __device__ __forceinline__ bool row(int cy, float rz, const StepConsts& C, uint32_t& q0, uint32_t& q1) const {
         const float ey = slab(gy, cy, C.cell[1]);
         const float rem = fmaf(-ey, ey, rz);
         if (!(rem >= 0.0f)) return false;
-        const float ex = __builtin_amdgcn_sqrtf(rem) * C.inv_cell[0];
-        c0 = (uint32_t)(row0 + (int)__builtin_amdgcn_fmed3f(fx - ex, 0.0f, mx));
-        c1 = (uint32_t)(row0 + (int)__builtin_amdgcn_fmed3f(fx + ex, 0.0f, mx) + 1);
+        const float ex = __builtin_amdgcn_sqrtf(rem) * sinv;
+        q0 = (uint32_t)(int)__builtin_amdgcn_fmed3f(fx - ex, 0.0f, mx);
+        q1 = (uint32_t)(int)__builtin_amdgcn_fmed3f(fx + ex, 0.0f, mx) + 1u;
         return true;
+    }
+
+    // the candidates [k0, k1) of slabs q0 .. q1 - 1 of the row whose first cell is row0, from
+    // the cell-table entries of the cell holding q0 and of the cell holding q1 (the next
+    // cell's start when q1 is a cell's first slab), ld(c) reading entry c
+    template <typename LD>
+    __device__ __forceinline__ static void range(int row0, uint32_t q0, uint32_t q1, const StepConsts& C, LD&& ld,
+                                                 uint32_t& k0, uint32_t& k1) {
+        const int sh = C.slab_shift;
+        const uint32_t m = (1u << sh) - 1u;
+        const int e = row0 + (int)(q1 >> sh);
+        const uint32_t a = ld(row0 + (int)(q0 >> sh)), b = ld(e);
+        const uint32_t fa = slab_field(a, q0 & m, C), fb = slab_field(b, q1 & m, C);
+        k0 = table_start(a, C) + (fa ? fa - 1u : 0u);
+        k1 = table_start(b, C) + (fb ? fb - 1u : 0u);
+        // the end lies inside a cell whose slab offsets are not known: the whole cell, from its
+        // count if the entry keeps it, else from the next cell's start
+        if ((q1 & m) != 0u && fb == 0u) {
+            const uint32_t nb = C.count_bits ? __builtin_amdgcn_ubfe(b, (uint32_t)C.count_pos, (uint32_t)C.count_bits) : 0u;
+            k1 = nb ? table_start(b, C) + nb - 1u : table_start(ld(e + 1), C);
+        }
     }
 };
 
-template <typename F>
-__device__ __forceinline__ void for_each_row(float px, float py, float pz, const StepConsts& C, F&& f) {
+template <typename LD, typename F>
+__device__ __forceinline__ void for_each_row(float px, float py, float pz, const StepConsts& C, LD&& ld, F&& f) {
     const RowGrid R(px, py, pz, C);
     for (int cz = R.zlo; cz <= R.zhi; ++cz) {
         const float rz = R.rest_z(cz, C);
         int row0 = C.cell_off + C.dims[0] * (R.ylo + C.dims[1] * cz);
         for (int cy = R.ylo; cy <= R.yhi; ++cy, row0 += C.dims[0]) {
-            uint32_t c0, c1;
-            if (R.row(cy, row0, rz, C, c0, c1)) f(c0, c1);
+            uint32_t q0, q1;
+            if (R.row(cy, rz, C, q0, q1)) {
+                uint32_t k0, k1;
+                RowGrid::range(row0, q0, q1, C, ld, k0, k1);
+                f(k0, k1);
+            }
         }
     }
 }
@@ -246,11 +289,9 @@ __device__ __forceinline__ void for_each_row_range(float px, float py, float pz,
             const int q = (int)(((float)t + 0.5f) * inv_ny);  // exact: t and ny are small
             const int cz = R.zlo + q, cy = R.ylo + (t - q * ny);
             const int row0 = C.cell_off + C.dims[0] * (cy + C.dims[1] * cz);
-            uint32_t c0, c1;
-            if (R.row(cy, row0, R.rest_z(cz, C), C, c0, c1)) {
-                a0 = ld_u32(rcs, c0 * 4u);
-                a1 = ld_u32(rcs, c1 * 4u);
-            }
+            uint32_t q0, q1;
+            if (R.row(cy, R.rest_z(cz, C), C, q0, q1))
+                RowGrid::range(row0, q0, q1, C, [&](int c) { return ld_u32(rcs, (uint32_t)c * 4u); }, a0, a1);
         }
         const int last = min(L, npairs - t0);
         for (int j = 0; j < last; ++j) {
@@ -263,9 +304,8 @@ __device__ __forceinline__ void for_each_row_range(float px, float py, float pz,
 template <typename F>
 __device__ __forceinline__ void for_each_candidate(float px, float py, float pz, const StepConsts& C,
                                                    const uint32_t* __restrict__ cs, F&& f) {
-    for_each_row(px, py, pz, C, [&](uint32_t c0, uint32_t c1) {
-        const uint32_t k1 = cs[c1];
-        for (uint32_t k = cs[c0]; k < k1; ++k) f(k);
+    for_each_row(px, py, pz, C, [&](int c) { return cs[c]; }, [&](uint32_t k0, uint32_t k1) {
+        for (uint32_t k = k0; k < k1; ++k) f(k);
     });
 }
 

@@ -11,6 +11,8 @@
 //                         (valI unused); VOXEL: key and posB slot per particle id
 //   keyS/valS  u32[2][N]  radix ping-pong buffers
 //   cs         u32[C+1]   cell start table: cs[c] = first cell-sorted index with key >= c
+//                         (NAIVE: in the low start_bits bits, with the x-slab offsets of the
+//                         cell above them, see StepConsts::slab_shift)
 //   cell_cnt   u32[C]     NAIVE: per-cell counts of the counting sort (kept zero between steps)
 //   rng        u64[2N]    xoroshiro128+ stream per particle id (PIPE mode)
 //   cotx/cotv  float4[N]  reverse mode only: cotangent of the state in particle order
@@ -18,9 +20,10 @@
 //                         force pass: each entry is the neighbour's index minus the particle's
 //                         own.  Slots t and t + 1 (t even) of particle i form the 32-bit word
 //                         (t / 2) * N + i, so consecutive particles' words are contiguous.
-//                         The density pass appends each lane's first 32 entries to a
-//                         lane-private list in LDS and copies it out slot by slot at the end
-//                         (every store coalesced); hits beyond 32 go straight to HBM.  nn > 96 marks a
+//                         The density pass appends a particle's first 32 L entries (L lanes
+//                         per particle, at most 96) to its place in its wavefront's LDS band
+//                         and copies them out slot by slot at the end (every store coalesced);
+//                         hits beyond that go straight to HBM.  nn > 96 marks a
 //                         particle whose neighbours the force pass rescans (more than 96, or
 //                         one farther than an int16 offset).
 #pragma once
@@ -101,7 +104,25 @@ struct StepConsts {
     int lanes;
     int lane_shift;
     int tfirst;
+    // NAIVE grid keys: each cell is cut into S = 1 << slab_shift slabs along x (1: not cut).
+    // A key carries its particle's slab in bits kSlabKeyShift.. (the cell below them); the
+    // fused step's sort orders the particles of a cell by (slab, id), and the entry of an
+    // occupied cell in the cell table then carries, above its start_bits low bits, the
+    // offsets of the cell's slabs 1 .. S - 1 from its start, each as offset + 1 in a field of
+    // slab_bits bits (0: not known, the cell is searched whole); an empty cell's fields hold
+    // offset 0.  Where bits are left above them (count_bits from bit count_pos), the cell's
+    // particle count + 1 (0: not kept), which gives the next cell's start without reading its
+    // entry.  The same values in the totals and in every instance entry.
+    int slab_shift;
+    int start_bits;
+    int slab_bits;
+    int count_pos;
+    int count_bits;
+    uint32_t start_mask;  // the low start_bits bits
 };
+
+constexpr int kSlabKeyShift = 30;
+constexpr uint32_t kCellKeyMask = (1u << kSlabKeyShift) - 1u;
 
 struct PipeTable {
     float row[kMaxPipeRows][5];   // Pipe.to_numpy() rows

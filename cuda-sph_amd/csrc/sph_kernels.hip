@@ -244,8 +244,7 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
         }
     };
     if constexpr (L == 1) {
-        for_each_row(p.x, p.y, p.z, C,
-                     [&](uint32_t c0, uint32_t c1) { run_row(ld_u32(rcs, c0 * 4u), ld_u32(rcs, c1 * 4u)); });
+        for_each_row(p.x, p.y, p.z, C, [&](int c) { return ld_u32(rcs, (uint32_t)c * 4u); }, run_row);
     } else {
         // the group shares the row walk; rows without candidates still bound klo, khi
         for_each_row_range<L>(p.x, p.y, p.z, C, rcs, l, run_row);
@@ -374,7 +373,10 @@ __global__ __launch_bounds__(256) void naive_force_kernel(
     // wavefront's particles past n left at the top, so the lanes here are a prefix of it and
     // runs of equal keys share an atomic; with L lanes only the groups' first lanes are
     // here, at most 64 / L of them, and each counts its own particle.
-    if constexpr (COUNT) rank[i] = L == 1 ? rank_in_cell(key, true, cnt) : atomicAdd(&cnt[key], 1u);
+    if constexpr (COUNT) {
+        const uint32_t cell = key & kCellKeyMask;
+        rank[i] = L == 1 ? rank_in_cell(cell, true, cnt) : atomicAdd(&cnt[cell], 1u);
+    }
 }
 
 // Batched form of the force pass (box walls): the thread map of naive_density_batch_kernel.
@@ -620,7 +622,7 @@ __global__ __launch_bounds__(256) void read_buffer_kernel(StepConsts C, int whic
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (which == SPH_BUF_CELL_BEGIN) {
         if (i >= C.n_cells) return;
-        const uint32_t s = cs[i], e = cs[i + 1];
+        const uint32_t s = table_start(cs[i], C), e = table_start(cs[i + 1], C);
         ((int32_t*)out)[i] = s < e ? (int32_t)s : -1;
         return;
     }
@@ -629,7 +631,7 @@ __global__ __launch_bounds__(256) void read_buffer_kernel(StepConsts C, int whic
         case SPH_BUF_KEYS: {  // NAIVE: keys in posB slot order (see hash_particle)
             const uint32_t u = keyI[i];
             const int64_t o = C.formulation == SPH_FORM_NAIVE ? (int64_t)pidB[i] : i;
-            ((int32_t*)out)[o] = C.ref_keys ? (int32_t)(u ^ 0x80000000u) : (int32_t)u;
+            ((int32_t*)out)[o] = C.ref_keys ? (int32_t)(u ^ 0x80000000u) : (int32_t)(u & kCellKeyMask);
             break;
         }
         case SPH_BUF_PERM: {
@@ -638,7 +640,7 @@ __global__ __launch_bounds__(256) void read_buffer_kernel(StepConsts C, int whic
         }
         case SPH_BUF_SORTED_KEYS: {
             const uint32_t u = skeys[i];
-            ((int32_t*)out)[i] = C.ref_keys ? (int32_t)(u ^ 0x80000000u) : (int32_t)u;
+            ((int32_t*)out)[i] = C.ref_keys ? (int32_t)(u ^ 0x80000000u) : (int32_t)(u & kCellKeyMask);
             break;
         }
         case SPH_BUF_DENSITY:

@@ -63,7 +63,7 @@ EXPORTED = [
     "sph_run_stage", "sph_buffer_elems", "sph_read_buffer", "sph_diagnostics", "sph_grid_info", "sph_set_launch",
     "sph_create_batch", "sph_batch_info", "sph_set_constants", "sph_set_lanes", "sph_adjoint_seed", "sph_adjoint_step",
     "sph_adjoint_read", "sph_set_batch_lanes", "sph_adjoint_snapshot_bytes", "sph_adjoint_save", "sph_step_save", "sph_adjoint_step_from",
-    "sph_check_finite", "sph_noalloc_scratch_bytes", "sph_step_noalloc",
+    "sph_check_finite", "sph_noalloc_scratch_bytes", "sph_step_noalloc", "sph_step_timed",
 ]
 
 _lib = None
@@ -89,6 +89,7 @@ def load_library() -> ctypes.CDLL:
         "sph_upload_state": ([vp, vp, vp, vp], i32),
         "sph_upload_state_f32": ([vp] + [vp] * 6 + [vp], i32),
         "sph_step": ([vp, i32, vp], i32),
+        "sph_step_timed": ([vp, i32, POINTER(ctypes.c_double), vp], i32),
         "sph_download_state": ([vp, vp, vp, vp, vp, vp], i32),
         "sph_download_state_f32": ([vp] + [vp] * 7 + [vp], i32),
         "sph_run_stage": ([vp, i32, vp], i32),
@@ -217,6 +218,14 @@ class NativeContext:
     def step(self, nsteps: int = 1):
         self.generation += 1
         check(self.lib.sph_step(self.handle, int(nsteps), _stream(self.device)), "sph_step")
+
+    def step_timed(self, nsteps: int) -> dict:
+        """step(nsteps) with HIP events between the stages (waits for them): mean ms per step
+        of the sort (with the gather into cell order), the density pass and the force pass."""
+        self.generation += 1
+        out = (ctypes.c_double * 3)()
+        check(self.lib.sph_step_timed(self.handle, int(nsteps), out, _stream(self.device)), "sph_step_timed")
+        return {"sort": out[0], "density": out[1], "force": out[2]}
 
     def run_stage(self, stage: int):
         self.generation += 1

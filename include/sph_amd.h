@@ -137,6 +137,13 @@ int sph_upload_state_f32(sph_ctx* ctx, const float* dev_x, const float* dev_y, c
  * collision (base_kernels.py:56-98).  Returns SPH_E_STATE before the first upload. */
 int sph_step(sph_ctx* ctx, int32_t nsteps, void* stream);
 
+/* sph_step(ctx, nsteps) with HIP events recorded on the stream between its stages; waits
+ * for them and writes to stage_ms[0..2] the mean milliseconds per step of the sort (with
+ * the gather into cell order and the cell table), the density pass and the force pass
+ * (1 <= nsteps <= 100000; not on a stream under graph capture).  No reference
+ * counterpart: it is for measurement, the kernels and results are sph_step's. */
+int sph_step_timed(sph_ctx* ctx, int32_t nsteps, double* stage_ms, void* stream);
+
 /* Waits for the stream and reads the number of particle positions found NaN or infinite
  * when hashed since the last read (every step hashes every particle once, as does an upload),
  * then clears it.  *count (may be NULL) receives the number; returns SPH_E_NONFINITE if it is
