@@ -648,12 +648,15 @@ def main() -> int:
         traffic = json.load(open(traffic_file)).get(args.workload, {})
     valu_ops = {"density": n * cand * DENSITY_OPS_PER_CANDIDATE, "force": n * pairs * FLOPS_PER_PAIR}
 
+    # the timed force passes count the next sort's keys (all but the last of the call)
+    traffic_key = {"naive_density_kernel": "naive_density_kernel", "naive_force_kernel": "naive_force_kernel_counting"}
+
     def kernel_roofline(stage, name):
         ms = st[stage]
         achieved = per_particle[stage] * n / (ms / 1e3) / 1e9
         return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": (traffic.get(name) or {}).get("bytes_per_launch"),
+                "traffic": (traffic.get(traffic_key[name]) or {}).get("bytes_per_launch"),
                 "bytes_per_particle": per_particle[stage], "kernel_ms": round(ms, 4),
                 "valu": {"achieved_tflops": round(valu_ops[stage] / (ms / 1e3) / 1e12, 2),
                          "peak_tflops": FP32_VECTOR_PEAK_TFLOPS, "candidates_per_particle": round(cand, 1),
