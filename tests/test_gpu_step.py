@@ -267,11 +267,13 @@ def test_box_collisions_all_walls(gpu):
     assert out.velocity[0, 0] > 0 and out.velocity[1, 0] < 0 and abs(out.velocity[0, 0]) < 2
 
 
-@pytest.mark.parametrize("grid", [None, 2.0 ** -4])
+@pytest.mark.parametrize("grid", [None, 2.0 ** -4, 0.0391])
 @pytest.mark.parametrize("n", [0, 1, 2, 63, 1000, 2049, 8192, 8193])
 def test_small_and_ragged_sizes(gpu, n, grid):
     """Empty, single and ragged sizes; with the 2^-4 grid (4,096 cells) up to 8,192 particles
-    go through the one-workgroup sort and 8,193 through the multi-kernel one."""
+    go through the one-workgroup sort and 8,193 through the multi-kernel one.  A cell edge
+    that is not a power of two (0.0391) forms the keys in FP64 and the x slabs of the cells
+    from the FP32 quotient."""
     w = box_uniform(max(n, 1), seed=16)
     params = SimulationParameters(n, w.params.external_force, 1, 1000, w.params.pipe, w.params.space_size,
                                   w.params.voxel_size)
