@@ -141,12 +141,14 @@ def make_workload(name):
             "pipe_512k": workloads.pipe_flow}[name]()
 
 
-def neighbour_stats(w, sample=2048, seed=0):
+def neighbour_stats(w, sample=2048, seed=0, slabs=4, field_max=6):
     """Per particle, from a sample (host): candidates the density pass tests, cell rows it
     walks and list entries the force pass walks (pairs within h, the particle included).
-    The enumeration is the kernel's: for each row of cells (y, z) whose slab comes within
-    h, the x cells overlapping [x - e, x + e], e = sqrt(h^2 - dy^2 - dz^2), boundary cells
-    extending to infinity."""
+    The enumeration is the fused step's: for each row of cells (y, z) whose slab comes within
+    h, the x slabs (`slabs` per cell) overlapping [x - e, x + e], e = sqrt(h^2 - dy^2 - dz^2),
+    boundary cells extending to infinity; a cell at either end of the row whose slab offsets
+    the table does not keep (more than field_max particles below its last slab) is searched
+    whole."""
     import math
 
     import numpy as np
@@ -158,11 +160,18 @@ def neighbour_stats(w, sample=2048, seed=0):
     dims = np.maximum(np.ceil(np.asarray(w.params.space_size) / v).astype(np.int64), 1)
     cell = np.clip(np.floor(pos / v), 0, dims - 1).astype(np.int64)
     key = cell[:, 0] + dims[0] * (cell[:, 1] + dims[1] * cell[:, 2])
-    order = np.argsort(key, kind="stable")
-    skey, spos = key[order], pos[order]
+    sub = np.clip(np.floor(pos[:, 0] / v[0] * slabs) - slabs * cell[:, 0], 0, slabs - 1).astype(np.int64)
+    gkey = key * slabs + sub  # the order of the fused step's sort, as one global slab index
+    order = np.argsort(gkey, kind="stable")
+    sg, spos = gkey[order], pos[order]
+    below_last = np.bincount(key[sub < slabs - 1], minlength=int(np.prod(dims)))
+    whole = below_last > field_max  # cells whose slab offsets are not kept
 
     def gc(x, d):
         return int(min(max(math.floor(x / v[d]), 0), dims[d] - 1))
+
+    def gs(x):  # global x slab index, clamped to the row
+        return int(min(max(math.floor(x / v[0] * slabs), 0), dims[0] * slabs - 1))
 
     def slab(p, c, d):
         lo = -math.inf if c == 0 else c * v[d]
@@ -183,7 +192,11 @@ def neighbour_stats(w, sample=2048, seed=0):
                     continue
                 e = math.sqrt(rem)
                 row = dims[0] * (cy + dims[1] * cz)
-                a, b = np.searchsorted(skey, [row + gc(px - e, 0), row + gc(px + e, 0) + 1])
+                q0, q1 = gs(px - e), gs(px + e) + 1
+                c0, c1 = row + q0 // slabs, row + (q1 - 1) // slabs
+                lo = c0 * slabs if whole[c0] else row * slabs + q0
+                hi = (c1 + 1) * slabs if whole[c1] else row * slabs + q1
+                a, b = np.searchsorted(sg, [lo, hi])
                 cand += b - a
                 rows += 1
                 pairs += int((((spos[a:b] - pos[i]) ** 2).sum(-1) <= h * h).sum())
