@@ -1,9 +1,10 @@
 // sph_cellsort.hip -- counting sort of the NAIVE formulation's cell keys for gfx950.
 //
-// The NAIVE path's keys are cell indices in [0, C) (clamped to the grid), so the order the
-// step needs -- by (cell, particle id), the order of the reference's stable sort of the voxel
-// map (voxel_sph_strategy.py:81-89) -- is a counting sort by cell followed by id order within
-// each cell.  The keys arrive in the state's current slot order (the previous step's cell
+// The NAIVE path's keys are cell indices in [0, C) (clamped to the grid; bits 30-31 carry the
+// particle's x slab in its cell), so the order the step needs -- by (cell, particle id), the
+// order of the reference's stable sort of the voxel map (voxel_sph_strategy.py:81-89), or by
+// (cell, slab, id) in the fused step -- is a counting sort by cell followed by that order
+// within each cell.  The keys arrive in the state's current slot order (the previous step's cell
 // order, see hash_particle), so neighbouring threads mostly hit the same or neighbouring
 // cells.  Per step:
 //   count    one thread per slot: rank = atomicAdd(&cnt[key], 1) (an arbitrary order among
