@@ -104,9 +104,10 @@ def parse() -> argparse.Namespace:
     return ap.parse_args()
 
 
-def tune_workload(w, mode="BOX", scales=(2.0, 1.0, 0.5), enabled=True, steps=None):
+def tune_workload(w, mode="BOX", scales=(2.0, 1.0, 0.5), enabled=True, steps=None, choose_slabs=True):
     """Autotuned (params with the chosen grid, (density, force) block sizes, lanes per
-    particle, summary) for a workload; the untuned defaults when disabled.  The grid edge is
+    particle, summary with the slab choice) for a workload; the untuned defaults when
+    disabled.  The grid edge is
     tried at 2, 1 and 1/2 times the largest power of two <= h (the coarse grid suits small
     problems with many lanes per particle: fewer cells to sort, fewer rows).  Candidates are
     timed over the first min(100, run length) steps and the best of each of the three best
@@ -123,11 +124,11 @@ def tune_workload(w, mode="BOX", scales=(2.0, 1.0, 0.5), enabled=True, steps=Non
 
     steps = steps or min(100, int(w.steps))
     res = tune_strategy(w.params, w.constants, w.state, mode=mode, scales=scales, steps=steps,
-                        final_steps=int(w.steps), log=log)
+                        final_steps=int(w.steps), log=log, choose_slabs=choose_slabs)
     gv = res.grid_voxel(w.constants.h)
     params = w.params if gv is None else dataclasses.replace(w.params, voxel_size=np.asarray([gv] * 3))
     info = {"cell_scale": res.cell_scale, "grid_voxel": gv, "density_block": res.density_block,
-            "force_block": res.force_block, "lanes": res.lanes, "trials": len(res.trials),
+            "force_block": res.force_block, "lanes": res.lanes, "slabs": res.slabs, "trials": len(res.trials),
             "seconds": round(res.seconds, 2),
             "candidate_ms_per_step": [min(t[4] for t in res.trials), max(t[4] for t in res.trials)]}
     log(f"tuned {w.name}: {info}")
@@ -289,7 +290,8 @@ def bench_small(warmup: int = 3, tune: bool = True):
     w = workloads.box_4k()
     n = w.params.particle_count
     params, launch, lanes, tuning = tune_workload(w, enabled=tune)
-    strat = NaiveSPHStrategy(params, constants=w.constants, launch=launch, lanes=lanes, check_finite=False)
+    strat = NaiveSPHStrategy(params, constants=w.constants, launch=launch, lanes=lanes, check_finite=False,
+                             slabs=tuning["slabs"] if tuning else True)
     pos = torch.from_numpy(np.ascontiguousarray(w.state.position)).cuda()
     vel = torch.from_numpy(np.ascontiguousarray(w.state.velocity)).cuda()
     strat.ctx.upload(pos, vel)
@@ -321,7 +323,7 @@ def bench_pipe(steps: int = 1000, warmup: int = 5, tune: bool = True, graph: boo
     n = w.params.particle_count
     params, launch, lanes, tuning = tune_workload(w, "PIPE", enabled=tune)
     strat = NaiveSPHStrategy(params, constants=w.constants, mode="PIPE", launch=launch, lanes=lanes,
-                             check_finite=False)
+                             check_finite=False, slabs=tuning["slabs"] if tuning else True)
     pos = torch.from_numpy(np.ascontiguousarray(w.state.position)).cuda()
     vel = torch.from_numpy(np.ascontiguousarray(w.state.velocity)).cuda()
     run = timed_steps(strat, steps, graph, pos, vel)
@@ -471,19 +473,19 @@ def bench_batched(steps: int, warmup: int = 3, tune: bool = True):
     per_instance = []
     plain_launches = None
     for b, w in enumerate(bw.instances):
-        configs = [(None, None, 1)]
+        configs = [(None, None, 1, True)]
         if tune:
             from sph_amd.autotune import TuneCache, tune_strategy
 
             own = tune_strategy(w.params, w.constants, w.state, cache=TuneCache())
-            configs += [(scale[b], launch, lanes[b]), (own.cell_scale, own.launch, own.lanes)]
+            configs += [(scale[b], launch, lanes[b], True), (own.cell_scale, own.launch, own.lanes, own.slabs)]
         pos = torch.from_numpy(np.ascontiguousarray(w.state.position)).cuda()
         vel = torch.from_numpy(np.ascontiguousarray(w.state.velocity)).cuda()
         best = None
-        for sc, la, ln in dict.fromkeys(configs):
+        for sc, la, ln, sl in dict.fromkeys(configs):
             gv = None if sc is None else workloads.voxel_for(w.constants.h) * sc
             strat = NaiveSPHStrategy(w.params, constants=w.constants, grid_voxel=gv, launch=la, lanes=ln,
-                                             check_finite=False)
+                                     check_finite=False, slabs=sl)
             if plain_launches is None:
                 strat.ctx.upload(pos, vel)
                 plain_launches = launches_per_step(strat.ctx)
@@ -528,7 +530,8 @@ def bench_adjoint(steps: int = 200, every: int = 20, tune: bool = True):
 
     w = workloads.adjoint_16k()
     n = w.params.particle_count
-    _, launch, lanes, tuning = tune_workload(w, scales=(1.0,), enabled=tune)  # the grid stays the workload's
+    # the grid stays the workload's; an adjoint context keeps its cells cut into slabs
+    _, launch, lanes, tuning = tune_workload(w, scales=(1.0,), enabled=tune, choose_slabs=False)
     sim = DifferentiableSPH(w.params, w.constants, steps=steps, checkpoint_every=every, launch=launch, lanes=lanes)
     x0 = torch.from_numpy(w.state.position.astype(np.float32)).cuda()
     v0 = torch.from_numpy(w.state.velocity.astype(np.float32)).cuda()
@@ -607,7 +610,7 @@ def main() -> int:
     if args.block:
         launch = tuple(args.block)
     strat = NaiveSPHStrategy(w.params, constants=w.constants, mode=w.mode, device=dev, launch=launch, lanes=lanes,
-                             check_finite=False)
+                             check_finite=False, slabs=tuning["slabs"] if tuning else True)
     pos = torch.from_numpy(np.ascontiguousarray(w.state.position)).to(dev)
     vel = torch.from_numpy(np.ascontiguousarray(w.state.velocity)).to(dev)
     graph = args.graph == "on"

@@ -227,7 +227,7 @@ void plan_slabs(sph_ctx* c) {
     int shift = 0, start_bits = 32, field = 0;
     if (c->sc.formulation == SPH_FORM_NAIVE) {
         const int need = bit_length((uint64_t)(c->n > 0 ? c->n : 1));
-        for (int sh = SPH_SLAB_SHIFT; sh > 0; --sh) {
+        for (int sh = c->whole_cells ? 0 : SPH_SLAB_SHIFT; sh > 0; --sh) {
             if (need + kField * ((1 << sh) - 1) <= 32) {
                 shift = sh;
                 start_bits = need;
@@ -831,6 +831,27 @@ int sph_set_lanes(sph_ctx* c, int32_t lanes) {
         return sph_set_batch_lanes(c, all.data(), (int32_t)all.size(), nullptr);
     }
     c->lanes = lanes;
+    return SPH_OK;
+}
+
+int sph_set_slabs(sph_ctx* c, int32_t enable, void* stream) {
+    if (!c) return fail(SPH_E_INVALID, "null context");
+    if (!c->inst.empty()) return fail(SPH_E_STATE, "a batch's cells are always cut into slabs");
+    if (c->prm.formulation != SPH_FORM_NAIVE)
+        return fail(SPH_E_INVALID, "the VOXEL formulation keeps the reference's cell table");
+    if (c->adjoint)
+        return fail(SPH_E_STATE, "an SPH_FLAG_ADJOINT context keeps one cell layout (its snapshots hold tables)");
+    if (c->last_stage != 0 && c->last_stage != SPH_STAGE_FORCE)
+        return fail(SPH_E_STATE, "a staged step is in progress (run it to SPH_STAGE_FORCE first)");
+    c->whole_cells = enable ? 0 : 1;
+    plan_slabs(c);
+    if (c->bound && c->have_state) {
+        // the next sort reads keys of the new layout; the last step's table is not readable
+        int rc;
+        if ((rc = hip_check(sph::launch_rekey(c, (hipStream_t)stream), "rekey"))) return rc;
+        c->last_stage = 0;
+        c->a_in_snapshot = false;
+    }
     return SPH_OK;
 }
 

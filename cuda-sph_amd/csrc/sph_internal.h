@@ -184,6 +184,7 @@ struct sph_ctx {
     int check_finite;        // SPH_FLAG_CHECK_FINITE: sph_step reads the non-finite count
     int adjoint;             // SPH_FLAG_ADJOINT: reverse-mode buffers are part of the workspace
     int cell_sort;           // NAIVE formulation: counting sort by cell instead of the radix sort
+    int whole_cells;         // sph_set_slabs(ctx, 0): cells are not cut into x slabs
     bool adj_seeded;
     size_t ws_bytes;
     void* ws;
@@ -218,6 +219,8 @@ bool count_in_force(const sph_ctx* c);
 int64_t cell_scan_tiles(int64_t cells);
 hipError_t launch_cell_table(sph_ctx* c, hipStream_t s);
 hipError_t launch_reorder(sph_ctx* c, hipStream_t s);
+// keyI of the resident state (posB slots) again, under the context's current cell layout
+hipError_t launch_rekey(sph_ctx* c, hipStream_t s);
 hipError_t launch_density(sph_ctx* c, hipStream_t s);
 // count: also rank the new keys into cell_cnt / keyS[0] for the next sort (count_in_force)
 hipError_t launch_force(sph_ctx* c, hipStream_t s, bool count = false);

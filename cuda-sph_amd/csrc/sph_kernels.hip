@@ -122,6 +122,16 @@ __device__ __forceinline__ void store_density(const StepConsts& C, int i, float 
     reinterpret_cast<float*>(vel + i)[3] = 1.0f / rho;
 }
 
+// Keys of the resident state (NAIVE: indexed by posB slot) under the current cell layout,
+// after sph_set_slabs changed it; the non-finite count was taken when the state was hashed.
+__global__ __launch_bounds__(256) void rekey_kernel(StepConsts C, const float4* __restrict__ posB,
+                                                     uint32_t* __restrict__ keyI) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C.n) return;
+    const float4 p = posB[i];
+    keyI[i] = sort_key(p.x, p.y, p.z, C);
+}
+
 // Density (naive_kernels.py:7-21) and the particle's neighbour list: every k with
 // r^2 <= h^2 in candidate order, the particle itself included (the force pass gives the
 // self pair a zero contribution).  (1 - r^2/h^2)^3 is accumulated for exactly the listed
@@ -763,6 +773,11 @@ hipError_t launch_reorder(sph_ctx* c, hipStream_t s) {
     if (c->n > 0)
         reorder_kernel<<<blocks_for(c->n, 256), 256, 0, s>>>(c->sc, c->sorted_vals, c->b.posB, c->b.velB,
                                                                c->b.pidB, c->b.posA, c->b.velA, c->b.pidA, c->b.flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_rekey(sph_ctx* c, hipStream_t s) {
+    if (c->n > 0) rekey_kernel<<<blocks_for(c->n, 256), 256, 0, s>>>(c->sc, c->b.posB, c->b.keyI);
     return hipGetLastError();
 }
 

@@ -63,7 +63,7 @@ EXPORTED = [
     "sph_run_stage", "sph_buffer_elems", "sph_read_buffer", "sph_diagnostics", "sph_grid_info", "sph_set_launch",
     "sph_create_batch", "sph_batch_info", "sph_set_constants", "sph_set_lanes", "sph_adjoint_seed", "sph_adjoint_step",
     "sph_adjoint_read", "sph_set_batch_lanes", "sph_adjoint_snapshot_bytes", "sph_adjoint_save", "sph_step_save", "sph_adjoint_step_from",
-    "sph_check_finite", "sph_noalloc_scratch_bytes", "sph_step_noalloc", "sph_step_timed",
+    "sph_check_finite", "sph_noalloc_scratch_bytes", "sph_step_noalloc", "sph_step_timed", "sph_set_slabs",
 ]
 
 _lib = None
@@ -102,6 +102,7 @@ def load_library() -> ctypes.CDLL:
         "sph_batch_info": ([vp, POINTER(i32), POINTER(i32)], i32),
         "sph_set_constants": ([vp, POINTER(SphConstantsC)], i32),
         "sph_set_lanes": ([vp, i32], i32),
+        "sph_set_slabs": ([vp, i32, vp], i32),
         "sph_set_batch_lanes": ([vp, vp, i32, vp], i32),
         "sph_adjoint_seed": ([vp, vp, vp, vp], i32),
         "sph_adjoint_step": ([vp, vp], i32),
@@ -302,6 +303,10 @@ class NativeContext:
 
     def set_lanes(self, lanes: int):
         check(self.lib.sph_set_lanes(self.handle, int(lanes)), "sph_set_lanes")
+
+    def set_slabs(self, enable: bool):
+        self.generation += 1
+        check(self.lib.sph_set_slabs(self.handle, 1 if enable else 0, _stream(self.device)), "sph_set_slabs")
 
     def set_batch_lanes(self, lanes):
         """Lanes per particle of each instance of a batched context."""

@@ -32,7 +32,7 @@ import hashlib
 import json
 import os
 import time
-from dataclasses import asdict, dataclass, field
+from dataclasses import asdict, dataclass, field, replace
 from typing import Dict, Iterable, List, Optional, Sequence, Tuple, Union
 
 import numpy as np
@@ -63,6 +63,7 @@ class TuneResult:
     # lanes per particle of the density and force passes (batched form: one per instance)
     lanes: Union[int, Tuple[int, ...]] = 1
     rule: Tuple[str, ...] = ()    # batched form: the winning (grid, lanes) rules
+    slabs: bool = True            # plain NAIVE step: cells cut into x slabs (sph_set_slabs)
 
     @property
     def launch(self) -> Tuple[int, int]:
@@ -83,7 +84,7 @@ class TuneResult:
         lanes = tuple(int(x) for x in lanes) if isinstance(lanes, list) else int(lanes)
         return TuneResult(scale, int(d["density_block"]), int(d["force_block"]), float(d["ms_per_step"]),
                           tuple(tuple(t) for t in d.get("trials", ())), 0.0, True, lanes,
-                          tuple(d.get("rule", ())))
+                          tuple(d.get("rule", ())), bool(d.get("slabs", True)))
 
 
 def default_cache_path() -> str:
@@ -182,9 +183,10 @@ def _search(make, scales: Sequence, blocks: Sequence[int], steps: int, warmup: i
     scale and lanes are opaque keys.  With cross, every (grid, lanes) pair is timed with the
     largest workgroups first; otherwise every grid with the middle lanes key, then every
     lanes key on the fastest grid.  Then every block pair is timed on the fastest of those;
-    the fastest trial of each of the `finalists` best lanes keys is timed once more, over
-    final_steps steps (default: steps; a longer window than the search's when a workload's
-    cost changes as it evolves), and the fastest of that second timing wins (the first
+    the fastest trial of each of the `finalists` best lanes keys (and the runner-up block pair
+    of the best) is timed once more, over final_steps steps (default: steps; a longer window
+    than the search's when a workload's cost changes as it evolves), and the fastest of that
+    second timing wins (the first
     timing of a run also carries warm-up effects).  The very first timing is repeated and
     discarded for the same reason."""
     t0 = time.perf_counter()
@@ -239,6 +241,12 @@ def _search(make, scales: Sequence, blocks: Sequence[int], steps: int, warmup: i
     for t in sorted(trials, key=lambda t: t[4]):
         if len(ranked) < max(1, finalists) and all(r[3] != t[3] for r in ranked):
             ranked.append(t)
+    # the best lanes key's runner-up block pair too: block pairs a few tenths of a percent
+    # apart in the short search trade places over the whole run
+    runner = [t for t in sorted(trials, key=lambda t: t[4])
+              if t[0] == ranked[0][0] and t[3] == ranked[0][3] and t[1:3] != ranked[0][1:3]]
+    if runner:
+        ranked.append(runner[0])
     final = []
     for scale in dict.fromkeys(t[0] for t in ranked):
         final += run(scale, [t[1:4] for t in ranked if t[0] == scale], False, final_steps or steps)
@@ -246,16 +254,38 @@ def _search(make, scales: Sequence, blocks: Sequence[int], steps: int, warmup: i
     return TuneResult(best[0], best[1], best[2], best[4], tuple(trials), time.perf_counter() - t0, False, best[3])
 
 
+def _choose_slabs(make, res: TuneResult, steps: int, warmup: int, log=None) -> TuneResult:
+    """The winning configuration timed again with the cells cut into x slabs (the default)
+    and searched whole; the faster is kept.  Cells of many particles (coarse grids, dense
+    problems) gain little from the slabs and pay for ordering them."""
+    configure, step, reset, close = make(res.cell_scale)
+    ms = {}
+    try:
+        for slabs in (True, False):
+            configure(res.density_block, res.force_block, res.lanes, slabs)
+            ms[slabs] = _time_steps(step, reset, steps, warmup)
+            if log:
+                log(f"autotune: scale {res.cell_scale} blocks {res.density_block}/{res.force_block} "
+                    f"lanes {res.lanes} slabs {'on' if slabs else 'off'}: {ms[slabs]:.4f} ms/step")
+    finally:
+        close()
+    best = min(ms, key=ms.get)
+    return replace(res, slabs=best, ms_per_step=round(ms[best], 5))
+
+
 def tune_strategy(params: SimulationParameters, constants: SPHConstants, state: SimulationState, mode: str = "BOX",
                   formulation: str = "NAIVE", steps: int = 20, warmup: int = 3,
                   scales: Sequence[float] = (1.0, 0.5), blocks: Sequence[int] = BLOCKS,
                   lanes: Sequence[int] = LANES, cache: Optional[TuneCache] = None, device="cuda:0",
-                  log=None, final_steps: Optional[int] = None) -> TuneResult:
+                  log=None, final_steps: Optional[int] = None, choose_slabs: bool = True) -> TuneResult:
     """Tunes the plain step for one simulation.  VOXEL: block sizes only.  final_steps: the
-    window over which the finalists are re-timed (see _search)."""
+    window over which the finalists are re-timed (see _search).  choose_slabs: NAIVE, also
+    decide whether the cells are cut into x slabs (False: they are, as in a context that
+    cannot change it, one with SPH_FLAG_ADJOINT)."""
     from .strategy import NaiveSPHStrategy, VoxelSPHStrategy
 
-    key = signature("plain", [(params, constants, state)], mode, formulation, (steps, final_steps), device)
+    key = signature("plain", [(params, constants, state)], mode, formulation, (steps, final_steps, choose_slabs),
+                    device)
     if cache is not None:
         hit = cache.get(key)
         if hit is not None:
@@ -269,13 +299,17 @@ def tune_strategy(params: SimulationParameters, constants: SPHConstants, state: 
         gv = None if scale is None else voxel_for(constants.h) * scale
         strat = cls(params, constants=constants, mode=mode, device=device, grid_voxel=gv)
 
-        def configure(db, fb, ln):
+        def configure(db, fb, ln, slabs=True):
             strat.ctx.set_launch(db, fb)
             strat.ctx.set_lanes(ln)
+            if formulation == "NAIVE":
+                strat.ctx.set_slabs(slabs)
         return configure, strat.ctx.step, lambda: strat.ctx.upload(pos, vel), strat.ctx.close
 
     res = _search(make, grid_scales, blocks, steps, warmup, log, lanes if formulation == "NAIVE" else (1,),
                   final_steps=final_steps)
+    if formulation == "NAIVE" and choose_slabs:
+        res = _choose_slabs(make, res, final_steps or steps, warmup, log)
     if cache is not None:
         cache.put(key, res)
     return res

@@ -45,13 +45,15 @@ class SPHStrategy:
 
     def __init__(self, params: SimulationParameters, constants: Optional[SPHConstants] = None, mode: str = "BOX",
                  device="cuda:0", keep_terms: bool = False, seed: int = RNG_SEED, grid_voxel: Optional[float] = None,
-                 launch: Optional[Tuple[int, int]] = None, lanes: int = 1, check_finite: bool = True):
+                 launch: Optional[Tuple[int, int]] = None, lanes: int = 1, check_finite: bool = True,
+                 slabs: bool = True):
         """grid_voxel: NAIVE formulation only, the edge of the search grid's cells in place
         of params.voxel_size (the NAIVE sums do not depend on it; see autotune.py).
         launch: (density, force) threads per workgroup.  lanes: NAIVE only, lanes of a
         wavefront per particle in the density and force passes.  check_finite: every step
         call waits for its steps and raises NonFiniteStateError if a position turned
-        NaN or infinite."""
+        NaN or infinite.  slabs: NAIVE only, whether the search cuts its cells into x slabs
+        (False: whole cells; see autotune.py)."""
         if mode not in _MODES:
             raise Exception(f"Wrong simulation mode! ({mode})")
         if grid_voxel is not None and self.formulation != N.SPH_FORM_NAIVE:
@@ -81,6 +83,9 @@ class SPHStrategy:
         if lanes != 1:
             self.ctx.set_lanes(lanes)
             self._lanes = int(lanes)
+        self.slabs = bool(slabs) if self.formulation == N.SPH_FORM_NAIVE else False
+        if self.formulation == N.SPH_FORM_NAIVE and not slabs:
+            self.ctx.set_slabs(False)
         logger.info("SPH strategy on %s: %s", self.device, self.ctx.grid_info())
 
     # -- reference entry point -----------------------------------------------------------

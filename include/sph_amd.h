@@ -229,6 +229,15 @@ int sph_set_launch(sph_ctx* ctx, int32_t density_block, int32_t force_block);
  * context every instance gets `lanes`. */
 int sph_set_lanes(sph_ctx* ctx, int32_t lanes);
 
+/* NAIVE formulation, plain context without SPH_FLAG_ADJOINT: whether the fused step cuts
+ * each cell into four x slabs (enable = 1, the default), so that a row of cells is searched
+ * slab by slab, or searches whole cells (enable = 0), for the autotuner: cells of many
+ * particles gain little from the slabs and pay for ordering them.  The pairs found are the
+ * same either way; sums are added in a different order (FP32 rounding only).  With a state
+ * resident, its keys are rewritten on `stream` for the next sort, and the last step's cell
+ * table can no longer be read.  No counterpart in the reference (an all-pairs search). */
+int sph_set_slabs(sph_ctx* ctx, int32_t enable, void* stream);
+
 /* Batched context: lanes per particle of each instance (count = the number of instances, each
  * 1, 2, 4, 8, 16, 32 or 64), so that instances with many neighbours per particle get more lanes and
  * sparse ones fewer.  Each instance computes what a plain context with its lane count

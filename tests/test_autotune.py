@@ -86,8 +86,9 @@ def test_finalists_are_retimed(monkeypatch):
 
 def test_finalists_are_one_per_lane_count_and_retimed_over_the_run(monkeypatch):
     """The three fastest search trials are all one-lane block pairs; the finalists are the
-    fastest trial of each lane count instead, re-timed over final_steps, where 4 lanes win
-    (a workload whose cost per step changes as it evolves)."""
+    fastest trial of each lane count instead, and the runner-up block pair of the fastest
+    lane count, re-timed over final_steps, where 4 lanes win (a workload whose cost per step
+    changes as it evolves)."""
     search = {1: {(128, 128): 1.0, (64, 64): 1.1, (64, 128): 1.2, (128, 64): 1.3}, 4: {(128, 128): 1.5}, 2: {(128, 128): 1.4}}
     final = {1: 2.0, 2: 2.2, 4: 1.8}
     cfg, steps_seen = {}, []
@@ -99,13 +100,14 @@ def test_finalists_are_one_per_lane_count_and_retimed_over_the_run(monkeypatch):
 
     def fake_time(step, reset, steps, warmup):
         db, fb, ln = cfg["k"]
-        steps_seen.append(steps)
+        steps_seen.append((steps, cfg["k"]))
         return final[ln] if steps == 50 else search[ln][(db, fb)]
 
     monkeypatch.setattr(T, "_time_steps", fake_time)
     res = T._search(make, [1.0], (64, 128), 5, 1, lanes=(1, 2, 4), final_steps=50)
     assert (res.lanes, res.launch, res.ms_per_step) == (4, (128, 128), 1.8)
-    assert steps_seen.count(50) == 3  # one finalist per lane count
+    finals = [k for n, k in steps_seen if n == 50]
+    assert sorted(finals) == sorted([(128, 128, 1), (64, 64, 1), (128, 128, 2), (128, 128, 4)])
 
 
 def test_grid_voxel_is_power_of_two():

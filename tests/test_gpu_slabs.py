@@ -139,3 +139,61 @@ def test_dense_cells_searched_whole(gpu):
                                 space=w.params.space_size, voxel=np.full(3, edge), use_grid=True)
     assert np.max(np.abs(ra - rrho) / rrho) <= 1e-5
     assert np.max(np.abs(pa - rp) / np.maximum(np.abs(rp), 1e-3)) <= 1e-5
+
+
+def test_whole_cells_option(gpu):
+    """sph_set_slabs(ctx, 0): the fused step searches whole cells and orders each cell by id,
+    the reference's order (voxel_sph_strategy.py:84-88); the same pairs as with slabs, so the
+    state agrees to FP32 rounding.  Switching with a state resident rewrites its keys: the
+    next steps are those of a context made with that setting, bit for bit."""
+    w = box_uniform(20000, seed=95)
+    pos = f32(w.state.position)
+    vel = f32(np.random.default_rng(95).normal(0.0, 0.5, size=pos.shape))
+    edge = 2.0 ** -5
+
+    def make(slabs):
+        s = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu, grid_voxel=edge, slabs=slabs)
+        upload(s, pos, vel)
+        return s
+
+    whole, cut = make(False), make(True)
+    whole.ctx.step(1)
+    cut.ctx.step(1)
+    dims = np.asarray(whole.ctx.grid_info()["dims"])
+    c = np.clip(np.floor(pos / edge), 0, dims - 1).astype(np.int64)
+    key = c[:, 0] + dims[0] * (c[:, 1] + dims[1] * c[:, 2])
+    assert np.array_equal(read(whole, N.BUF_PERM), oracle.sort_map(key))
+    pw, vw, rw = [t.cpu().numpy() for t in whole.ctx.download()][:3]
+    pc, vc, rc = [t.cpu().numpy() for t in cut.ctx.download()][:3]
+    assert np.max(np.abs(rw - rc) / rc) <= 1e-6
+    assert np.max(np.abs(pw - pc) / np.maximum(np.abs(pc), 1e-3)) <= 1e-6
+    # switch each context to the other setting and step on: bitwise the steps of a context
+    # made with that setting from the same state
+    refs = []
+    for a, slabs in ((cut, False), (whole, True)):
+        p, v = [t.cpu().numpy() for t in a.ctx.download()][:2]
+        ref = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu, grid_voxel=edge, slabs=slabs)
+        upload(ref, p, v)
+        refs.append(ref)
+    cut.ctx.set_slabs(False)
+    whole.ctx.set_slabs(True)
+    with pytest.raises(RuntimeError):  # the last step's table is not readable after a switch
+        read(cut, N.BUF_CELL_BEGIN)
+    for a, b in zip((cut, whole), refs):
+        a.ctx.step(3)
+        b.ctx.step(3)
+        for ta, tb in zip(a.ctx.download(), b.ctx.download()):
+            assert torch.equal(ta, tb)
+
+
+def test_set_slabs_refused_where_layout_is_fixed(gpu):
+    from sph_amd.batched import BatchedSPH
+    from sph_amd.strategy import VoxelSPHStrategy
+
+    w = box_uniform(3000, seed=96)
+    b = BatchedSPH([(w.params, w.constants)] * 2, device=gpu)
+    with pytest.raises(RuntimeError):
+        b.ctx.set_slabs(False)
+    v = VoxelSPHStrategy(w.params, constants=w.constants, device=gpu)
+    with pytest.raises(RuntimeError):
+        v.ctx.set_slabs(False)
