@@ -21,8 +21,12 @@ def test_tune_strategy_caches_and_keeps_parity(gpu, tmp_path):
     res = T.tune_strategy(w.params, w.constants, w.state, steps=4, warmup=1, scales=(1.0, 0.5, 0.25),
                           blocks=(64, 256), lanes=(1, 4), cache=cache, device=gpu)
     assert len(res.trials) == 3 * 2 + 3 and not res.cached
-    # the winner is the fastest trial of one of the lane counts, re-timed
+    # the winner is the fastest trial of one of the lane counts, or the runner-up block pair
+    # of the fastest, re-timed
     finalists = [min((t for t in res.trials if t[3] == ln), key=lambda t: t[4]) for ln in (1, 4)]
+    top = min(res.trials, key=lambda t: t[4])
+    finalists += sorted((t for t in res.trials if t[0] == top[0] and t[3] == top[3] and t[1:3] != top[1:3]),
+                        key=lambda t: t[4])[:1]
     assert (res.cell_scale, res.density_block, res.force_block, res.lanes) in [t[:4] for t in finalists]
     again = T.tune_strategy(w.params, w.constants, w.state, steps=4, warmup=1, scales=(1.0, 0.5, 0.25),
                             blocks=(64, 256), lanes=(1, 4), cache=cache, device=gpu)
