@@ -235,7 +235,9 @@ int sph_set_lanes(sph_ctx* ctx, int32_t lanes);
  * particles gain little from the slabs and pay for ordering them.  The pairs found are the
  * same either way; sums are added in a different order (FP32 rounding only).  With a state
  * resident, its keys are rewritten on `stream` for the next sort, and the last step's cell
- * table can no longer be read.  No counterpart in the reference (an all-pairs search). */
+ * table can no longer be read.  HIP graphs of sph_step recorded before the call keep the
+ * previous layout in their kernel arguments: record them again.  No counterpart in the
+ * reference (an all-pairs search). */
 int sph_set_slabs(sph_ctx* ctx, int32_t enable, void* stream);
 
 /* Batched context: lanes per particle of each instance (count = the number of instances, each
