@@ -16,8 +16,8 @@
  *     Pointers without the prefix are host pointers.
  *   - stream is a hipStream_t passed as void*; NULL means the default stream.  All work
  *     is enqueued on that stream; the calls that wait for it are sph_bind_workspace,
- *     sph_set_batch_lanes, sph_check_finite and sph_step on a context created with
- *     SPH_FLAG_CHECK_FINITE.
+ *     sph_set_batch_lanes, sph_check_finite, sph_step_timed and sph_step on a context
+ *     created with SPH_FLAG_CHECK_FINITE.
  *   - The library never allocates device memory: the caller allocates the workspace
  *     (sph_workspace_bytes) and keeps it alive while the context is used.
  *   - A context is not re-entrant: calls on one context must not overlap.
