@@ -157,12 +157,15 @@ def test_density_and_force_stages(gpu):
     assert eP2.max() <= TOL
 
 
+@pytest.mark.parametrize("fused", [False, True])
 @pytest.mark.parametrize("lanes", [1, 4])
-def test_density_of_particles_outside_the_grid(gpu, lanes):
+def test_density_of_particles_outside_the_grid(gpu, lanes, fused):
     """naive_kernels.py:7-21 sums over all pairs, wherever the particles are.  The grid keeps
     particles beyond its faces in its edge cells and searches from their clamped positions;
     clusters of mutual neighbours beyond a face, straddling the far face and beyond a corner
-    must get the all-pairs density, and so must the particles inside."""
+    must get the all-pairs density, and so must the particles inside.  Through the staged
+    stages (whole cells) and the fused step (x slabs, the edge cells' outer slabs extending
+    beyond the grid)."""
     w = box_uniform(4096, seed=15)
     c = w.constants
     L = np.asarray(w.params.space_size, dtype=np.float64)
@@ -178,8 +181,11 @@ def test_density_of_particles_outside_the_grid(gpu, lanes):
     pos = f32(pos)
     strat = NaiveSPHStrategy(w.params, constants=c, device=gpu, lanes=lanes)
     upload(strat, pos, np.zeros_like(pos))
-    for st in (N.STAGE_SORT, N.STAGE_TABLE, N.STAGE_REORDER, N.STAGE_DENSITY):
-        strat.ctx.run_stage(st)
+    if fused:
+        strat.ctx.step(1)
+    else:
+        for st in (N.STAGE_SORT, N.STAGE_TABLE, N.STAGE_REORDER, N.STAGE_DENSITY):
+            strat.ctx.run_stage(st)
     rho = read(strat, N.BUF_DENSITY).astype(np.float64)
     rho_ref = oracle.naive_density(pos, c)
     rel = np.abs(rho - rho_ref) / rho_ref
