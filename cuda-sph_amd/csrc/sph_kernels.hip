@@ -192,6 +192,16 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
     // first and one past the last candidate: every list entry fits an int16 offset when the
     // candidates do (checked once per particle, not per hit)
     uint32_t klo = 0xffffffffu, khi = 0u;
+    // L = 1 appends its hits in slot order: the next staged slot is reached from the last by a
+    // step that alternates between 1 (the other half of the word) and the band's row pitch
+    // less 1, in place of working out its address from the count on every hit
+    ListEntry* slot = &band.at(0);
+    uint32_t slot_step = 1;
+    auto stage = [&](ListEntry e) {
+        *slot = e;
+        slot += slot_step;
+        slot_step = 2u * (64u / L) - slot_step;
+    };
     // one candidate: a hit (r^2 <= h^2) adds (1 - r^2/h^2)^3 to the sum and is appended to the
     // list; CHECKED: the list may be full (staged slots, then HBM slots, then only counted)
     auto visit = [&](float4 q, uint32_t k, auto checked) {
@@ -201,10 +211,10 @@ __device__ __forceinline__ void density_particle(const StepConsts& G, const Step
             const float t = 1.0f - r2 * C.inv_h2;
             acc += t * t * t;
             if constexpr (decltype(checked)::value) {
-                if (cnt < kCap) band.at(cnt) = list_encode(k, i);
+                if (cnt < kCap) stage(list_encode(k, i));
                 else if (cnt < (uint32_t)kListCap) nl[list_at(G.n, i, cnt)] = list_encode(k, i);
             } else {
-                band.at(cnt) = list_encode(k, i);
+                stage(list_encode(k, i));
             }
             ++cnt;
         }
