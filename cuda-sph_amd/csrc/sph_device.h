@@ -499,12 +499,14 @@ __device__ __forceinline__ size_t list_at(int n, int i, uint32_t t) {
 struct PairSums {
     float ax = 0.f, ay = 0.f, az = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
     // self = the pair is the particle itself: 1/r is taken as 0, which makes both terms zero
-    // (the pressure term carries a factor 1/r, the viscosity term a factor v_j - v_i = 0)
+    // (the pressure term carries a factor 1/r, the viscosity term a factor v_j - v_i = 0).
+    // 1/r is the hardware reciprocal square root: rsqrtf gives the same for every r^2 above
+    // the FP32 denormal range and only rescales below it (distances under 1e-19).
     __device__ __forceinline__ void add(const StepConsts& C, float4 p, float4 v, float4 q, float4 vj,
                                         bool self = false) {
         const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
-        const float rinv = self ? 0.0f : rsqrtf(r2);
+        const float rinv = self ? 0.0f : __builtin_amdgcn_rsqf(r2);
         const float hr = C.h - r2 * rinv;
         const float cp = (p.w + q.w) * vj.w * hr * hr * rinv;
         ax += cp * dx;
