@@ -115,7 +115,9 @@ Layout plan_layout(const sph_ctx* c) {
     sz[S_MUP] = c->adjoint ? 8 * nn : 16;
     sz[S_SUMS] = c->adjoint ? 32 * nn : 16;
     sz[S_CCNT] = c->cell_sort ? 4 * (size_t)(c->sc.cs_cells > 0 ? c->sc.cs_cells : 1) : 16;
-    sz[S_CPART] = c->cell_sort ? 4 * (size_t)sph::cell_scan_tiles(c->sc.cs_cells > 0 ? c->sc.cs_cells : 1) : 16;
+    sz[S_CPART] = c->cell_sort ? 8 * (size_t)sph::kTileSumCopies *
+                                     (size_t)sph::cell_scan_tiles(c->sc.cs_cells > 0 ? c->sc.cs_cells : 1)
+                               : 16;
     Layout L;
     size_t o = 0;
     for (int i = 0; i < S_COUNT; ++i) {

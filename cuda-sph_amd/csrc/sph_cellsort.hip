@@ -11,11 +11,12 @@
 //            the particles of a cell).  Between the steps of one multi-step call the previous
 //            step's force pass has done this already as it wrote the keys (rank_in_cell), and
 //            the sort starts at the scan
-//   scan     exclusive prefix sum of cnt over all cells: tile sums (16-byte loads), then each
-//            tile's offset is the sum of the tile sums before it (added up by the tile's own
-//            workgroup when there are few tiles, else by a one-workgroup scan first), then
-//            the tile is scanned; the result is the cell start table cs[0..C] itself, and the
-//            tile scan resets cnt to zero for the next step
+//   scan     exclusive prefix sum of cnt over all cells: tile sums (16-byte loads; a force
+//            pass that counted has added them up already), then each tile's offset is the
+//            sum of the tile sums before it (added up by the tile's own workgroup when there
+//            are few tiles, else by a one-workgroup scan first), then the tile is scanned;
+//            the result is the cell start table cs[0..C] itself, and the tile scan resets
+//            cnt, and the other array of tile sums, to zero for the next step
 //   scatter  one thread per slot: tmp[cs[key] + rank] = slot
 //   order    one thread per position: the particle there moves to cs[key] + (number of
 //            particles of its cell with a smaller id), which is id order within the cell; the
@@ -37,11 +38,14 @@
 namespace sph {
 namespace {
 
-constexpr int CT = 256;          // threads per workgroup
-constexpr int CI = 16;           // cells per thread in the scan
-constexpr int CTILE = CT * CI;   // cells per scan tile
+constexpr int CT = 256;                        // threads per workgroup
+constexpr int CTILE = 1 << kScanTileShift;     // cells per scan tile
+constexpr int CI = CTILE / CT;                 // cells per thread in the scan
+static_assert(CI == 16, "the scan's loads assume 16 cells per thread");
 constexpr int CWAVES = CT / 64;
-constexpr int kMergedTiles = 2048;  // up to this many tiles, each tile's workgroup sums the counts before it
+// up to this many tiles, each tile's workgroup sums the tile sums before it (at most 32
+// loads per thread)
+constexpr int kMergedTiles = 8192 / kTileSumCopies;
 
 // The slab layout of the cell table (StepConsts::slab_shift .. start_mask).
 struct SlabTable {
@@ -102,6 +106,7 @@ __device__ __forceinline__ uint32_t block_excl(uint32_t v, uint32_t* lw, uint32_
 
 constexpr int CV = CI / 4;  // 16-byte vectors per thread and tile
 
+// Tile sums into part copy 0 (the other copies of the tile cleared).
 __global__ __launch_bounds__(CT) void cell_scan_reduce(const uint32_t* __restrict__ cnt, int m,
                                                         uint32_t* __restrict__ part) {
     __shared__ uint32_t lw[CWAVES];
@@ -122,16 +127,19 @@ __global__ __launch_bounds__(CT) void cell_scan_reduce(const uint32_t* __restric
     }
     uint32_t tot;
     block_excl(s, lw, &tot);
-    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+    if (threadIdx.x < kTileSumCopies) part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = threadIdx.x == 0 ? tot : 0u;
 }
 
-// One workgroup: exclusive scan of the tile sums, in chunks of CT with a carry.
+// One workgroup: exclusive scan of the tile sums (over their copies) into copy 0, in chunks
+// of CT with a carry.
 __global__ __launch_bounds__(CT) void cell_scan_partials(uint32_t* __restrict__ part, int nparts) {
     __shared__ uint32_t lw[CWAVES];
     uint32_t carry = 0;
     for (int base = 0; base < nparts; base += CT) {
         const int i = base + threadIdx.x;
-        const uint32_t v = i < nparts ? part[i] : 0u;
+        uint32_t v = 0u;
+        if (i < nparts)
+            for (int k = 0; k < kTileSumCopies; ++k) v += part[(size_t)k * nparts + i];
         uint32_t tot;
         const uint32_t ex = block_excl(v, lw, &tot);
         if (i < nparts) part[i] = carry + ex;
@@ -139,11 +147,17 @@ __global__ __launch_bounds__(CT) void cell_scan_partials(uint32_t* __restrict__ 
     }
 }
 
-// Sum of part[0 .. b) over the workgroup (the tile's offset when the tile counts were not
-// scanned first).
-__device__ __forceinline__ uint32_t tiles_before(const uint32_t* __restrict__ part, int b, uint32_t* lw) {
+// Sum of the tile sums before tile b, over their copies (the tile's offset when the tile
+// sums were not scanned first).
+__device__ __forceinline__ uint32_t tiles_before(const uint32_t* __restrict__ part, int b, int tiles, uint32_t* lw) {
     uint32_t s = 0;
-    for (int j = threadIdx.x; j < b; j += CT) s += part[j];
+    for (int j = threadIdx.x; j < b; j += CT) {  // the copies of a tile in flight together
+        uint32_t v[kTileSumCopies];
+#pragma unroll
+        for (int k = 0; k < kTileSumCopies; ++k) v[k] = part[(size_t)k * tiles + j];
+#pragma unroll
+        for (int k = 0; k < kTileSumCopies; ++k) s += v[k];
+    }
     uint32_t tot;
     block_excl(s, lw, &tot);
     return tot;
@@ -153,18 +167,20 @@ __device__ __forceinline__ uint32_t tiles_before(const uint32_t* __restrict__ pa
 // scans CI consecutive cells there, and cs is written back coalesced.  LDS index i is padded
 // by i / 32 so that the thread-contiguous pass does not hit one bank.  The last tile also
 // writes cs[m] = n.  MERGED: part holds tile sums (the offset is summed here); otherwise
-// part holds the tiles' exclusive prefix sums.
+// copy 0 of part holds the tiles' exclusive prefix sums.  Each workgroup also clears its
+// tile's copies in next, the tile sums that the next force pass adds into.
 
 __device__ __forceinline__ int cpad(int i) { return i + (i >> 5); }
 
 template <bool MERGED>
 __global__ __launch_bounds__(CT) void cell_scan_down(uint32_t* __restrict__ cnt, int m,
-                                                      const uint32_t* __restrict__ part, uint32_t* __restrict__ cs,
-                                                      uint32_t n, uint32_t empty) {
+                                                      const uint32_t* __restrict__ part, uint32_t* __restrict__ next,
+                                                      uint32_t* __restrict__ cs, uint32_t n, uint32_t empty) {
     __shared__ uint32_t lw[CWAVES];
     __shared__ uint32_t tile[CTILE + CTILE / 32];
     const int64_t base = (int64_t)blockIdx.x * CTILE;
-    const uint32_t offset = MERGED ? tiles_before(part, blockIdx.x, lw) : part[blockIdx.x];
+    if (threadIdx.x < kTileSumCopies) next[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = 0u;
+    const uint32_t offset = MERGED ? tiles_before(part, blockIdx.x, gridDim.x, lw) : part[blockIdx.x];
     const bool full = base + CTILE <= m;  // base is a multiple of 4 cells: 16-byte aligned
     if (full) {
         uint4* c4 = reinterpret_cast<uint4*>(cnt + base);
@@ -341,8 +357,10 @@ bool cell_sort_global(const sph_ctx* c) { return !(c->sc.cs_cells <= kLdsSortCel
 int64_t cell_scan_tiles(int64_t cells) { return (cells + CTILE - 1) / CTILE; }
 
 // Sorted keys and slots go to keyS[1] / valS[1]; the cell table to cs.  counted: the
-// per-cell counts and the ranks (in keyS[0]) of the current keys are there already
-// (launch_force with count; only on the global path, see count_in_force).
+// per-cell counts and the ranks (in keyS[0]) of the current keys are there already, and with
+// tile_sums_in_force also the tile sums (cell_part array part_phase) (launch_force with
+// count; only on the global path, see count_in_force).  The scan clears the other array and
+// part_phase names it afterwards.
 hipError_t launch_cell_sort(sph_ctx* c, bool gather, hipStream_t s, bool counted) {
     const int n = (int)c->n;
     const int m = c->sc.cs_cells;
@@ -365,13 +383,18 @@ hipError_t launch_cell_sort(sph_ctx* c, bool gather, hipStream_t s, bool counted
         c->sorted_vals = b.valS[1];
         return hipGetLastError();
     }
+    const bool summed = counted && tile_sums_in_force(c);
+    const int cur = summed ? c->part_phase : 0;
+    uint32_t* part = b.cell_part + (size_t)cur * kTileSumCopies * tiles;
+    uint32_t* next = b.cell_part + (size_t)(cur ^ 1) * kTileSumCopies * tiles;
+    c->part_phase = cur ^ 1;
     if (n > 0 && !counted) cell_count_kernel<<<cblocks(n), CT, 0, s>>>(n, b.keyI, b.cell_cnt, rank);
-    cell_scan_reduce<<<tiles, CT, 0, s>>>(b.cell_cnt, m, b.cell_part);
+    if (!summed) cell_scan_reduce<<<tiles, CT, 0, s>>>(b.cell_cnt, m, part);
     if (tiles <= kMergedTiles) {
-        cell_scan_down<true><<<tiles, CT, 0, s>>>(b.cell_cnt, m, b.cell_part, b.cs, (uint32_t)n, T.empty());
+        cell_scan_down<true><<<tiles, CT, 0, s>>>(b.cell_cnt, m, part, next, b.cs, (uint32_t)n, T.empty());
     } else {
-        cell_scan_partials<<<1, CT, 0, s>>>(b.cell_part, tiles);
-        cell_scan_down<false><<<tiles, CT, 0, s>>>(b.cell_cnt, m, b.cell_part, b.cs, (uint32_t)n, T.empty());
+        cell_scan_partials<<<1, CT, 0, s>>>(part, tiles);
+        cell_scan_down<false><<<tiles, CT, 0, s>>>(b.cell_cnt, m, part, next, b.cs, (uint32_t)n, T.empty());
     }
     if (n > 0) {
         cell_scatter_kernel<<<cblocks(n), CT, 0, s>>>(n, b.keyI, rank, b.cs, tmp, T.start_mask);

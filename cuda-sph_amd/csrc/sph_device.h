@@ -157,6 +157,24 @@ __device__ __forceinline__ uint32_t rank_in_cell(uint32_t key, bool valid, uint3
     return base + (uint32_t)(lane - first);
 }
 
+// Adds each active lane's cell to the sum of its scan tile, the counting sort's tile sums:
+// part[cell >> kScanTileShift] of the copy that part points to.  The lanes with the first
+// remaining lane's tile add their number with one atomic, until none is left: the keys of a
+// wavefront are nearly in cell order, so that is mostly one atomic per wavefront.  Every
+// active lane must call this together (the loop is uniform across the wavefront).
+__device__ __forceinline__ void add_to_tile_sums(uint32_t cell, uint32_t* __restrict__ part) {
+    const uint32_t tile = cell >> kScanTileShift;
+    bool pending = true;
+    for (uint64_t left = __ballot(1); left != 0; left = __ballot(pending)) {
+        const int lead = __ffsll((unsigned long long)left) - 1;
+        const uint32_t t = (uint32_t)__builtin_amdgcn_readlane((int)tile, lead);
+        const bool mine = pending && tile == t;
+        const uint64_t same = __ballot(mine);
+        if ((int)(threadIdx.x & 63) == lead) atomicAdd(&part[t], (uint32_t)__popcll(same));
+        pending = pending && !mine;
+    }
+}
+
 // VOXEL formulation: whether a finite position lies outside the voxel grid, i.e. its key
 // (truncated voxel index per axis, voxel_kernels.py:9-17) leaves [0, dims) on some axis and
 // so wraps onto the key of another voxel or off the table.  Positions in (-voxel, 0) truncate

@@ -14,6 +14,10 @@
 //                         (NAIVE: in the low start_bits bits, with the x-slab offsets of the
 //                         cell above them, see StepConsts::slab_shift)
 //   cell_cnt   u32[C]     NAIVE: per-cell counts of the counting sort (kept zero between steps)
+//   cell_part  u32[2][K][T]  NAIVE: sums of the counts over each scan tile of 2^kScanTileShift
+//                         cells, in K = kTileSumCopies copies; the scan reads one array while
+//                         it clears the other, into which a counting force pass adds the next
+//                         step's keys
 //   rng        u64[2N]    xoroshiro128+ stream per particle id (PIPE mode)
 //   cotx/cotv  float4[N]  reverse mode only: cotangent of the state in particle order
 //   nl/nn      i16[96][N], u32[N]  NAIVE neighbour lists of the density pass, read by the
@@ -123,6 +127,16 @@ struct StepConsts {
 
 constexpr int kSlabKeyShift = 30;
 constexpr uint32_t kCellKeyMask = (1u << kSlabKeyShift) - 1u;
+// The counting sort's scan works in tiles of 2^kScanTileShift cells.  A tile's sum is kept
+// in kTileSumCopies places (cell_part: copy k of tile t at k * tiles + t, the sum is over the
+// copies), so that the force pass's wavefronts, which add into copy (wavefront index mod
+// kTileSumCopies), do not all queue on one address.
+constexpr int kScanTileShift = 12;
+#ifndef SPH_TILE_COPIES
+#define SPH_TILE_COPIES 4
+#endif
+constexpr int kTileSumCopies = SPH_TILE_COPIES;
+static_assert((kTileSumCopies & (kTileSumCopies - 1)) == 0 && kTileSumCopies <= 64, "a power of two up to 64");
 
 struct PipeTable {
     float row[kMaxPipeRows][5];   // Pipe.to_numpy() rows
@@ -153,7 +167,8 @@ struct Buffers {
     // position cotangent so far (.w: the particle's dL/dmu term), velocity cotangent of the
     // integration and the total rho cotangent
     uint32_t* cell_cnt;    // NAIVE counting sort: particles per cell (zero between steps)
-    uint32_t* cell_part;   // NAIVE counting sort: per-tile sums of the cell-count scan
+    uint32_t* cell_part;   // NAIVE counting sort: two arrays of per-tile sums of the cell counts,
+                           // the one the scan reads and the one the force pass adds into
     float4 *cotx, *cotv;
     float4 *adj_fb, *adj_gx, *adj_gv;
     float* adj_rt;
@@ -185,6 +200,8 @@ struct sph_ctx {
     int adjoint;             // SPH_FLAG_ADJOINT: reverse-mode buffers are part of the workspace
     int cell_sort;           // NAIVE formulation: counting sort by cell instead of the radix sort
     int whole_cells;         // sph_set_slabs(ctx, 0): cells are not cut into x slabs
+    int part_phase;          // which cell_part array a force pass adds the tile sums into (the
+                             // other was the last sort's), set by launch_cell_sort
     bool adj_seeded;
     size_t ws_bytes;
     void* ws;
@@ -216,13 +233,17 @@ bool cell_sort_global(const sph_ctx* c);
 // whether the force pass can count the keys it writes for the next step's counting sort:
 // a plain (not batched) NAIVE step on the global sort path
 bool count_in_force(const sph_ctx* c);
+// whether such a force pass also adds the keys into the scan's tile sums (one lane per
+// particle)
+bool tile_sums_in_force(const sph_ctx* c);
 int64_t cell_scan_tiles(int64_t cells);
 hipError_t launch_cell_table(sph_ctx* c, hipStream_t s);
 hipError_t launch_reorder(sph_ctx* c, hipStream_t s);
 // keyI of the resident state (posB slots) again, under the context's current cell layout
 hipError_t launch_rekey(sph_ctx* c, hipStream_t s);
 hipError_t launch_density(sph_ctx* c, hipStream_t s);
-// count: also rank the new keys into cell_cnt / keyS[0] for the next sort (count_in_force)
+// count: also rank the new keys into cell_cnt / keyS[0] (count_in_force) and add them to the
+// tile sums in cell_part array part_phase (tile_sums_in_force), for the next sort
 hipError_t launch_force(sph_ctx* c, hipStream_t s, bool count = false);
 hipError_t launch_download_f64(sph_ctx* c, double* pos, double* vel, double* rho, double* force, hipStream_t s);
 hipError_t launch_download_f32(sph_ctx* c, float* x, float* y, float* z, float* vx, float* vy, float* vz,

@@ -372,7 +372,8 @@ __global__ __launch_bounds__(256) void naive_force_kernel(
     uint32_t* __restrict__ pidOut, uint32_t* __restrict__ keyI, uint32_t* __restrict__ valI,
     uint64_t* __restrict__ rng, const PipeTable* __restrict__ pipe, float4* __restrict__ dbgP,
     float4* __restrict__ dbgV, float4* __restrict__ dbgF, uint32_t* __restrict__ flags,
-    float4* __restrict__ sums, uint32_t* __restrict__ cnt, uint32_t* __restrict__ rank) {
+    float4* __restrict__ sums, uint32_t* __restrict__ cnt, uint32_t* __restrict__ rank,
+    uint32_t* __restrict__ part) {
     const uint32_t gt = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int i = (int)(gt / L);
     if (i >= G.n) return;
@@ -392,10 +393,21 @@ __global__ __launch_bounds__(256) void naive_force_kernel(
     // the next sort's count (cell_count_kernel's work).  One lane per particle: the
     // wavefront's particles past n left at the top, so the lanes here are a prefix of it and
     // runs of equal keys share an atomic; with L lanes only the groups' first lanes are
-    // here, at most 64 / L of them, and each counts its own particle.
+    // here, at most 64 / L of them, and each counts its own particle.  With one lane per
+    // particle also the tile sums (cell_scan_reduce's work; tile_sums_in_force), whose
+    // atomics return nothing and go first, so that their latency overlaps the count's.  With
+    // L lanes a wavefront holds 64 / L particles, and L times as many wavefronts would queue
+    // on each tile sum (the 16k adjoint workload's backward at L = 32: 150M -> 112M
+    // particle-updates/s), so the sort adds the tile sums up itself.
     if constexpr (COUNT) {
         const uint32_t cell = key & kCellKeyMask;
-        rank[i] = L == 1 ? rank_in_cell(cell, true, cnt) : atomicAdd(&cnt[cell], 1u);
+        if constexpr (L == 1) {
+            const uint32_t tiles = (uint32_t)(G.cs_cells + (1 << kScanTileShift) - 1) >> kScanTileShift;
+            add_to_tile_sums(cell, part + ((gt >> 6) & (kTileSumCopies - 1)) * tiles);
+            rank[i] = rank_in_cell(cell, true, cnt);
+        } else {
+            rank[i] = atomicAdd(&cnt[cell], 1u);
+        }
     }
 }
 
@@ -831,7 +843,9 @@ void launch_force_lanes(sph_ctx* c, hipStream_t s, float4* dP, float4* dV, float
     naive_force_kernel<MODE, L, COUNT><<<blocks_for(c->n * L, bs), bs, 0, s>>>(
         c->sc, c->b.posA, c->b.velA, c->b.pidA, c->b.rhoA, c->b.cs, c->b.nl, c->b.nn, c->b.posB, c->b.velB,
         c->b.pidB, c->b.keyI, c->b.valI, c->b.rng, c->b.pipe, dP, dV, dF, c->b.flags, c->b.sums,
-        COUNT ? c->b.cell_cnt : nullptr, COUNT ? c->b.keyS[0] : nullptr);
+        COUNT ? c->b.cell_cnt : nullptr, COUNT ? c->b.keyS[0] : nullptr,
+        COUNT && L == 1 ? c->b.cell_part + (size_t)c->part_phase * kTileSumCopies * cell_scan_tiles(c->sc.cs_cells)
+                        : nullptr);
 }
 
 template <int MODE>
@@ -878,6 +892,8 @@ void launch_force_mode(sph_ctx* c, hipStream_t s, bool count) {
 bool count_in_force(const sph_ctx* c) {
     return c->cell_sort && c->inst.empty() && c->sc.formulation == SPH_FORM_NAIVE && cell_sort_global(c);
 }
+
+bool tile_sums_in_force(const sph_ctx* c) { return count_in_force(c) && c->lanes == 1; }
 
 hipError_t launch_force(sph_ctx* c, hipStream_t s, bool count) {
     if (c->n == 0) return hipGetLastError();

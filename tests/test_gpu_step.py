@@ -518,18 +518,21 @@ def _same_after(make, upload_state, steps):
     return a, b
 
 
-@pytest.mark.parametrize("mode,lanes", [("BOX", 1), ("BOX", 4), ("PIPE", 1)])
-def test_steps_in_one_call_equal_single_calls(gpu, mode, lanes):
+@pytest.mark.parametrize("mode,lanes,edge", [("BOX", 1, 2.0 ** -5), ("BOX", 4, 2.0 ** -5), ("PIPE", 1, None),
+                                             ("BOX", 1, 2.0 ** -8)])
+def test_steps_in_one_call_equal_single_calls(gpu, mode, lanes, edge):
     """One sph_step(k) call and k single-step calls give bitwise the same state, keys,
     permutation and cell table.  20,000 particles on 32,768 cells or more (the counting
     sort's multi-workgroup path, which keeps counts in device memory): with one lane per
-    particle the k-step call's force passes count the keys for the next sort and the sorts
-    after the first skip their count; single calls count in every sort; with 4 lanes every
-    sort counts.  Particles outside the grid (BOX) and re-injection (PIPE)."""
+    particle the k-step call's force passes count the keys and add up the scan's tile sums
+    for the next sort, and the sorts after the first start at the scan; single calls count
+    in every sort; with 4 lanes every sort counts.  Particles outside the grid (BOX) and
+    re-injection (PIPE).  2^-8 cells: 256^3 cells in 4,096 scan tiles, more than a tile's
+    workgroup sums itself, so the tile sums are scanned by a workgroup of their own first."""
     if mode == "BOX":
         w = box_uniform(20000, seed=31)
         pos = outside_points(f32(w.state.position), w.params.space_size)
-        grid = 2.0 ** -5
+        grid = edge
     else:
         w = pipe_flow(n=20000, seed=32)
         pos = f32(w.state.position)
