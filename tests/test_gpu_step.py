@@ -561,6 +561,37 @@ def test_steps_in_one_call_equal_single_calls(gpu, mode, lanes, edge):
         assert torch.equal(ta, tb)
 
 
+def test_multi_step_calls_across_lane_and_slab_switches(gpu):
+    """Calls of several steps with the lanes and the slab setting changed between them: with
+    one lane the force passes inside a call also add the next sort's tile sums (the scan
+    then reads the array they went to and clears the other), with four lanes the sort adds
+    them up itself, and a call's first sort always does.  The states equal single-step calls
+    with the same switches, bit for bit, and a call that ends on an odd step leaves nothing
+    behind for the next."""
+    w = box_uniform(20000, seed=35)
+    pos = f32(w.state.position)
+    vel = f32(np.random.default_rng(35).normal(0.0, 0.5, size=pos.shape))
+    plan = [(1, True, 3), (4, True, 4), (1, True, 5), (1, False, 2), (1, True, 3), (2, True, 1), (1, True, 4)]
+
+    def make():
+        s = NaiveSPHStrategy(w.params, constants=w.constants, device=gpu, grid_voxel=2.0 ** -6)
+        upload(s, pos, vel)
+        return s
+
+    a, b = make(), make()
+    for lanes, slabs, k in plan:
+        for s in (a, b):
+            s.ctx.set_lanes(lanes)
+            s.ctx.set_slabs(slabs)
+        a.ctx.step(k)
+        for _ in range(k):
+            b.ctx.step(1)
+        for ta, tb in zip(a.ctx.download(), b.ctx.download()):
+            assert torch.equal(ta, tb), (lanes, slabs, k)
+    for which in (N.BUF_KEYS, N.BUF_PERM, N.BUF_SORTED_KEYS, N.BUF_CELL_BEGIN, N.BUF_DENSITY):
+        assert torch.equal(a.ctx.read_buffer(which), b.ctx.read_buffer(which)), which
+
+
 def test_batch_steps_in_one_call_equal_single_calls(gpu):
     """The batched form (per-instance cell offsets in one counting sort): one sph_step(6) call
     equals six single-step calls bitwise, instances with 1 and 4 lanes."""
