@@ -425,8 +425,7 @@ def test_graph_replays_match_eager(gpu, mode, n):
     """A captured run of steps, replayed several times with eager work and synchronisation
     in between, advances the state exactly as eager steps do (PIPE: RNG streams included).
     3,000 particles sort in one workgroup; 20,000 take the multi-workgroup counting sort,
-    whose counts and scan-tile sums the captured force passes make for the next step's sort
-    (5 steps: the replay ends with the tile sums in the other array than it began with)."""
+    whose counts and scan-tile sums the captured force passes make for the next step's sort."""
     w = box_uniform(n, seed=23) if mode == "BOX" else pipe_flow(n=n, seed=24)
     pos, vel = f32(w.state.position), f32(w.state.velocity)
     eager = NaiveSPHStrategy(w.params, constants=w.constants, mode=mode, device=gpu)
