@@ -106,7 +106,7 @@ __device__ __forceinline__ uint32_t block_excl(uint32_t v, uint32_t* lw, uint32_
 
 constexpr int CV = CI / 4;  // 16-byte vectors per thread and tile
 
-// Tile sums into part copy 0 (the other copies of the tile cleared).
+// Tile sums into part copy 0 (the scan that follows reads that copy only).
 __global__ __launch_bounds__(CT) void cell_scan_reduce(const uint32_t* __restrict__ cnt, int m,
                                                         uint32_t* __restrict__ part) {
     __shared__ uint32_t lw[CWAVES];
@@ -127,19 +127,19 @@ __global__ __launch_bounds__(CT) void cell_scan_reduce(const uint32_t* __restric
     }
     uint32_t tot;
     block_excl(s, lw, &tot);
-    if (threadIdx.x < kTileSumCopies) part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = threadIdx.x == 0 ? tot : 0u;
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
-// One workgroup: exclusive scan of the tile sums (over their copies) into copy 0, in chunks
-// of CT with a carry.
-__global__ __launch_bounds__(CT) void cell_scan_partials(uint32_t* __restrict__ part, int nparts) {
+// One workgroup: exclusive scan of the tile sums (over their first `copies` copies) into
+// copy 0, in chunks of CT with a carry.
+__global__ __launch_bounds__(CT) void cell_scan_partials(uint32_t* __restrict__ part, int nparts, int copies) {
     __shared__ uint32_t lw[CWAVES];
     uint32_t carry = 0;
     for (int base = 0; base < nparts; base += CT) {
         const int i = base + threadIdx.x;
         uint32_t v = 0u;
         if (i < nparts)
-            for (int k = 0; k < kTileSumCopies; ++k) v += part[(size_t)k * nparts + i];
+            for (int k = 0; k < copies; ++k) v += part[(size_t)k * nparts + i];
         uint32_t tot;
         const uint32_t ex = block_excl(v, lw, &tot);
         if (i < nparts) part[i] = carry + ex;
@@ -147,14 +147,15 @@ __global__ __launch_bounds__(CT) void cell_scan_partials(uint32_t* __restrict__ 
     }
 }
 
-// Sum of the tile sums before tile b, over their copies (the tile's offset when the tile
-// sums were not scanned first).
-__device__ __forceinline__ uint32_t tiles_before(const uint32_t* __restrict__ part, int b, int tiles, uint32_t* lw) {
+// Sum of the tile sums before tile b, over their first `copies` copies (the tile's offset
+// when the tile sums were not scanned first).
+__device__ __forceinline__ uint32_t tiles_before(const uint32_t* __restrict__ part, int b, int tiles, int copies,
+                                                 uint32_t* lw) {
     uint32_t s = 0;
     for (int j = threadIdx.x; j < b; j += CT) {  // the copies of a tile in flight together
         uint32_t v[kTileSumCopies];
 #pragma unroll
-        for (int k = 0; k < kTileSumCopies; ++k) v[k] = part[(size_t)k * tiles + j];
+        for (int k = 0; k < kTileSumCopies; ++k) v[k] = k < copies ? part[(size_t)k * tiles + j] : 0u;
 #pragma unroll
         for (int k = 0; k < kTileSumCopies; ++k) s += v[k];
     }
@@ -166,21 +167,23 @@ __device__ __forceinline__ uint32_t tiles_before(const uint32_t* __restrict__ pa
 // The tile is read (and cnt cleared) with coalesced 16-byte accesses into LDS, each thread
 // scans CI consecutive cells there, and cs is written back coalesced.  LDS index i is padded
 // by i / 32 so that the thread-contiguous pass does not hit one bank.  The last tile also
-// writes cs[m] = n.  MERGED: part holds tile sums (the offset is summed here); otherwise
-// copy 0 of part holds the tiles' exclusive prefix sums.  Each workgroup also clears its
-// tile's copies in next, the tile sums that the next force pass adds into.
+// writes cs[m] = n.  MERGED: the first `copies` copies of part hold tile sums (all of them
+// when a force pass added them up, copy 0 when cell_scan_reduce did; the offset is summed
+// here); otherwise copy 0 of part holds the tiles' exclusive prefix sums.  Each workgroup also
+// clears its tile's copies in next, the tile sums that the next force pass adds into.
 
 __device__ __forceinline__ int cpad(int i) { return i + (i >> 5); }
 
 template <bool MERGED>
 __global__ __launch_bounds__(CT) void cell_scan_down(uint32_t* __restrict__ cnt, int m,
                                                       const uint32_t* __restrict__ part, uint32_t* __restrict__ next,
-                                                      uint32_t* __restrict__ cs, uint32_t n, uint32_t empty) {
+                                                      uint32_t* __restrict__ cs, uint32_t n, uint32_t empty,
+                                                      int copies) {
     __shared__ uint32_t lw[CWAVES];
     __shared__ uint32_t tile[CTILE + CTILE / 32];
     const int64_t base = (int64_t)blockIdx.x * CTILE;
     if (threadIdx.x < kTileSumCopies) next[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = 0u;
-    const uint32_t offset = MERGED ? tiles_before(part, blockIdx.x, gridDim.x, lw) : part[blockIdx.x];
+    const uint32_t offset = MERGED ? tiles_before(part, blockIdx.x, gridDim.x, copies, lw) : part[blockIdx.x];
     const bool full = base + CTILE <= m;  // base is a multiple of 4 cells: 16-byte aligned
     if (full) {
         uint4* c4 = reinterpret_cast<uint4*>(cnt + base);
@@ -390,11 +393,12 @@ hipError_t launch_cell_sort(sph_ctx* c, bool gather, hipStream_t s, bool counted
     c->part_phase = cur ^ 1;
     if (n > 0 && !counted) cell_count_kernel<<<cblocks(n), CT, 0, s>>>(n, b.keyI, b.cell_cnt, rank);
     if (!summed) cell_scan_reduce<<<tiles, CT, 0, s>>>(b.cell_cnt, m, part);
+    const int copies = summed ? kTileSumCopies : 1;
     if (tiles <= kMergedTiles) {
-        cell_scan_down<true><<<tiles, CT, 0, s>>>(b.cell_cnt, m, part, next, b.cs, (uint32_t)n, T.empty());
+        cell_scan_down<true><<<tiles, CT, 0, s>>>(b.cell_cnt, m, part, next, b.cs, (uint32_t)n, T.empty(), copies);
     } else {
-        cell_scan_partials<<<1, CT, 0, s>>>(part, tiles);
-        cell_scan_down<false><<<tiles, CT, 0, s>>>(b.cell_cnt, m, part, next, b.cs, (uint32_t)n, T.empty());
+        cell_scan_partials<<<1, CT, 0, s>>>(part, tiles, copies);
+        cell_scan_down<false><<<tiles, CT, 0, s>>>(b.cell_cnt, m, part, next, b.cs, (uint32_t)n, T.empty(), copies);
     }
     if (n > 0) {
         cell_scatter_kernel<<<cblocks(n), CT, 0, s>>>(n, b.keyI, rank, b.cs, tmp, T.start_mask);
