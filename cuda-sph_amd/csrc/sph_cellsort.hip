@@ -12,11 +12,12 @@
 //            step's force pass has done this already as it wrote the keys (rank_in_cell), and
 //            the sort starts at the scan
 //   scan     exclusive prefix sum of cnt over all cells: tile sums (16-byte loads; a force
-//            pass that counted has added them up already), then each tile's offset is the
-//            sum of the tile sums before it (added up by the tile's own workgroup when there
-//            are few tiles, else by a one-workgroup scan first), then the tile is scanned;
-//            the result is the cell start table cs[0..C] itself, and the tile scan resets
-//            cnt, and the other array of tile sums, to zero for the next step
+//            pass that counted with one lane per particle has added them up already, into
+//            kTileSumCopies copies per tile), then each tile's offset is the sum of the tile
+//            sums before it (added up by the tile's own workgroup when there are few tiles,
+//            else by a one-workgroup scan first), then the tile is scanned; the result is
+//            the cell start table cs[0..C] itself, and the tile scan resets cnt, and the
+//            other array of tile sums, to zero for the next step
 //   scatter  one thread per slot: tmp[cs[key] + rank] = slot
 //   order    one thread per position: the particle there moves to cs[key] + (number of
 //            particles of its cell with a smaller id), which is id order within the cell; the
